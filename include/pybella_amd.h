@@ -1,0 +1,164 @@
+/*
+ * pybella_amd — C ABI of the MI355X (gfx950) nodal elliptic pressure solve.
+ *
+ * Every array argument is a DEVICE pointer owned by the caller, float64 unless
+ * stated, row-major with x fastest, i.e. the transpose of pyBELLA's
+ * ``arr[x, y]`` numpy layout.  Shapes for a grid of nx * ny inner nodes
+ * (nx = ud.inx, ny = ud.iny):
+ *   cells  (ny+3) x (nx+3)    Sol.rho ... Sol.rhoX, mpv.wplus[d], mpv.u/v/w,
+ *                             dSdy, nu_c        (elem.sc, two ghost layers)
+ *   nodes  (ny+4) x (nx+4)    mpv.p2_nodes      (node.sc)
+ *   isc    (ny+2) x (nx+2)    mpv.rhs, mpv.wcenter, diag_inv (node.isc)
+ *   vector  ny    x  nx       the Krylov vector p2_nodes[2:-2,2:-2].T.ravel()
+ * Calls are asynchronous on the context's stream unless stated; no call
+ * allocates device memory except *_create and the first solve on a handle.
+ * Return value: 0 ok, < 0 error (pb_last_error gives the message),
+ * > 0 only where documented.
+ *
+ * Reference interfaces replaced (pyBELLA tree, file:line):
+ *   pb_bc_cells            set_explicit_boundary_data   src/dycore/utils/boundary.py:7
+ *   pb_ghostnodes          set_ghostnodes_p2            src/dycore/utils/boundary.py:296
+ *   pb_op2d_assemble       operator_coefficients_nodes  second_projection.py:308
+ *                          + multiply_inverse_coriolis(get_coeffs=True) :604
+ *                          + precon_diag_prepare        laplacian.py:1284
+ *                          + stencil_9pt_numba_test     laplacian.py:496
+ *   pb_op2d_matvec         LinearOperator.matvec -> lap2D_gather_new laplacian.py:546
+ *   pb_divergence_nodes    divergence_nodes             second_projection.py:525
+ *   pb_correction_nodes    correction_nodes             second_projection.py:280
+ *   pb_scatter_update_p2   p2_full scatter + p2_nodes += second_projection.py:241-277
+ *   pb_bicgstab_solve      scipy.sparse.linalg.bicgstab call second_projection.py:230
+ *   pb_op3d_*              stencil_27pt / lap3D         laplacian.py:872-1026
+ */
+#ifndef PYBELLA_AMD_H
+#define PYBELLA_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PB_OK 0
+#define PB_ERR_ARG (-1)
+#define PB_ERR_HIP (-2)
+#define PB_ERR_STATE (-3)
+
+#define PB_BC_PERIODIC 0
+#define PB_BC_WALL 1
+
+/* precision selector for matvec / solver vectors */
+#define PB_F64 0
+#define PB_F32 1
+
+/* solver modes */
+#define PB_SOLVE_F64 0      /* FP64 BiCGStab, reference semantics            */
+#define PB_SOLVE_MIXED 1    /* FP32 BiCGStab inside FP64 iterative refinement */
+#define PB_SOLVE_NN 2       /* learned preconditioner, flexible outer Krylov  */
+
+typedef struct pb_ctx pb_ctx;
+typedef struct pb_op2d pb_op2d;
+typedef struct pb_op3d pb_op3d;
+
+typedef struct pb_grid2d {
+  int nx, ny;          /* inner node counts (ud.inx, ud.iny)            */
+  double dx, dy;       /* node spacings (node.dx, node.dy)              */
+  int bc_x, bc_y;      /* PB_BC_PERIODIC or PB_BC_WALL (RAYLEIGH->WALL) */
+} pb_grid2d;
+
+typedef struct pb_phys {
+  double gamm;             /* ud.gamm                                    */
+  double Msq;              /* ud.Msq                                     */
+  double g;                /* ud.gravity_strength[1]                     */
+  double coriolis[3];      /* ud.coriolis_strength                       */
+  double nonhydro;         /* ud.nonhydrostasy                           */
+  int is_compressible;     /* ud.is_compressible                         */
+  double compressibility;  /* ud.compressibility                         */
+  int is_arakawa_konor;    /* ud.is_ArakawaKonor                         */
+} pb_phys;
+
+typedef struct pb_solve_opts {
+  int mode;            /* PB_SOLVE_*                                          */
+  double tol;          /* ud.tol (legacy semantics in PB_SOLVE_F64)           */
+  int maxiter;         /* ud.max_iterations (per Krylov run)                  */
+  double inner_tol;    /* PB_SOLVE_MIXED / PB_SOLVE_NN: inner relative tol    */
+  int max_outer;       /* PB_SOLVE_MIXED / PB_SOLVE_NN: refinement cap        */
+  int check_every;     /* iterations between host convergence checks (0=auto) */
+  int x0_given;        /* 0: x0 = 0 as the reference; 1: use x as the guess   */
+} pb_solve_opts;
+
+typedef struct pb_solve_info {
+  int info;            /* 0 converged, >0 not converged (iterations), <0 breakdown */
+  int iters;           /* total Krylov iterations (callback count)                 */
+  int outer;           /* refinement steps taken (mixed / NN)                      */
+  int half_step;       /* 1 if the last run stopped at the early ||s|| test        */
+  double resid;        /* last recursive residual norm                             */
+  double bnrm2;        /* ||b||                                                    */
+  double true_rel_resid; /* ||b - A x|| / ||b|| recomputed in FP64                 */
+} pb_solve_info;
+
+int pb_version(void);
+int pb_ctx_create(int device, pb_ctx** out);
+int pb_ctx_destroy(pb_ctx* ctx);
+/* Issue all later work on hip_stream (NULL = the null stream).  A new context
+ * uses a non-blocking stream of its own until this is called. */
+int pb_ctx_set_stream(pb_ctx* ctx, void* hip_stream);
+int pb_ctx_synchronize(pb_ctx* ctx);
+const char* pb_last_error(const pb_ctx* ctx);
+
+/* cell ghost fill: wrap / mirror in x, hydrostatic images in y (gravity axis).
+ * fields[6] = rho, rhou, rhov, rhow, rhoY, rhoX (cells); rhoY0_bg = the cell
+ * background column mpv.HydroState.rhoY0 (ny+3), used when !is_compressible. */
+int pb_bc_cells(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys,
+                double* const fields[6], const double* rhoY0_bg);
+
+/* node ghost fill of a (ny+4)x(nx+4) node array (periodic_plus_one / reflect) */
+int pb_ghostnodes(pb_ctx* ctx, const pb_grid2d* grid, double* p);
+
+int pb_op2d_create(pb_ctx* ctx, const pb_grid2d* grid, pb_op2d** out);
+int pb_op2d_destroy(pb_op2d* op);
+
+/* Assemble the operator on the device.  rho_w/rhoY_w: state for wplus and
+ * wcenter (Sol0 when given, else Sol); rho_c/rhoY_c: state for the Coriolis
+ * factors (Sol).  dSdy: cells.  Outputs (any may be NULL): wplus0, wplus1,
+ * nu_c (cells), wcenter, diag_inv (isc).  wcenter is scaled by the regime
+ * (x compressibility for pseudo-incompressible) before diag_inv is formed, as
+ * second_projection.py:141-153 does before precon_diag_prepare. */
+int pb_op2d_assemble(pb_op2d* op, const pb_phys* phys, double dt,
+                     const double* rho_w, const double* rhoY_w,
+                     const double* rho_c, const double* rhoY_c, const double* dSdy,
+                     double* wplus0, double* wplus1, double* nu_c,
+                     double* wcenter, double* diag_inv);
+
+/* y = diag_inv * (A x); precision PB_F64 (double*) or PB_F32 (float*) */
+int pb_op2d_matvec(pb_op2d* op, const void* x, void* y, int precision);
+
+/* b_vec = diag_inv[1:-1,1:-1] * rhs[1:-1,1:-1] as a Krylov vector (rhs: isc) */
+int pb_op2d_scale_rhs(pb_op2d* op, const double* rhs_isc, double* b_vec);
+
+/* Solve (D^-1 A) x = b.  Synchronous (returns after the solve).  Returns the
+ * info code (0 converged, >0 iterations when not converged, <0 breakdown) or
+ * a PB_ERR_* code. */
+int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x,
+                      const pb_solve_opts* opts, pb_solve_info* info);
+
+/* divergence of (rhou, rhov)*Y at nodes -> rhs (isc), divided by dt.
+ * Zeroes rhou/rhov/rhow in the two cell rows next to y walls first. */
+int pb_divergence_nodes(pb_ctx* ctx, const pb_grid2d* grid, double dt,
+                        double* rho, double* rhou, double* rhov, double* rhow,
+                        const double* rhoY, double* rhs);
+
+/* momentum correction from node pressure p (nodes): rhou, rhov += ...,
+ * rhoX -= updt_chi dt dSdy rhov; mpv u, v, w (cells) receive the corrections. */
+int pb_correction_nodes(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys,
+                        double dt, double updt_chi, const double* p,
+                        const double* rho, double* rhou, double* rhov,
+                        const double* rhoY, double* rhoX, const double* dSdy,
+                        double* u, double* v, double* w);
+
+/* p2_full = scatter(x) with node ghosts; p2_nodes += p2_full; ghost fill.
+ * p2_full (nodes) receives the increment. */
+int pb_scatter_update_p2(pb_ctx* ctx, const pb_grid2d* grid, const double* x,
+                         double* p2_full, double* p2_nodes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYBELLA_AMD_H */

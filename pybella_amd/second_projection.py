@@ -1,0 +1,289 @@
+"""Drop-in GPU replacement of pyBELLA's nodal pressure solve (second projection).
+
+``euler_backward_non_advective_impl_part`` keeps the reference signature
+(src/dycore/physics/low_mach/second_projection.py:97) and its in-place side
+effects on ``Sol`` (rhou, rhov, rhoX, ghost cells, wall-row momentum zeroing),
+``Sol0`` (ghost cells) and ``mpv`` (wplus, wcenter, nu_c, rhs = div/dt, u, v,
+w, p2_nodes += dp2 with ghost nodes).  Every arithmetic step runs in HIP
+kernels on the GPU (see include/pybella_amd.h); the host only moves arrays in
+and out.  There is no CPU path: without the built library or a GPU this raises.
+
+The reference's 3D / slice branches (second_projection.py:183-203) reference
+``diag_inv`` before assignment and cannot run; this module implements the 2D
+vertical slice (gravity on y), which is the reference's runnable path.
+
+Counters ``total_iter`` / ``total_calls`` are kept as module globals like the
+reference (:95-96, :235-239).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import logging
+
+import numpy as np
+
+from . import _abi
+from .state import is_periodic, is_wall
+
+total_iter = 0
+total_calls = 0
+
+# Solver settings used by the drop-in.  mode: "f64" (reference semantics:
+# FP64 BiCGStab, x0 = 0, legacy tolerance), "mixed" (FP32 Krylov in FP64
+# iterative refinement, stop on the FP64 true residual).
+solver_options = {"mode": "f64", "inner_tol": 1e-5, "max_outer": 12, "check_every": 32}
+
+_MODES = {"f64": _abi.PB_SOLVE_F64, "mixed": _abi.PB_SOLVE_MIXED, "nn": _abi.PB_SOLVE_NN}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def to_dev(a):
+    """Reference-layout [x, y] float64 numpy array -> device [y][x] tensor."""
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)).to("cuda")
+
+
+def from_dev(t, out=None):
+    """Device [y][x] tensor -> reference-layout numpy array (written into ``out``)."""
+    a = t.detach().cpu().numpy().T
+    if out is None:
+        return np.ascontiguousarray(a)
+    out[...] = a
+    return out
+
+
+def grid_of(node, ud) -> _abi.Grid2D:
+    if node.ndim != 2:
+        raise NotImplementedError("the 2D vertical-slice path needs ndim == 2")
+    for d in (0, 1):
+        bt = ud.bdry_type[d]
+        if not (is_periodic(bt) or is_wall(bt)):
+            raise ValueError(f"unsupported boundary type {bt}")
+    return _abi.Grid2D(node.iicx, node.iicy, float(node.dx), float(node.dy),
+                       _abi.PB_BC_WALL if is_wall(ud.bdry_type[0]) else _abi.PB_BC_PERIODIC,
+                       _abi.PB_BC_WALL if is_wall(ud.bdry_type[1]) else _abi.PB_BC_PERIODIC)
+
+
+def phys_of(ud, th) -> _abi.Phys:
+    cor = (C.c_double * 3)(*[float(v) for v in ud.coriolis_strength])
+    return _abi.Phys(float(th.gamm), float(ud.Msq), float(ud.gravity_strength[1]), cor,
+                     float(ud.nonhydrostasy), int(ud.is_compressible), float(ud.compressibility),
+                     int(ud.is_ArakawaKonor))
+
+
+def dSdy_cells(mpv, elem, node):
+    """dS/dy on cells from the node background (variable.py:162-177)."""
+    hs = mpv.HydroState_n
+    if hasattr(hs, "get_dSdy"):
+        return hs.get_dSdy(elem, node)
+    d = (hs.S0[1:] - hs.S0[:-1]) / node.dy
+    return np.repeat(d[None, :], elem.sc[0], axis=0)
+
+
+class Operator2D:
+    """A device operator handle (pb_op2d) with its solver workspace."""
+
+    def __init__(self, grid: _abi.Grid2D, ctx: _abi.Context | None = None):
+        self.ctx = ctx or _abi.default_context()
+        self.grid = grid
+        self.nx, self.ny = grid.nx, grid.ny
+        h = _abi.VP()
+        self.ctx.check(self.ctx.lib.pb_op2d_create(self.ctx.h, C.byref(grid), C.byref(h)), "pb_op2d_create")
+        self.h = h
+
+    def assemble(self, phys, dt, rho_w, rhoY_w, rho_c, rhoY_c, dSdy, wplus0=None, wplus1=None,
+                 nu_c=None, wcenter=None, diag_inv=None):
+        P = _abi.ptr
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.lib.pb_op2d_assemble(
+            self.h, C.byref(phys), float(dt), P(rho_w), P(rhoY_w), P(rho_c), P(rhoY_c), P(dSdy),
+            P(wplus0), P(wplus1), P(nu_c), P(wcenter), P(diag_inv)), "pb_op2d_assemble")
+
+    def matvec(self, x, y=None):
+        torch = _torch()
+        if y is None:
+            y = torch.empty_like(x)
+        prec = _abi.PB_F64 if x.dtype == torch.float64 else _abi.PB_F32
+        assert x.numel() == self.nx * self.ny and y.dtype == x.dtype
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.lib.pb_op2d_matvec(self.h, _abi.ptr(x), _abi.ptr(y), prec), "pb_op2d_matvec")
+        return y
+
+    def scale_rhs(self, rhs_isc, b=None):
+        torch = _torch()
+        if b is None:
+            b = torch.empty(self.ny, self.nx, dtype=torch.float64, device="cuda")
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.lib.pb_op2d_scale_rhs(self.h, _abi.ptr(rhs_isc), _abi.ptr(b)), "pb_op2d_scale_rhs")
+        return b
+
+    def solve(self, b, x=None, tol=1e-8, maxiter=6000, mode="f64", inner_tol=1e-5, max_outer=12,
+              check_every=32, x0=None):
+        torch = _torch()
+        if x is None:
+            x = torch.empty(self.ny, self.nx, dtype=torch.float64, device="cuda")
+        given = 0
+        if x0 is not None:
+            x.copy_(x0)
+            given = 1
+        opts = _abi.SolveOpts(_MODES[mode], float(tol), int(maxiter), float(inner_tol), int(max_outer),
+                              int(check_every), given)
+        info = _abi.SolveInfo()
+        self.ctx.sync_stream()
+        rc = self.ctx.lib.pb_bicgstab_solve(self.h, _abi.ptr(b), _abi.ptr(x), C.byref(opts), C.byref(info))
+        self.ctx.check(rc, "pb_bicgstab_solve")
+        return x, info.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.pb_op2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_op_cache: dict = {}
+
+
+def operator_for(node, ud) -> Operator2D:
+    g = grid_of(node, ud)
+    key = (g.nx, g.ny, g.dx, g.dy, g.bc_x, g.bc_y)
+    op = _op_cache.get(key)
+    if op is None:
+        op = Operator2D(g)
+        _op_cache[key] = op
+    return op
+
+
+_CELL_FIELDS = ("rho", "rhou", "rhov", "rhow", "rhoY", "rhoX")
+
+
+class _DevState:
+    """Device copies of a Vars object's six cell fields."""
+
+    def __init__(self, Sol):
+        self.f = {k: to_dev(getattr(Sol, k)) for k in _CELL_FIELDS}
+
+    def ptrs(self):
+        return (_abi.VP * 6)(*[_abi.VP(self.f[k].data_ptr()) for k in _CELL_FIELDS])
+
+    def download(self, Sol):
+        for k in _CELL_FIELDS:
+            from_dev(self.f[k], getattr(Sol, k))
+
+
+def _bc_cells(ctx, grid, phys, ds: _DevState, rhoY0):
+    ctx.check(ctx.lib.pb_bc_cells(ctx.h, C.byref(grid), C.byref(phys), ds.ptrs(), _abi.ptr(rhoY0)),
+              "pb_bc_cells")
+
+
+def euler_backward_non_advective_impl_part(Sol, mpv, elem, node, ud, th, t, dt, alpha_diff, Sol0=None,
+                                           writer=None, label=None, debug=False):
+    """GPU second projection with the reference's signature and side effects."""
+    global total_iter, total_calls
+    torch = _torch()
+    if ud.is_compressible == 0 and ud.is_ArakawaKonor:
+        raise NotImplementedError("Arakawa-Konor branch (second_projection.py:145-147) is not runnable "
+                                  "in the reference (shape mismatch of wcenter and dp2_nodes)")
+    if hasattr(ud, "LAMB_BDRY"):
+        raise NotImplementedError("LAMB_BDRY operator variant is not part of this path")
+    grid = grid_of(node, ud)
+    phys = phys_of(ud, th)
+    op = operator_for(node, ud)
+    ctx = op.ctx
+    ctx.sync_stream()
+    nx, ny = grid.nx, grid.ny
+    dS = to_dev(dSdy_cells(mpv, elem, node))
+    rhoY0 = torch.from_numpy(np.ascontiguousarray(mpv.HydroState.rhoY0, dtype=np.float64)).to("cuda")
+    sol = _DevState(Sol)
+    sol0 = _DevState(Sol0) if Sol0 is not None else None
+    S = sol0 if sol0 is not None else sol
+    p2n = to_dev(mpv.p2_nodes)
+    cells = lambda: torch.empty(ny + 3, nx + 3, dtype=torch.float64, device="cuda")
+    u, v, w = cells(), cells(), cells()
+    F = sol.f
+    # :113-118 ghost cells + coefficients' state
+    _bc_cells(ctx, grid, phys, S, rhoY0)
+    # :130-132 correction with the old p2, then ghost cells
+    ctx.check(ctx.lib.pb_ghostnodes(ctx.h, C.byref(grid), _abi.ptr(p2n)), "pb_ghostnodes")
+    ctx.check(ctx.lib.pb_correction_nodes(ctx.h, C.byref(grid), C.byref(phys), float(dt), 0.0, _abi.ptr(p2n),
+                                          _abi.ptr(F["rho"]), _abi.ptr(F["rhou"]), _abi.ptr(F["rhov"]),
+                                          _abi.ptr(F["rhoY"]), _abi.ptr(F["rhoX"]), _abi.ptr(dS),
+                                          _abi.ptr(u), _abi.ptr(v), _abi.ptr(w)), "pb_correction_nodes")
+    _bc_cells(ctx, grid, phys, sol, rhoY0)
+    # :134-139 divergence / dt
+    rhs = torch.empty(ny + 2, nx + 2, dtype=torch.float64, device="cuda")
+    ctx.check(ctx.lib.pb_divergence_nodes(ctx.h, C.byref(grid), float(dt), _abi.ptr(F["rho"]),
+                                          _abi.ptr(F["rhou"]), _abi.ptr(F["rhov"]), _abi.ptr(F["rhow"]),
+                                          _abi.ptr(F["rhoY"]), _abi.ptr(rhs)), "pb_divergence_nodes")
+    c = float(ud.compressibility)
+    if ud.is_compressible == 0 and c not in (0.0, 1.0):
+        rhs = c * rhs + (1.0 - c) * rhs          # :150, exact elementwise blend
+    # :115/:118, :141-153, :165-175 operator assembly (wcenter regime-scaled)
+    wplus0, wplus1, nu_c = cells(), cells(), cells()
+    wcenter = torch.empty(ny + 2, nx + 2, dtype=torch.float64, device="cuda")
+    diag_inv = torch.empty_like(wcenter)
+    op.assemble(phys, dt, S.f["rho"], S.f["rhoY"], F["rho"], F["rhoY"], dS, wplus0, wplus1, nu_c,
+                wcenter, diag_inv)
+    # :170, :214 scaled right-hand side; :230 solve
+    b = op.scale_rhs(rhs)
+    mode = solver_options.get("mode", "f64")
+    x, info = op.solve(b, tol=float(ud.tol), maxiter=int(ud.max_iterations), mode=mode,
+                       inner_tol=solver_options.get("inner_tol", 1e-5),
+                       max_outer=solver_options.get("max_outer", 12),
+                       check_every=solver_options.get("check_every", 32))
+    total_iter += info["iters"]
+    total_calls += 1
+    logging.info(info["iters"])
+    logging.info("Total calls to BiCGStab routine = %i, total iterations = %i" % (total_calls, total_iter))
+    # :241-277 scatter, correction with dp2, p2 update, ghost cells
+    p2_full = torch.empty(ny + 4, nx + 4, dtype=torch.float64, device="cuda")
+    ctx.check(ctx.lib.pb_scatter_update_p2(ctx.h, C.byref(grid), _abi.ptr(x), _abi.ptr(p2_full),
+                                           _abi.ptr(p2n)), "pb_scatter_update_p2")
+    ctx.check(ctx.lib.pb_correction_nodes(ctx.h, C.byref(grid), C.byref(phys), float(dt), 1.0,
+                                          _abi.ptr(p2_full), _abi.ptr(F["rho"]), _abi.ptr(F["rhou"]),
+                                          _abi.ptr(F["rhov"]), _abi.ptr(F["rhoY"]), _abi.ptr(F["rhoX"]),
+                                          _abi.ptr(dS), _abi.ptr(u), _abi.ptr(v), _abi.ptr(w)),
+              "pb_correction_nodes")
+    _bc_cells(ctx, grid, phys, sol, rhoY0)
+    torch.cuda.synchronize()
+    # write back the side effects
+    sol.download(Sol)
+    if sol0 is not None:
+        sol0.download(Sol0)
+    from_dev(p2n, mpv.p2_nodes)
+    mpv.rhs[...] = from_dev(rhs)
+    mpv.wplus[0][...] = from_dev(wplus0)
+    mpv.wplus[1][...] = from_dev(wplus1)
+    mpv.wcenter = from_dev(wcenter)
+    mpv.nu_c = from_dev(nu_c)
+    mpv.u, mpv.v, mpv.w = from_dev(u), from_dev(v), from_dev(w)
+    return info
+
+
+def solve_isolated(Sol, mpv, elem, node, ud, th, dt, rhs_isc, mode="f64", tol=None, maxiter=None,
+                   inner_tol=1e-5, check_every=32):
+    """Operator assembly from ``Sol`` + scaled RHS + solve: the inner part of
+    the projection (second_projection.py:141-230) for an isolated system.
+    Returns (x as a (NY, NX) numpy array, info dict)."""
+    torch = _torch()
+    grid = grid_of(node, ud)
+    phys = phys_of(ud, th)
+    op = operator_for(node, ud)
+    rho, rhoY = to_dev(Sol.rho), to_dev(Sol.rhoY)
+    dS = to_dev(dSdy_cells(mpv, elem, node))
+    op.assemble(phys, dt, rho, rhoY, rho, rhoY, dS)
+    b = op.scale_rhs(to_dev(rhs_isc))
+    x, info = op.solve(b, tol=ud.tol if tol is None else tol,
+                       maxiter=ud.max_iterations if maxiter is None else maxiter,
+                       mode=mode, inner_tol=inner_tol, check_every=check_every)
+    torch.cuda.synchronize()
+    return x.cpu().numpy(), info
