@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Benchmark: solves/s of the 1025x513-node nodal elliptic solve on one MI355X.
+
+Workload (the metric's configuration): isolated 2D nodal 9-point pressure
+solve at 1025 x 513 nodes (x in [-10, 10] km periodic, z in [0, 10] km walls),
+coefficients of a hydrostatically stratified atmosphere (theta 300 -> 330 K
+linear in z, Exner from hydrostatic balance), pseudo-incompressible
+(wcenter = 0), half-step dt = 0.05 (5 s), seeded smooth random RHS (64
+Fourier modes, |k| <= 16, seed 20261202) projected onto the operator's range.
+
+One step = FP64 operator assembly on the GPU (coefficients, Coriolis factors,
+diagonal, folded stencil weights) + D^-1 scaling of the RHS + the solve to
+true relative residual ||D^-1 (b - A x)|| / ||D^-1 b|| <= 1e-8.  Inputs are
+resident in HBM before the timed region; every timed step does all of that
+work.
+
+Printed: ONE JSON line with the contract fields plus ``roofline`` (dominant
+kernel, HBM-bound) and ``cpu_baseline`` (the C restatement of the reference's
+serial matvec + BiCGStab, oracle/c/pb_oracle.c, on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, public spec
+
+# Algorithmic bytes per node of each solver kernel (FP32 Krylov vectors, zero
+# Coriolis, zero centre term): every distinct array read once and written once.
+#   k1: p-update + matvec + rt.v : read r, p_old, v_old, rt, Wxx, Wyy, dinv; write p, v  -> 9 words
+#   k2: s + matvec + 3 dots       : read r, v, Wxx, Wyy, dinv; write t                  -> 6 words
+#   k3: x/r update + 2 dots       : read x, r, rt, p, v, t; write x, r                   -> 8 words
+KERNEL_WORDS = {0: 5, 1: 9, 2: 6, 3: 8}
+KERNEL_NAMES = {0: "matvec", 1: "bicg_pupdate_matvec_dot", 2: "bicg_s_matvec_dots", 3: "bicg_xr_update_dots"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(nx=1025, ny=513, seed=20261202):
+    from pybella_amd import workloads as W
+    Sol, mpv, elem, node, ud, th = W.stratified_state(nx, ny)
+    rhs = W.stratified_rhs(node, seed=seed)
+    return (Sol, mpv, elem, node, ud, th), rhs
+
+
+def cpu_baseline(state, rhs, full_iters, sample_iters, dt=0.05):
+    """Time the C restatement (oracle) on ``sample_iters`` iterations of the same
+    solve and scale to the full iteration count."""
+    import copy
+    from oracle import cport
+    from oracle import laplacian as olp
+    from oracle import projection as opj
+    Sol, mpv, elem, node, ud, th = [copy.deepcopy(o) for o in state]
+    t0 = time.perf_counter()
+    opj.operator_coefficients_nodes(elem, node, Sol, mpv, ud, th, dt)
+    mpv.wcenter[...] *= ud.compressibility
+    cor = opj.multiply_inverse_coriolis(mpv, Sol, mpv, ud, elem, node, dt, attrs=("u", "v", "w"), get_coeffs=True)
+    dinv = olp.precon_diag_prepare(mpv, elem, node, ud)
+    op = olp.op2d_arrays(mpv, node, cor, dinv, ud)
+    b = (rhs * dinv)[1:-1, 1:-1].T.ravel().copy()
+    t_asm = time.perf_counter() - t0
+    c = cport.COp(op)
+    t0 = time.perf_counter()
+    _, info, it, half, _ = c.bicgstab(b, tol=1e-30, maxiter=sample_iters)
+    t_it = (time.perf_counter() - t0) / max(it, 1)
+    t_solve = t_asm + t_it * full_iters
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": 1.0 / t_solve,
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{it} BiCGStab iterations (serial C restatement of lap2D_gather_new + Templates "
+                   f"BiCGStab, FP64) of the same 1025x513 solve, {t_it * 1e3:.2f} ms/iteration, scaled to "
+                   f"the {full_iters} iterations the FP64 solve needs, plus {t_asm:.2f} s NumPy assembly; "
+                   f"CPU: {cpu}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", default="auto", choices=["auto", "f64", "f64fast", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-iters", type=int, default=300)
+    ap.add_argument("--kernel-reps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(a solve shards by domain decomposition; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pybella_amd import second_projection as sp
+
+    log("building workload 1025x513 ...")
+    state, rhs = build_workload()
+    Sol, mpv, elem, node, ud, th = state
+    dt = 0.05
+    grid = sp.grid_of(node, ud)
+    phys = sp.phys_of(ud, th)
+    op = sp.Operator2D(grid)
+    rho, rhoY = sp.to_dev(Sol.rho), sp.to_dev(Sol.rhoY)
+    dS = sp.to_dev(sp.dSdy_cells(mpv, elem, node))
+    rhs_d = sp.to_dev(rhs)
+    b = torch.empty(grid.ny, grid.nx, dtype=torch.float64, device="cuda")
+    x = torch.empty_like(b)
+    torch.cuda.synchronize()
+
+    def step(mode):
+        op.assemble(phys, dt, rho, rhoY, rho, rhoY, dS)
+        op.scale_rhs(rhs_d, b)
+        _, info = op.solve(b, x, tol=1e-8, maxiter=6000, mode=mode, inner_tol=1e-3)
+        return info
+
+    from pybella_amd.autotune import tune_tile
+    # plain FP64 reference-semantics solve (iteration count, parity anchor)
+    info64 = step("f64")
+    log("f64 (exact) solve:", info64)
+    # calibrate every plain mode (tile height autotuned per mode), pick the fastest
+    rates, tiles, infos_cal = {}, {}, {}
+    modes = ["f64", "f64fast", "mixed"] if args.mode == "auto" else [args.mode]
+    for m in modes:
+        best, table = tune_tile(op, lambda: step(m), key=(grid.nx, grid.ny, m))
+        infos_cal[m] = step(m)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step(m)
+        torch.cuda.synchronize()
+        rates[m] = 1.0 / (time.perf_counter() - t0)
+        tiles[m] = {"rows_per_thread": best, "seconds": {str(k): round(v, 5) for k, v in table.items()}}
+        log(f"mode {m}: {rates[m]:.2f} solves/s, tile rows/thread {best}, {infos_cal[m]}")
+    mode = max(rates, key=rates.get)
+    op.set_tile(tiles[mode]["rows_per_thread"])
+    for _ in range(args.warmup):
+        step(mode)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = []
+    for _ in range(args.steps):
+        infos.append(step(mode))
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    log("timed:", mode, infos[-1])
+    for inf in infos:
+        if inf["info"] != 0 or inf["true_rel_resid"] > 1e-8:
+            raise SystemExit(f"solve did not reach the tolerance: {inf}")
+    value = args.steps / el
+
+    # roofline of the solver kernels (HIP events, same stream)
+    prec = "f32" if mode == "mixed" else "f64"
+    wbytes = 4 if prec == "f32" else 8
+    n = grid.nx * grid.ny
+    kt = {k: op.time_kernel(k, prec, args.kernel_reps) for k in (0, 1, 2, 3)}
+    per_iter = {k: kt[k] for k in (1, 2, 3)}
+    dom = max(per_iter, key=per_iter.get)
+    alg = KERNEL_WORDS[dom] * wbytes * n
+    achieved = alg / (kt[dom] * 1e-6) / 1e9
+    roof = {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "avg_us": round(kt[dom], 3), "alg_bytes": alg,
+            "kernels_us": {KERNEL_NAMES[k]: round(v, 3) for k, v in kt.items()}}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(state, rhs, info64["iters"], args.cpu_sample_iters)
+
+    out = {
+        "metric": "solves/s at rel. resid 1e-8, 1024x512 nodal solve: NN-precond vs plain BiCGStab",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * el / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if mode == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "isolated 2D nodal 9-point solve, 1025x513 nodes, stratified theta 300->330 K, "
+                               "psinc, seeded 64-mode RHS, true rel. resid <= 1e-8",
+                   "solver": {"mixed": "plain BiCGStab, FP32 Krylov (FP64 iterate) in FP64 iterative refinement",
+                              "f64": "plain BiCGStab FP64 (reference semantics, 3 launches/iteration)",
+                              "f64fast": "plain BiCGStab FP64 (rho, ||r|| from s/t dots, 2 launches/iteration)"}[mode],
+                   "iterations": infos[-1]["iters"], "refinements": infos[-1]["outer"],
+                   "true_rel_resid": infos[-1]["true_rel_resid"],
+                   "f64_iterations": info64["iters"],
+                   "plain_modes_solves_per_s": {k: round(v, 3) for k, v in rates.items()},
+                   "tiles": tiles},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -49,9 +49,12 @@ extern "C" {
 #define PB_F32 1
 
 /* solver modes */
-#define PB_SOLVE_F64 0      /* FP64 BiCGStab, reference semantics            */
-#define PB_SOLVE_MIXED 1    /* FP32 BiCGStab inside FP64 iterative refinement */
-#define PB_SOLVE_NN 2       /* learned preconditioner, flexible outer Krylov  */
+#define PB_SOLVE_F64 0      /* FP64 BiCGStab, reference semantics: rho = rt.r and ||r||
+                               from the stored r (3 launches per iteration)      */
+#define PB_SOLVE_MIXED 1    /* FP32 BiCGStab inside FP64 iterative refinement  */
+#define PB_SOLVE_NN 2       /* learned preconditioner, flexible outer Krylov   */
+#define PB_SOLVE_F64_FAST 3 /* FP64 BiCGStab, rho and ||r|| from s/t dots
+                               (2 launches per iteration)                      */
 
 typedef struct pb_ctx pb_ctx;
 typedef struct pb_op2d pb_op2d;
@@ -138,6 +141,20 @@ int pb_op2d_scale_rhs(pb_op2d* op, const double* rhs_isc, double* b_vec);
  * a PB_ERR_* code. */
 int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x,
                       const pb_solve_opts* opts, pb_solve_info* info);
+
+/* Tile height of the solver's stencil launches: 64 x (4 * rows_per_thread)
+ * nodes per 256-thread block; rows_per_thread in {2, 4, 8} (default 4).  Used
+ * by the tile autotuner.  No reference counterpart. */
+int pb_op2d_set_tile(pb_op2d* op, int rows_per_thread);
+
+/* Average device time (microseconds) of one launch of a solver kernel, timed
+ * with HIP events over ``reps`` back-to-back launches on the context stream on
+ * pattern-filled workspace vectors.  kernel: 0 plain matvec, 1 BiCGStab
+ * p-update + matvec + rt.v reduction, 2 s + matvec + (s.s, t.s, t.t), 3 x/r
+ * update + (rt.r, r.r).  Synchronous.  Measurement aid; no reference
+ * counterpart.  Kernels 1 and 2 are the two launches of the PB_SOLVE_F64_FAST
+ * iteration; kernel 3 is the FP64 true-residual stencil. */
+int pb_op2d_time_kernel(pb_op2d* op, int kernel, int precision, int reps, double* avg_us);
 
 /* divergence of (rhou, rhov)*Y at nodes -> rhs (isc), divided by dt.
  * Zeroes rhou/rhov/rhow in the two cell rows next to y walls first. */

@@ -18,6 +18,7 @@ PB_F32 = 1
 PB_SOLVE_F64 = 0
 PB_SOLVE_MIXED = 1
 PB_SOLVE_NN = 2
+PB_SOLVE_F64_FAST = 3
 
 
 class Grid2D(C.Structure):
@@ -66,6 +67,8 @@ SIGNATURES = {
     "pb_op2d_matvec": (I, [VP, VP, VP, I]),
     "pb_op2d_scale_rhs": (I, [VP, VP, VP]),
     "pb_bicgstab_solve": (I, [VP, VP, VP, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+    "pb_op2d_time_kernel": (I, [VP, I, I, I, C.POINTER(D)]),
+    "pb_op2d_set_tile": (I, [VP, I]),
     "pb_divergence_nodes": (I, [VP, C.POINTER(Grid2D), D, VP, VP, VP, VP, VP, VP]),
     "pb_correction_nodes": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), D, D, VP, VP, VP, VP,
                                 VP, VP, VP, VP, VP, VP]),

@@ -1,0 +1,936 @@
+// BiCGStab on the 2D nodal operator, two fused stencil launches per iteration
+// (gfx950).  Replaces the reference's scipy.sparse.linalg.bicgstab call
+// (second_projection.py:230; SciPy 1.7.3 BiCGSTAB of Barrett et al.,
+// Templates 1994, sec. 2.3.8, M = I).
+//
+// Iteration k of the Templates algorithm
+//   p_k = r_{k-1} + beta (p_{k-1} - omega v_{k-1}),  v_k = A p_k,
+//   alpha = rho / (rt.v_k),  s = r_{k-1} - alpha v_k,  t = A s,
+//   omega = (t.s)/(t.t),  x += alpha p_k + omega s,  r_k = s - omega t,
+// is split into two launches:
+//   A(k): x update of iteration k-1, r_{k-1} = s_{k-1} - omega t formed on the
+//         fly, p_k, v_k = A p_k; block partials of rt.v_k;
+//   B(k): alpha from A's partials; s = r - alpha v on the fly, t = A s; block
+//         partials of s.s, t.s, t.t, rt.s, rt.t.
+// The scalars of the next iteration follow from B's sums without another pass:
+//   ||r_k||^2 = s.s - 2 omega t.s + omega^2 t.t,  rho_k = rt.s - omega rt.t.
+// Stopping follows the reference's legacy semantics: x0 = 0, return at once if
+// ||b - A x0|| <= tol (absolute), stop at ||s|| <= tol ||b|| (x += alpha p) or
+// ||r|| <= tol ||b||, eps^2 breakdown tests on rho and omega.
+//
+// Each launch derives its scalars itself: every block sums the previous
+// launch's block partials in block order (deterministic, no grid fence), and
+// block 0 stores the new state in the other half of a double buffer.  x is
+// kept in FP64 also when the Krylov vectors are FP32.
+#include <cstdlib>
+
+#include "pb_op2d.hpp"
+
+using namespace pb2d;
+
+namespace {
+
+__device__ __forceinline__ double eps2() { return 4.930380657631324e-32; }   // DBL_EPSILON^2
+
+__device__ __forceinline__ int block_id() { return blockIdx.x + blockIdx.y * gridDim.x; }
+__device__ __forceinline__ int nblocks() { return gridDim.x * gridDim.y; }
+__device__ __forceinline__ int thread_id() { return threadIdx.x + threadIdx.y * blockDim.x; }
+
+// ---------------------------------------------------------------------------
+// init: r0 = b - A x0, rt = r0, p = v = t = 0, x = x0 (or 0); partials (r.r, b.b)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct InitIn {
+  const double* x0;
+  int given;
+  __device__ T operator()(long g) const { return given ? (T)x0[g] : T(0); }
+};
+template <typename T>
+struct InitOut {
+  const T* b;
+  T *r, *rt, *p, *v, *t;
+  double* x;
+  const double* x0;
+  int given;
+  double rr, bb;
+  __device__ void operator()(long g, T y, T) {
+    const T bg = b[g];
+    const T rg = bg - y;
+    r[g] = rg;
+    rt[g] = rg;
+    p[g] = T(0);
+    v[g] = T(0);
+    t[g] = T(0);
+    x[g] = given ? x0[g] : 0.0;
+    rr += (double)rg * (double)rg;
+    bb += (double)bg * (double)bg;
+  }
+};
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_init(OpDev<T> op, InitIn<T> in, InitOut<T> out, double* PB) {
+  __shared__ TileLds<T, RY> L;
+  __shared__ double sred[8 * (NTHR / kWave)];
+  out.rr = 0.0;
+  out.bb = 0.0;
+  stencil_tile<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L);
+  double v[2] = {out.rr, out.bb};
+  write_partials<2>(v, sred, PB, nblocks(), block_id());
+}
+
+// ---------------------------------------------------------------------------
+// launch A(k)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct IterPtrs {
+  const T* r_prev;
+  const T* p_prev;
+  const T* v_prev;
+  const T* t;
+  const T* rt;
+  T* r_new;
+  T* p_new;
+  T* v_new;
+  double* x;
+};
+
+template <typename T>
+struct AIn {
+  const T *r, *p, *v, *t;
+  T alpha, omega, beta;
+  __device__ T operator()(long g) const {
+    const T s = r[g] - alpha * v[g];
+    const T rn = s - omega * t[g];
+    return rn + beta * (p[g] - omega * v[g]);
+  }
+};
+
+template <typename T>
+struct AOut {
+  IterPtrs<T> q;
+  T alpha, omega;
+  double alpha_d, omega_d;
+  int xupd;
+  double rv;
+  __device__ void operator()(long g, T y, T pn) {
+    const T s = q.r_prev[g] - alpha * q.v_prev[g];
+    const T rn = s - omega * q.t[g];
+    if (xupd) {
+      const double xg = q.x[g] + alpha_d * (double)q.p_prev[g];
+      q.x[g] = xg + omega_d * (double)s;
+    }
+    q.r_new[g] = rn;
+    q.p_new[g] = pn;
+    q.v_new[g] = y;
+    rv += (double)q.rt[g] * (double)y;
+  }
+};
+
+// The scalar step shared by every block of A: returns the new state from the
+// previous state and the previous launch's totals.
+__device__ __forceinline__ KState scalars_after_init(KState st, double rr, double bb) {
+  st.bnrm2 = sqrt(bb);
+  st.rnorm0 = sqrt(rr);
+  st.resid = st.rnorm0;
+  st.iter = 0;
+  st.half = 0;
+  st.alpha = 0.0;
+  st.omega = 0.0;
+  st.beta = 0.0;
+  st.x_upd = 0;
+  st.rho = rr;
+  st.atol = st.tol * st.bnrm2;
+  if (st.legacy ? (st.rnorm0 <= st.tol) : (st.rnorm0 <= st.atol)) {
+    st.status = ST_CONV;
+  } else if (fabs(rr) < eps2()) {
+    st.status = ST_BRK_RHO;
+  } else {
+    st.status = ST_RUN;
+  }
+  return st;
+}
+
+__device__ __forceinline__ KState scalars_after_b(KState st, const double (&tot)[5]) {
+  const double ss = tot[0], ts = tot[1], tt = tot[2], rts = tot[3], rtt = tot[4];
+  const double snrm = sqrt(ss);
+  st.x_upd = 1;
+  if (snrm <= st.atol) {
+    st.status = ST_P_HALF;
+    st.resid = snrm;
+    return st;
+  }
+  const double om = ts / tt;
+  if (!isfinite(om) || !isfinite(snrm)) {
+    st.status = ST_BRK_OMEGA;
+    return st;
+  }
+  st.omega = om;
+  const double rr = ss - 2.0 * om * ts + om * om * tt;
+  const double rho_new = rts - om * rtt;
+  st.iter += 1;
+  st.resid = sqrt(fmax(rr, 0.0));
+  if (st.resid <= st.atol) {
+    st.status = ST_P_CONV;
+  } else if (fabs(om) < eps2()) {
+    st.status = ST_P_BRK_OMEGA;
+  } else if (st.iter >= st.maxiter) {
+    st.status = ST_P_MAXIT;
+  } else {
+    const double beta = (rho_new / st.rho) * (st.alpha / om);
+    st.rho = rho_new;
+    if (fabs(rho_new) < eps2() || !isfinite(beta)) {
+      st.status = ST_P_BRK_RHO;
+    } else {
+      st.beta = beta;
+      st.status = ST_RUN;
+    }
+  }
+  return st;
+}
+
+__device__ __forceinline__ int finalize(int pending) {
+  switch (pending) {
+    case ST_P_CONV: return ST_CONV;
+    case ST_P_HALF: return ST_CONV;
+    case ST_P_MAXIT: return ST_MAXIT;
+    case ST_P_BRK_RHO: return ST_BRK_RHO;
+    case ST_P_BRK_OMEGA: return ST_BRK_OMEGA;
+    default: return pending;
+  }
+}
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_iter_a(OpDev<T> op, IterPtrs<T> q, const KState* Sin, KState* Sout,
+                                                 const double* PB, double* PA) {
+  __shared__ TileLds<T, RY> L;
+  __shared__ double sred[8 * (NTHR / kWave)];
+  const int tid = thread_id(), bid = block_id(), nb = nblocks();
+  KState st = *Sin;
+  if (st_final(st.status)) {
+    if (bid == 0 && tid == 0) *Sout = st;
+    return;
+  }
+  if (st.phase == PH_INIT) {
+    double tot[2];
+    sum_partials<2>(PB, nb, tot, sred);
+    st = scalars_after_init(st, tot[0], tot[1]);
+  } else {
+    double tot[5];
+    sum_partials<5>(PB, nb, tot, sred);
+    st = scalars_after_b(st, tot);
+  }
+  st.phase = PH_A;
+  if (st.status != ST_RUN) {
+    if (st.status >= ST_P_CONV) {   // final x update of a finished run
+      const bool half = st.status == ST_P_HALF;
+      const T alpha = (T)st.alpha;
+      constexpr int TY = TYT * RY;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const int j = blockIdx.y * TY + threadIdx.y + TYT * r, i = blockIdx.x * TX + threadIdx.x;
+        if (j < op.ny && i < op.nx) {
+          const long g = (long)j * op.nx + i;
+          const double xg = q.x[g] + st.alpha * (double)q.p_prev[g];
+          if (half) {
+            q.x[g] = xg;
+          } else {
+            const T s = q.r_prev[g] - alpha * q.v_prev[g];
+            q.x[g] = xg + st.omega * (double)s;
+          }
+        }
+      }
+      if (half) st.half = 1;
+      st.status = finalize(st.status);
+    }
+    if (bid == 0 && tid == 0) *Sout = st;
+    return;
+  }
+  const T alpha = (T)st.alpha, omega = (T)st.omega, beta = (T)st.beta;
+  AIn<T> in{q.r_prev, q.p_prev, q.v_prev, q.t, alpha, omega, beta};
+  AOut<T> out{q, alpha, omega, st.alpha, st.omega, st.x_upd, 0.0};
+  stencil_tile<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L);
+  double v[1] = {out.rv};
+  write_partials<1>(v, sred, PA, nb, bid);
+  if (bid == 0 && tid == 0) *Sout = st;
+}
+
+// ---------------------------------------------------------------------------
+// launch B(k)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct BIn {
+  const T *r, *v;
+  T alpha;
+  __device__ T operator()(long g) const { return r[g] - alpha * v[g]; }
+};
+template <typename T>
+struct BOut {
+  T* t;
+  const T* rt;
+  double ss, ts, tt, rts, rtt;
+  __device__ void operator()(long g, T y, T s) {
+    t[g] = y;
+    const double sd = (double)s, yd = (double)y, rd = (double)rt[g];
+    ss += sd * sd;
+    ts += yd * sd;
+    tt += yd * yd;
+    rts += rd * sd;
+    rtt += rd * yd;
+  }
+};
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_iter_b(OpDev<T> op, const T* r, const T* v, const T* rt, T* t,
+                                                 const KState* Sin, KState* Sout, const double* PA, double* PB) {
+  __shared__ TileLds<T, RY> L;
+  __shared__ double sred[8 * (NTHR / kWave)];
+  const int tid = thread_id(), bid = block_id(), nb = nblocks();
+  KState st = *Sin;
+  if (st.status != ST_RUN) {
+    if (bid == 0 && tid == 0) *Sout = st;
+    return;
+  }
+  double tot[1];
+  sum_partials<1>(PA, nb, tot, sred);
+  const double a = st.rho / tot[0];
+  st.phase = PH_B;
+  if (tot[0] == 0.0 || !isfinite(a)) {
+    st.status = ST_BRK_OMEGA;
+    if (bid == 0 && tid == 0) *Sout = st;
+    return;
+  }
+  st.alpha = a;
+  BIn<T> in{r, v, (T)a};
+  BOut<T> out{t, rt, 0.0, 0.0, 0.0, 0.0, 0.0};
+  stencil_tile<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L);
+  double vals[5] = {out.ss, out.ts, out.tt, out.rts, out.rtt};
+  write_partials<5>(vals, sred, PB, nb, bid);
+  if (bid == 0 && tid == 0) *Sout = st;
+}
+
+// ---------------------------------------------------------------------------
+// exact variant: three launches per iteration with rho = rt.r and ||r||
+// formed from the stored r as the reference does.
+//   L1(k): beta from L3's partials (or init); p_k = beta (p - omega v) + r,
+//          v_k = A p_k; partials rt.v
+//   L2(k): the B launch (alpha; s = r - alpha v, t = A s; dots)
+//   L3(k): ||s|| test, omega; x += alpha p + omega s; r = s - omega t;
+//          partials rt.r, r.r
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ KState scalars_after_l3(KState st, double rtr, double rr) {
+  const double rnrm = sqrt(rr);
+  st.iter += 1;
+  st.resid = rnrm;
+  if (!isfinite(rnrm) || !isfinite(rtr)) {
+    st.status = ST_BRK_RHO;
+  } else if (rnrm <= st.atol) {
+    st.status = ST_CONV;
+  } else if (fabs(st.omega) < eps2()) {
+    st.status = ST_BRK_OMEGA;
+  } else if (st.iter >= st.maxiter) {
+    st.status = ST_MAXIT;
+  } else if (fabs(rtr) < eps2()) {
+    st.status = ST_BRK_RHO;
+  } else {
+    st.beta = (rtr / st.rho) * (st.alpha / st.omega);
+    st.rho = rtr;
+    st.status = ST_RUN;
+  }
+  return st;
+}
+
+template <typename T>
+struct L1In {
+  const T *r, *p, *v;
+  T omega, beta;
+  __device__ T operator()(long g) const { return beta * (p[g] - omega * v[g]) + r[g]; }
+};
+template <typename T>
+struct L1Out {
+  T *p_new, *v_new;
+  const T* rt;
+  double rv;
+  __device__ void operator()(long g, T y, T pn) {
+    p_new[g] = pn;
+    v_new[g] = y;
+    rv += (double)rt[g] * (double)y;
+  }
+};
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_ex_p(OpDev<T> op, const T* r, const T* p_prev, const T* v_prev,
+                                               T* p_new, T* v_new, const T* rt, const KState* Sin, KState* Sout,
+                                               const double* PC, double* PA) {
+  __shared__ TileLds<T, RY> L;
+  __shared__ double sred[8 * (NTHR / kWave)];
+  const int tid = thread_id(), bid = block_id(), nb = nblocks();
+  KState st = *Sin;
+  if (st_final(st.status)) {
+    if (bid == 0 && tid == 0) *Sout = st;
+    return;
+  }
+  double tot[2];
+  sum_partials<2>(PC, nb, tot, sred);
+  st = (st.phase == PH_INIT) ? scalars_after_init(st, tot[0], tot[1]) : scalars_after_l3(st, tot[0], tot[1]);
+  st.phase = PH_A;
+  if (st.status != ST_RUN) {
+    if (bid == 0 && tid == 0) *Sout = st;
+    return;
+  }
+  L1In<T> in{r, p_prev, v_prev, (T)st.omega, (T)st.beta};
+  L1Out<T> out{p_new, v_new, rt, 0.0};
+  stencil_tile<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L);
+  double v[1] = {out.rv};
+  write_partials<1>(v, sred, PA, nb, bid);
+  if (bid == 0 && tid == 0) *Sout = st;
+}
+
+template <typename T, int RY>
+__global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const T* p, const T* v, const T* t,
+                                                    const T* rt, double* x, const KState* Sin, KState* Sout,
+                                                    const double* PB, double* PC) {
+  __shared__ double sred[8 * (NTHR / kWave)];
+  const int tid = thread_id(), bid = block_id(), nb = nblocks();
+  KState st = *Sin;
+  if (st.status != ST_RUN) {
+    if (bid == 0 && tid == 0) *Sout = st;
+    return;
+  }
+  double tot[5];
+  sum_partials<5>(PB, nb, tot, sred);
+  const double snrm = sqrt(tot[0]);
+  bool half = false;
+  if (snrm <= st.atol) {
+    half = true;
+    st.resid = snrm;
+    st.half = 1;
+    st.status = ST_CONV;
+  } else {
+    const double om = tot[1] / tot[2];
+    if (!isfinite(om) || !isfinite(snrm)) {
+      st.status = ST_BRK_OMEGA;
+      if (bid == 0 && tid == 0) *Sout = st;
+      return;
+    }
+    st.omega = om;
+  }
+  st.phase = PH_L3;
+  const T alpha = (T)st.alpha, omega = (T)st.omega;
+  double rtr = 0.0, rr = 0.0;
+  constexpr int TY = TYT * RY;
+#pragma unroll
+  for (int q = 0; q < RY; ++q) {
+    const int j = blockIdx.y * TY + threadIdx.y + TYT * q, i = blockIdx.x * TX + threadIdx.x;
+    if (j < ny && i < nx) {
+      const long g = (long)j * nx + i;
+      const double xg = x[g] + st.alpha * (double)p[g];
+      if (half) {
+        x[g] = xg;
+      } else {
+        const T s = r[g] - alpha * v[g];
+        x[g] = xg + st.omega * (double)s;
+        const T rn = s - omega * t[g];
+        r[g] = rn;
+        rtr += (double)rt[g] * (double)rn;
+        rr += (double)rn * (double)rn;
+      }
+    }
+  }
+  double vals[2] = {rtr, rr};
+  write_partials<2>(vals, sred, PC, nb, bid);
+  if (bid == 0 && tid == 0) *Sout = st;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 true residual r = b - A x with partials of r.r, and a one-block sum
+// ---------------------------------------------------------------------------
+struct TrOut {
+  const double* b;
+  double* r;
+  double rr;
+  __device__ void operator()(long g, double y, double) {
+    const double rg = b[g] - y;
+    r[g] = rg;
+    rr += rg * rg;
+  }
+};
+
+template <bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_true_res(OpDev<double> op, const double* x, const double* b, double* r,
+                                                   double* PT) {
+  __shared__ TileLds<double, RY> L;
+  __shared__ double sred[8 * (NTHR / kWave)];
+  struct In {
+    const double* x;
+    __device__ double operator()(long g) const { return x[g]; }
+  } in{x};
+  TrOut out{b, r, 0.0};
+  stencil_tile<double, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L);
+  double v[1] = {out.rr};
+  write_partials<1>(v, sred, PT, nblocks(), block_id());
+}
+
+__global__ __launch_bounds__(NTHR) void k_sum1(const double* P, int nb, double* out) {
+  __shared__ double sred[8 * (NTHR / kWave)];
+  double tot[1];
+  sum_partials<1>(P, nb, tot, sred);
+  if (thread_id() == 0) *out = tot[0];
+}
+
+template <typename Tin, typename Tout>
+__global__ void k_cast(long n, const Tin* a, Tout* b) {
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
+    b[g] = (Tout)a[g];
+}
+
+__global__ void k_axpy(long n, double a, const double* dx, double* x) {
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
+    x[g] += a * dx[g];
+}
+
+template <typename T>
+__global__ void k_fill_pattern(long n, T* a, double phase) {
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
+    a[g] = (T)(0.5 + 0.25 * sin(0.37 * (double)g + phase));
+}
+
+__global__ void k_fill_const(long n, double* a, double v) {
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) a[g] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <typename T>
+Work<T>& work_of(pb_op2d* op);
+template <>
+Work<double>& work_of<double>(pb_op2d* op) { return op->w64; }
+template <>
+Work<float>& work_of<float>(pb_op2d* op) { return op->w32; }
+
+template <typename T>
+int alloc_vecs(pb_op2d* op, Work<T>& w, int nb) {
+  const long n = op->n;
+  int rc = 0;
+  for (int k = 0; k < 2; ++k) {
+    rc |= dmalloc(op, &w.r[k], n);
+    rc |= dmalloc(op, &w.p[k], n);
+    rc |= dmalloc(op, &w.v[k], n);
+  }
+  rc |= dmalloc(op, &w.t, n);
+  rc |= dmalloc(op, &w.rt, n);
+  rc |= dmalloc(op, &w.b, n);
+  rc |= dmalloc(op, &w.x, n);
+  rc |= dmalloc(op, &w.PA, nb);
+  rc |= dmalloc(op, &w.PB, 5L * nb);
+  rc |= dmalloc(op, &w.PC, 2L * nb);
+  rc |= dmalloc(op, &w.S, 3);
+  return rc;
+}
+
+int alloc_work(pb_op2d* op) {
+  if (op->have_work) return PB_OK;
+  const dim3 sg = stencil_grid(op, 2);   // the finest tiling bounds the block count
+  const int nb = (int)(sg.x * sg.y);
+  int rc = alloc_vecs<double>(op, op->w64, nb);
+  rc |= alloc_vecs<float>(op, op->w32, nb);
+  rc |= dmalloc(op, &op->rtrue, op->n);
+  rc |= dmalloc(op, &op->xcorr, op->n);
+  rc |= dmalloc(op, &op->zero, op->n);
+  rc |= dmalloc(op, &op->PT, nb);
+  rc |= dmalloc(op, &op->red, 2);
+  if (rc) return PB_ERR_HIP;
+  void* h = nullptr;
+  if (hipHostMalloc(&h, sizeof(KState) * 2, hipHostMallocDefault) != hipSuccess)
+    return pb_fail(op->ctx, PB_ERR_HIP, "hipHostMalloc failed");
+  op->hstate = (KState*)h;
+  if (hipHostMalloc(&h, sizeof(double) * 2, hipHostMallocDefault) != hipSuccess)
+    return pb_fail(op->ctx, PB_ERR_HIP, "hipHostMalloc failed");
+  op->hred = (double*)h;
+  hipLaunchKernelGGL(k_fill_const, dim3(elem_blocks(op->n)), dim3(256), 0, op->ctx->own_stream, op->n, op->zero,
+                     0.0);
+  PB_HIP(op->ctx, hipStreamSynchronize(op->ctx->own_stream));
+  op->have_work = true;
+  return PB_OK;
+}
+
+// kernel dispatch over (CROSS, CENTER) and the tile height RY = op->ry (2, 4, 8)
+#define PB_DISPATCH_RY(KERNEL, T, RYV, grd, s, ...)                                                    \
+  do {                                                                                                \
+    const dim3 _blk(TX, TYT);                                                                         \
+    if (_c && _h) hipLaunchKernelGGL((KERNEL<T, true, true, RYV>), grd, _blk, 0, s, __VA_ARGS__);     \
+    else if (_c) hipLaunchKernelGGL((KERNEL<T, true, false, RYV>), grd, _blk, 0, s, __VA_ARGS__);     \
+    else if (_h) hipLaunchKernelGGL((KERNEL<T, false, true, RYV>), grd, _blk, 0, s, __VA_ARGS__);     \
+    else hipLaunchKernelGGL((KERNEL<T, false, false, RYV>), grd, _blk, 0, s, __VA_ARGS__);            \
+  } while (0)
+
+#define PB_DISPATCH(KERNEL, T, op, grd, s, ...)                                 \
+  do {                                                                         \
+    const bool _c = (op)->cross, _h = (op)->center;                           \
+    if ((op)->ry == 8) PB_DISPATCH_RY(KERNEL, T, 8, grd, s, __VA_ARGS__);       \
+    else if ((op)->ry == 2) PB_DISPATCH_RY(KERNEL, T, 2, grd, s, __VA_ARGS__);  \
+    else PB_DISPATCH_RY(KERNEL, T, 4, grd, s, __VA_ARGS__);                     \
+  } while (0)
+
+template <typename T>
+IterPtrs<T> iter_ptrs(Work<T>& w, int k) {
+  const int po = (k - 1) & 1, pn = k & 1;
+  return IterPtrs<T>{w.r[po], w.p[po], w.v[po], w.t, w.rt, w.r[pn], w.p[pn], w.v[pn], w.x};
+}
+
+template <typename T>
+void launch_a(pb_op2d* op, Work<T>& w, int k, hipStream_t s) {
+  const dim3 grd = stencil_grid(op, op->ry);
+  const OpDev<T> d = opdev<T>(op);
+  PB_DISPATCH(k_iter_a, T, op, grd, s, d, iter_ptrs(w, k), (const KState*)w.S, w.S + 1, (const double*)w.PB, w.PA);
+}
+
+template <typename T>
+void launch_b(pb_op2d* op, Work<T>& w, int k, hipStream_t s, int sin = 1, int sout = 0, bool exact = false) {
+  const dim3 grd = stencil_grid(op, op->ry);
+  const OpDev<T> d = opdev<T>(op);
+  const int pn = k & 1;
+  const T* r = exact ? w.r[0] : w.r[pn];   // the exact variant keeps r in place
+  PB_DISPATCH(k_iter_b, T, op, grd, s, d, r, (const T*)w.v[pn], (const T*)w.rt, w.t,
+              (const KState*)(w.S + sin), w.S + sout, (const double*)w.PA, w.PB);
+}
+
+// exact variant, iteration k: L1 reads S2 writes S0, L2 reads S0 writes S1,
+// L3 reads S1 writes S2.  r lives in w.r[0]; p and v alternate with k.
+template <typename T>
+void launch_l1(pb_op2d* op, Work<T>& w, int k, hipStream_t s) {
+  const dim3 grd = stencil_grid(op, op->ry);
+  const OpDev<T> d = opdev<T>(op);
+  const int po = (k - 1) & 1, pn = k & 1;
+  PB_DISPATCH(k_ex_p, T, op, grd, s, d, (const T*)w.r[0], (const T*)w.p[po], (const T*)w.v[po], w.p[pn],
+              w.v[pn], (const T*)w.rt, (const KState*)(w.S + 2), w.S, (const double*)w.PC, w.PA);
+}
+
+template <typename T>
+void launch_l3(pb_op2d* op, Work<T>& w, int k, hipStream_t s) {
+  const dim3 grd = stencil_grid(op, op->ry), blk(TX, TYT);
+  const int pn = k & 1;
+  if (op->ry == 8)
+    hipLaunchKernelGGL((k_ex_update<T, 8>), grd, blk, 0, s, op->g.nx, op->g.ny, w.r[0], (const T*)w.p[pn],
+                       (const T*)w.v[pn], (const T*)w.t, (const T*)w.rt, w.x, (const KState*)(w.S + 1), w.S + 2,
+                       (const double*)w.PB, w.PC);
+  else if (op->ry == 2)
+    hipLaunchKernelGGL((k_ex_update<T, 2>), grd, blk, 0, s, op->g.nx, op->g.ny, w.r[0], (const T*)w.p[pn],
+                       (const T*)w.v[pn], (const T*)w.t, (const T*)w.rt, w.x, (const KState*)(w.S + 1), w.S + 2,
+                       (const double*)w.PB, w.PC);
+  else
+    hipLaunchKernelGGL((k_ex_update<T, 4>), grd, blk, 0, s, op->g.nx, op->g.ny, w.r[0], (const T*)w.p[pn],
+                       (const T*)w.v[pn], (const T*)w.t, (const T*)w.rt, w.x, (const KState*)(w.S + 1), w.S + 2,
+                       (const double*)w.PB, w.PC);
+}
+
+// chunk of c iterations starting at an odd k (the buffer parity pattern
+// repeats every two iterations).  fast: B(k), A(k+1); exact: L2(k), L3(k), L1(k+1)
+template <typename T>
+void launch_chunk(pb_op2d* op, Work<T>& w, int c, bool exact, hipStream_t s) {
+  for (int i = 0; i < c; ++i) {
+    const int k = 1 + i;
+    if (exact) {
+      launch_b<T>(op, w, k, s, 0, 1, true);
+      launch_l3<T>(op, w, k, s);
+      launch_l1<T>(op, w, k + 1, s);
+    } else {
+      launch_b<T>(op, w, k, s);
+      launch_a<T>(op, w, k + 1, s);
+    }
+  }
+}
+
+bool graphs_enabled() {
+  const char* e = std::getenv("PYBELLA_AMD_NO_GRAPH");
+  return !(e && e[0] == '1');
+}
+
+template <typename T>
+int run_chunk(pb_op2d* op, Work<T>& w, int c, bool exact, hipStream_t s) {
+  if (graphs_enabled() && s != nullptr) {
+    const int key = c * 16 + (exact ? 1 : 0) + 2 * op->ry;
+    auto it = w.graphs.find(key);
+    if (it == w.graphs.end()) {
+      hipGraph_t gr = nullptr;
+      hipGraphExec_t ex = nullptr;
+      PB_HIP(op->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      launch_chunk<T>(op, w, c, exact, s);
+      PB_HIP(op->ctx, hipStreamEndCapture(s, &gr));
+      PB_HIP(op->ctx, hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(gr);
+      it = w.graphs.emplace(key, ex).first;
+    }
+    PB_HIP(op->ctx, hipGraphLaunch(it->second, s));
+  } else {
+    launch_chunk<T>(op, w, c, exact, s);
+  }
+  PB_LAUNCH_CHECK(op->ctx);
+  return PB_OK;
+}
+
+// One Krylov run: solves A x = b (b, x internal T / FP64 buffers of ``w``).
+// x0: optional FP64 initial guess (device); result in w.x.
+template <typename T>
+int krylov_run(pb_op2d* op, Work<T>& w, const T* b, const double* x0, double tol, int maxiter, int legacy,
+               int check_every, hipStream_t s, KState* out, bool exact = false) {
+  pb_ctx* ctx = op->ctx;
+  KState init{};
+  init.tol = tol;
+  init.maxiter = maxiter;
+  init.legacy = legacy;
+  init.status = ST_RUN;
+  init.phase = PH_INIT;
+  op->hstate[0] = init;
+  // the first launch of an iteration reads S0 (fast) or S2 (exact)
+  PB_HIP(ctx, hipMemcpyAsync(w.S + (exact ? 2 : 0), op->hstate, sizeof(KState), hipMemcpyHostToDevice, s));
+  {
+    const dim3 grd = stencil_grid(op, op->ry);
+    const OpDev<T> d = opdev<T>(op);
+    InitIn<T> in{x0, x0 != nullptr};
+    InitOut<T> o{b, w.r[0], w.rt, w.p[0], w.v[0], w.t, w.x, x0, x0 != nullptr, 0.0, 0.0};
+    PB_DISPATCH(k_init, T, op, grd, s, d, in, o, exact ? w.PC : w.PB);
+  }
+  if (exact) launch_l1<T>(op, w, 1, s);
+  else launch_a<T>(op, w, 1, s);
+  PB_LAUNCH_CHECK(ctx);
+  const int chk = exact ? 0 : 1;   // slot written by the iteration's first launch
+  const int chunk = std::max(2, (check_every > 0 ? check_every : 32) & ~1);
+  long issued = 1;
+  for (;;) {
+    PB_HIP(ctx, hipMemcpyAsync(op->hstate + 1, w.S + chk, sizeof(KState), hipMemcpyDeviceToHost, s));
+    PB_HIP(ctx, hipStreamSynchronize(s));
+    const KState h = op->hstate[1];
+    if (st_final(h.status)) {
+      *out = h;
+      return PB_OK;
+    }
+    if (issued > (long)maxiter + 2L * chunk + 2)
+      return pb_fail(ctx, PB_ERR_STATE, "bicgstab: %ld iterations issued without a final state (status %d)",
+                     issued, h.status);
+    int rc = run_chunk<T>(op, w, chunk, exact, s);
+    if (rc) return rc;
+    issued += chunk;
+  }
+}
+
+int true_residual(pb_op2d* op, const double* b, const double* x, hipStream_t s, double* nrm) {
+  const dim3 grd = stencil_grid(op, 4), blk(TX, TYT);
+  const OpDev<double> d = opdev<double>(op);
+  const bool c = op->cross, h = op->center;
+  if (c && h) hipLaunchKernelGGL((k_true_res<true, true, 4>), grd, blk, 0, s, d, x, b, op->rtrue, op->PT);
+  else if (c) hipLaunchKernelGGL((k_true_res<true, false, 4>), grd, blk, 0, s, d, x, b, op->rtrue, op->PT);
+  else if (h) hipLaunchKernelGGL((k_true_res<false, true, 4>), grd, blk, 0, s, d, x, b, op->rtrue, op->PT);
+  else hipLaunchKernelGGL((k_true_res<false, false, 4>), grd, blk, 0, s, d, x, b, op->rtrue, op->PT);
+  hipLaunchKernelGGL(k_sum1, dim3(1), dim3(TX, TYT), 0, s, (const double*)op->PT, (int)(grd.x * grd.y), op->red);
+  PB_LAUNCH_CHECK(op->ctx);
+  PB_HIP(op->ctx, hipMemcpyAsync(op->hred, op->red, sizeof(double), hipMemcpyDeviceToHost, s));
+  PB_HIP(op->ctx, hipStreamSynchronize(s));
+  *nrm = sqrt(op->hred[0]);
+  return PB_OK;
+}
+
+int info_of(const KState& h) {
+  switch (h.status) {
+    case ST_CONV: return 0;
+    case ST_MAXIT: return std::max(h.iter, 1);
+    case ST_BRK_RHO: return -10;
+    case ST_BRK_OMEGA: return -11;
+    default: return std::max(h.iter, 1);
+  }
+}
+
+// the stream solves run on: the caller's when it can be captured, else our own
+hipStream_t solve_stream(pb_ctx* ctx) {
+  if (ctx->stream != nullptr) return ctx->stream;
+  (void)hipStreamSynchronize(nullptr);
+  return ctx->own_stream;
+}
+
+}  // namespace
+
+extern "C" int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x, const pb_solve_opts* opts,
+                                 pb_solve_info* info) {
+  if (!op) return PB_ERR_ARG;
+  pb_ctx* ctx = op->ctx;
+  if (!b || !x || !opts || !info) return pb_fail(ctx, PB_ERR_ARG, "pb_bicgstab_solve: null argument");
+  if (!op->assembled) return pb_fail(ctx, PB_ERR_STATE, "pb_bicgstab_solve: operator not assembled");
+  if (!(opts->tol >= 0) || opts->maxiter < 1)
+    return pb_fail(ctx, PB_ERR_ARG, "pb_bicgstab_solve: bad tol/maxiter");
+  int rc = alloc_work(op);
+  if (rc) return rc;
+  hipStream_t s = solve_stream(ctx);
+  const long n = op->n;
+  const int nb = elem_blocks(n);
+  pb_solve_info out{};
+  if (opts->mode == PB_SOLVE_F64 || opts->mode == PB_SOLVE_F64_FAST) {
+    KState h{};
+    rc = krylov_run<double>(op, op->w64, b, opts->x0_given ? x : nullptr, opts->tol, opts->maxiter,
+                            opts->x0_given ? 0 : 1, opts->check_every, s, &h, opts->mode == PB_SOLVE_F64);
+    if (rc) return rc;
+    PB_HIP(ctx, hipMemcpyAsync(x, op->w64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    out.info = info_of(h);
+    out.iters = h.iter;
+    out.outer = 1;
+    out.half_step = h.half;
+    out.resid = h.resid;
+    out.bnrm2 = h.bnrm2;
+  } else if (opts->mode == PB_SOLVE_MIXED) {
+    const double inner = opts->inner_tol > 0 ? opts->inner_tol : 1e-5;
+    const int max_outer = opts->max_outer > 0 ? opts->max_outer : 10;
+    if (!opts->x0_given) hipLaunchKernelGGL(k_fill_const, dim3(nb), dim3(256), 0, s, n, x, 0.0);
+    double bn = 0.0, rn = 0.0;
+    rc = true_residual(op, b, op->zero, s, &bn);
+    if (rc) return rc;
+    rc = true_residual(op, b, x, s, &rn);   // op->rtrue = b - A x
+    if (rc) return rc;
+    out.bnrm2 = bn;
+    const double target = opts->tol * bn;
+    int outer = 0, iters = 0, last_info = 0;
+    KState h{};
+    bool fallback = false;
+    Work<float>& w = op->w32;
+    for (outer = 0; outer < max_outer && rn > target; ++outer) {
+      hipLaunchKernelGGL((k_cast<double, float>), dim3(nb), dim3(256), 0, s, n, (const double*)op->rtrue, w.b);
+      rc = krylov_run<float>(op, w, w.b, nullptr, inner, opts->maxiter, 0, opts->check_every, s, &h);
+      if (rc) return rc;
+      iters += h.iter;
+      last_info = info_of(h);
+      const double rn_old = rn;
+      hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, s, n, 1.0, (const double*)w.x, x);
+      rc = true_residual(op, b, x, s, &rn);
+      if (rc) return rc;
+      if (!(rn < rn_old)) {   // no progress: undo the correction
+        hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, s, n, -1.0, (const double*)w.x, x);
+        rc = true_residual(op, b, x, s, &rn);
+        if (rc) return rc;
+      }
+      if (!(rn < 0.5 * rn_old)) {
+        fallback = true;
+        break;
+      }
+    }
+    if (fallback && rn > target) {
+      // finish in FP64 from the current iterate: solve A d = r_true to ||.|| <= target
+      KState h64{};
+      PB_HIP(ctx, hipMemcpyAsync(op->xcorr, op->rtrue, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      rc = krylov_run<double>(op, op->w64, op->xcorr, nullptr, target / std::max(rn, 1e-300), opts->maxiter, 0,
+                              opts->check_every, s, &h64);
+      if (rc) return rc;
+      iters += h64.iter;
+      last_info = info_of(h64);
+      hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, s, n, 1.0, (const double*)op->w64.x, x);
+      rc = true_residual(op, b, x, s, &rn);
+      if (rc) return rc;
+      ++outer;
+    }
+    out.info = rn <= target ? 0 : (last_info < 0 ? last_info : std::max(iters, 1));
+    out.iters = iters;
+    out.outer = outer;
+    out.half_step = h.half;
+    out.resid = rn;
+  } else {
+    return pb_fail(ctx, PB_ERR_ARG, "pb_bicgstab_solve: mode %d not available on this handle", opts->mode);
+  }
+  double tr = 0.0;
+  rc = true_residual(op, b, x, s, &tr);
+  if (rc) return rc;
+  out.true_rel_resid = out.bnrm2 > 0 ? tr / out.bnrm2 : tr;
+  *info = out;
+  return out.info;
+}
+
+namespace {
+template <typename T>
+int time_kernel_t(pb_op2d* op, int kernel, int reps, hipStream_t s, double* avg_us) {
+  pb_ctx* ctx = op->ctx;
+  Work<T>& w = work_of<T>(op);
+  const long n = op->n;
+  const int nb = elem_blocks(n);
+  const dim3 sg = stencil_grid(op, op->ry);
+  const int nbs = (int)(sg.x * sg.y);
+  T* vecs[] = {w.r[0], w.r[1], w.p[0], w.p[1], w.v[0], w.v[1], w.t, w.rt, w.b};
+  for (int k = 0; k < 9; ++k) hipLaunchKernelGGL(k_fill_pattern<T>, dim3(nb), dim3(256), 0, s, n, vecs[k], 0.7 * k);
+  hipLaunchKernelGGL(k_fill_pattern<double>, dim3(nb), dim3(256), 0, s, n, w.x, 0.1);
+  // partials that keep the state running: B totals (1, .5, 1, .3, .1), A total 1
+  std::vector<double> pb(5L * nbs), pa(nbs, 1.0 / nbs);
+  const double tb[5] = {1.0, 0.5, 1.0, 0.3, 0.1};
+  for (int k = 0; k < 5; ++k)
+    for (int b = 0; b < nbs; ++b) pb[(long)b * 5 + k] = tb[k] / nbs;
+  PB_HIP(ctx, hipMemcpyAsync(w.PB, pb.data(), sizeof(double) * pb.size(), hipMemcpyHostToDevice, s));
+  PB_HIP(ctx, hipMemcpyAsync(w.PA, pa.data(), sizeof(double) * pa.size(), hipMemcpyHostToDevice, s));
+  KState st{};
+  st.rho = 1.0;
+  st.alpha = 0.5;
+  st.omega = 0.5;
+  st.beta = 0.5;
+  st.atol = -1.0;
+  st.tol = 0.0;
+  st.maxiter = 0x7fffffff;
+  st.status = ST_RUN;
+  st.x_upd = 1;
+  KState sts[2] = {st, st};
+  sts[0].phase = PH_B;   // read by A
+  sts[1].phase = PH_A;   // read by B
+  PB_HIP(ctx, hipMemcpyAsync(w.S, sts, sizeof(sts), hipMemcpyHostToDevice, s));
+  PB_HIP(ctx, hipStreamSynchronize(s));
+  auto launch = [&]() {
+    switch (kernel) {
+      case 0:
+        pb_op2d_matvec(op, w.p[0], w.v[1], sizeof(T) == 8 ? PB_F64 : PB_F32);
+        break;
+      case 1: launch_a<T>(op, w, 2, s); break;
+      case 2: launch_b<T>(op, w, 1, s); break;
+      default: {
+        const dim3 grd = stencil_grid(op, 4), blk(TX, TYT);
+        const OpDev<double> d = opdev<double>(op);
+        hipLaunchKernelGGL((k_true_res<false, false, 4>), grd, blk, 0, s, d, w.x, op->zero, op->rtrue, op->PT);
+      }
+    }
+  };
+  for (int i = 0; i < 3; ++i) launch();
+  hipEvent_t e0, e1;
+  PB_HIP(ctx, hipEventCreate(&e0));
+  PB_HIP(ctx, hipEventCreate(&e1));
+  PB_HIP(ctx, hipEventRecord(e0, s));
+  for (int i = 0; i < reps; ++i) launch();
+  PB_HIP(ctx, hipEventRecord(e1, s));
+  PB_HIP(ctx, hipEventSynchronize(e1));
+  float ms = 0.f;
+  PB_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  PB_LAUNCH_CHECK(ctx);
+  KState chk[2];
+  PB_HIP(ctx, hipMemcpy(chk, w.S, sizeof(chk), hipMemcpyDeviceToHost));
+  if ((kernel == 1 && chk[1].status != ST_RUN) || (kernel == 2 && chk[0].status != ST_RUN))
+    return pb_fail(ctx, PB_ERR_STATE, "pb_op2d_time_kernel: timing state left the running status");
+  *avg_us = 1000.0 * ms / reps;
+  return PB_OK;
+}
+}  // namespace
+
+extern "C" int pb_op2d_time_kernel(pb_op2d* op, int kernel, int precision, int reps, double* avg_us) {
+  if (!op) return PB_ERR_ARG;
+  if (!avg_us || reps < 1 || kernel < 0 || kernel > 3)
+    return pb_fail(op->ctx, PB_ERR_ARG, "pb_op2d_time_kernel: bad argument");
+  if (!op->assembled) return pb_fail(op->ctx, PB_ERR_STATE, "pb_op2d_time_kernel: operator not assembled");
+  int rc = alloc_work(op);
+  if (rc) return rc;
+  hipStream_t s = solve_stream(op->ctx);
+  if (kernel == 0) {
+    // the plain matvec runs on the context stream
+    s = op->ctx->stream;
+  }
+  if (precision == PB_F64) return time_kernel_t<double>(op, kernel, reps, s, avg_us);
+  if (precision == PB_F32) return time_kernel_t<float>(op, kernel, reps, s, avg_us);
+  return pb_fail(op->ctx, PB_ERR_ARG, "pb_op2d_time_kernel: unknown precision %d", precision);
+}
+
+extern "C" int pb_op2d_set_tile(pb_op2d* op, int rows_per_thread) {
+  if (!op) return PB_ERR_ARG;
+  if (rows_per_thread != 2 && rows_per_thread != 4 && rows_per_thread != 8)
+    return pb_fail(op->ctx, PB_ERR_ARG, "pb_op2d_set_tile: rows per thread must be 2, 4 or 8");
+  op->ry = rows_per_thread;
+  return PB_OK;
+}

@@ -1,0 +1,283 @@
+// 2D nodal operator handle and the LDS-tiled stencil machinery (gfx950).
+//
+// The 9-point operator of lap2D_gather_new (laplacian.py:546-746) in flux
+// form.  For op-cell (j, i) (rows 0..ny, cols 0..nx; corners are nodes
+// (j-1..j, i-1..i), with the reference's wrap onto the duplicated periodic end
+// node: i-1 -> nx-2 at i=0, i+1 -> 1 at i=nx-1, likewise in y):
+//   gx = (p_tr - p_tl) + (p_br - p_bl),  gy = (p_bl - p_tl) + (p_br - p_tr)
+//   Fx = Wxx gx + Wyx gy,                Fy = Wyy gy + Wxy gx
+// with folded weights Wxx = hx cxx/(4 dx^2), Wyy = hy cyy/(4 dy^2),
+// Wyx = hy cxy/(4 dx dy), Wxy = hx cyx/(4 dx dy) (hx = wplus[0], hy = wplus[1]
+// zeroed on cells outside walls; the 1/4 is the product of the two 0.5
+// factors of laplacian.py:721-735).  A node sums its four cells:
+//   (Fx_TR + Fx_BR - Fx_TL - Fx_BL) + (Fy_BL + Fy_BR - Fy_TL - Fy_TR) + hc p,
+// times diag_inv — term by term Dxx + Dyy + Dyx + Dxy + hcenter p
+// (laplacian.py:731-739).
+//
+// Tiles are 64 nodes wide (one wavefront per row) and 4*RY rows tall; a block
+// of 64x4 threads loads the (rows+2) x 66 halo tile of its input expression
+// into LDS once (coalesced: one row of 66 values per wavefront pass), forms
+// every cell flux of the tile once into LDS, then each thread gathers RY
+// nodes.  Halo re-reads between neighbouring tiles are the only redundancy.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <vector>
+
+#include "pb_common.hpp"
+
+namespace pb2d {
+
+constexpr int TX = 64;    // tile width = one wavefront
+constexpr int TYT = 4;    // thread rows per block
+constexpr int NTHR = TX * TYT;
+
+template <typename T>
+struct OpDev {
+  const T* wxx;
+  const T* wyy;
+  const T* wyx;
+  const T* wxy;
+  const T* hc;
+  const T* dinv;
+  int nx, ny;
+};
+
+// LDS of one tile: halo of the input, cell fluxes.
+template <typename T, int RY>
+struct TileLds {
+  static constexpr int TY = TYT * RY;
+  T sp[(TY + 2) * (TX + 2)];
+  T sfx[(TY + 1) * (TX + 1)];
+  T sfy[(TY + 1) * (TX + 1)];
+};
+
+// Apply the operator on tile (bx, by).  in(g) evaluates the input at node g;
+// out(g, y, center_value) consumes the result y = diag_inv * (A in)(g).
+template <typename T, bool CROSS, bool CENTER, int RY, class In, class Out>
+__device__ __forceinline__ void stencil_tile(const OpDev<T>& op, int bx, int by, const In& in, Out& out,
+                                             TileLds<T, RY>& L) {
+  constexpr int TY = TYT * RY;
+  constexpr int HR = TY + 2;                     // halo rows
+  constexpr int NPASS = (HR + TYT - 1) / TYT;    // row passes per thread row
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = bx * TX, j0 = by * TY;
+  const int nx = op.nx, ny = op.ny;
+  // interior columns: wavefront ty loads halo rows ty, ty+4, ... (64 contiguous
+  // nodes each, coalesced); lanes 0 and 1 also load the two edge columns.
+  {
+    int ic = i0 + tx;
+    ic = (ic == nx) ? 1 : ic;                    // periodic wrap past the last node
+    const bool col_ok = ic < nx;
+    int ie = (tx == 0) ? i0 - 1 : i0 + TX;     // edge column of lanes 0/1
+    ie = (ie < 0) ? nx - 2 : (ie == nx ? 1 : ie);
+    const bool edge_ok = tx < 2 && ie < nx;
+    T vals[NPASS], evals[NPASS];
+#pragma unroll
+    for (int q = 0; q < NPASS; ++q) {
+      const int ly = ty + TYT * q;
+      int j = j0 - 1 + ly;
+      j = (j < 0) ? ny - 2 : (j == ny ? 1 : j);
+      const bool row_ok = ly < HR && j < ny;
+      vals[q] = (row_ok && col_ok) ? in((long)j * nx + ic) : T(0);
+      evals[q] = (row_ok && edge_ok) ? in((long)j * nx + ie) : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < NPASS; ++q) {
+      const int ly = ty + TYT * q;
+      if (ly < HR) {
+        L.sp[ly * (TX + 2) + tx + 1] = vals[q];
+        if (tx < 2) L.sp[ly * (TX + 2) + (tx == 0 ? 0 : TX + 1)] = evals[q];
+      }
+    }
+  }
+  __syncthreads();
+  const int tid = tx + TX * ty;
+  for (int k = tid; k < (TY + 1) * (TX + 1); k += NTHR) {
+    const int cy = k / (TX + 1), cx = k - cy * (TX + 1);
+    const int j = j0 + cy, i = i0 + cx;
+    T fx = T(0), fy = T(0);
+    if (j <= ny && i <= nx) {
+      const long c = (long)j * (nx + 1) + i;
+      const T* sp = L.sp + cy * (TX + 2) + cx;
+      const T tl = sp[0], tr = sp[1], bl = sp[TX + 2], br = sp[TX + 3];
+      const T gx = (tr - tl) + (br - bl);
+      const T gy = (bl - tl) + (br - tr);
+      fx = op.wxx[c] * gx;
+      fy = op.wyy[c] * gy;
+      if (CROSS) {
+        fx += op.wyx[c] * gy;
+        fy += op.wxy[c] * gx;
+      }
+    }
+    L.sfx[k] = fx;
+    L.sfy[k] = fy;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int ly = ty + TYT * r, lx = tx;
+    const int j = j0 + ly, i = i0 + lx;
+    if (j < ny && i < nx) {
+      const long g = (long)j * nx + i;
+      const int c0 = ly * (TX + 1) + lx, c1 = c0 + (TX + 1);
+      T acc = ((L.sfx[c0 + 1] + L.sfx[c1 + 1]) - (L.sfx[c0] + L.sfx[c1])) +
+              ((L.sfy[c1] + L.sfy[c1 + 1]) - (L.sfy[c0] + L.sfy[c0 + 1]));
+      const T pc = L.sp[(ly + 1) * (TX + 2) + lx + 1];
+      if (CENTER) acc += op.hc[g] * pc;
+      out(g, op.dinv[g] * acc, pc);
+    }
+  }
+}
+
+// Sum ``nb`` block partials of NV values (record layout [b * NV + k]) in a
+// fixed order; the result is valid in every thread.  sred: NV * (NTHR/64).
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* P, int nb, double (&tot)[NV], double* sred) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthr = blockDim.x * blockDim.y;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = 0.0;
+  for (int b = tid; b < nb; b += nthr) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tot[k] += P[(long)b * NV + k];
+  }
+  block_sum<NV>(tot, sred);
+  __shared__ double bc[8];
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) bc[k] = tot[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = bc[k];
+  __syncthreads();
+}
+
+// Block sum of NV thread values, written as this block's record (plain stores).
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double* sred, double* P, int nb, int bid) {
+  block_sum<NV>(v, sred);
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  (void)nb;
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) P[(long)bid * NV + k] = v[k];
+  }
+}
+
+// Device state of a Krylov run, double-buffered: a kernel reads S[in] and its
+// block 0 writes S[out], so no block ever reads a value written in its own
+// launch.  Scalars are derived redundantly (identically) in every block from
+// the previous launch's block partials, so no grid-wide fence is needed.
+struct KState {
+  double rho, alpha, omega, beta;
+  double atol, bnrm2, resid, tol, rnorm0;
+  int iter, maxiter, status, half, legacy, phase, x_upd;
+};
+
+enum {
+  ST_RUN = 0,
+  ST_CONV = 1,
+  ST_MAXIT = 3,
+  ST_BRK_RHO = -10,
+  ST_BRK_OMEGA = -11,
+  // pending: the final x update is still to be applied by the next launch
+  ST_P_CONV = 10,
+  ST_P_HALF = 11,
+  ST_P_MAXIT = 12,
+  ST_P_BRK_RHO = 13,
+  ST_P_BRK_OMEGA = 14,
+};
+enum { PH_INIT = 0, PH_A = 1, PH_B = 2, PH_L3 = 3 };
+
+__host__ __device__ inline bool st_final(int s) {
+  return s == ST_CONV || s == ST_MAXIT || s == ST_BRK_RHO || s == ST_BRK_OMEGA;
+}
+
+template <typename T>
+struct Work {
+  T* r[2] = {nullptr, nullptr};
+  T* p[2] = {nullptr, nullptr};
+  T* v[2] = {nullptr, nullptr};
+  T* t = nullptr;
+  T* rt = nullptr;
+  T* b = nullptr;          // right-hand side copy in T
+  double* x = nullptr;     // iterate (always FP64)
+  double* PA = nullptr;    // partials of the A launch (1 value)
+  double* PB = nullptr;    // partials of the init / B launch (<= 5 values)
+  double* PC = nullptr;    // partials of the exact-mode update launch (2 values)
+  KState* S = nullptr;     // [3]
+  std::map<int, hipGraphExec_t> graphs;   // (variant, chunk) key -> instantiated graph
+};
+
+}  // namespace pb2d
+
+struct pb_op2d {
+  pb_ctx* ctx = nullptr;
+  pb_grid2d g{};
+  long n = 0;      // nodes
+  long nc = 0;     // op-cells
+  bool cross = false, center = true, assembled = false;
+  int ry = 4;      // tile rows per thread (4 or 8)
+  double* wxx = nullptr;
+  double* wyy = nullptr;
+  double* wyx = nullptr;
+  double* wxy = nullptr;
+  double* hc = nullptr;
+  double* dinv = nullptr;
+  float* wxx32 = nullptr;
+  float* wyy32 = nullptr;
+  float* wyx32 = nullptr;
+  float* wxy32 = nullptr;
+  float* hc32 = nullptr;
+  float* dinv32 = nullptr;
+  double* coeff = nullptr;   // cells scratch
+  double* rYpow = nullptr;   // cells scratch
+  // solver workspace (allocated at the first solve)
+  bool have_work = false;
+  pb2d::Work<double> w64;
+  pb2d::Work<float> w32;
+  double* rtrue = nullptr;   // FP64 true residual
+  double* xcorr = nullptr;   // FP64 correction (mixed mode)
+  double* zero = nullptr;    // FP64 zeros
+  double* PT = nullptr;      // partials of true-residual launches
+  double* red = nullptr;     // [2] true-residual result slots
+  pb2d::KState* hstate = nullptr;   // pinned host mirror
+  double* hred = nullptr;           // pinned host mirror of red
+  std::vector<void*> allocs;
+};
+
+namespace pb2d {
+
+template <typename T>
+inline OpDev<T> opdev(const pb_op2d* op);
+template <>
+inline OpDev<double> opdev<double>(const pb_op2d* op) {
+  return OpDev<double>{op->wxx, op->wyy, op->wyx, op->wxy, op->hc, op->dinv, op->g.nx, op->g.ny};
+}
+template <>
+inline OpDev<float> opdev<float>(const pb_op2d* op) {
+  return OpDev<float>{op->wxx32, op->wyy32, op->wyx32, op->wxy32, op->hc32, op->dinv32, op->g.nx, op->g.ny};
+}
+
+inline dim3 stencil_grid(const pb_op2d* op, int ry) {
+  return dim3(ceil_div(op->g.nx, TX), ceil_div(op->g.ny, TYT * ry));
+}
+
+template <typename T>
+int dmalloc(pb_op2d* op, T** p, long count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)std::max<long>(count, 1));
+  if (e != hipSuccess)
+    return pb_fail(op->ctx, PB_ERR_HIP, "hipMalloc(%ld bytes) failed: %s", (long)(sizeof(T) * count),
+                   hipGetErrorString(e));
+  op->allocs.push_back(q);
+  *p = (T*)q;
+  return PB_OK;
+}
+
+inline int elem_blocks(long n) { return (int)std::min<long>((n + 1023) / 1024, 2048); }
+
+}  // namespace pb2d
