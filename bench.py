@@ -30,13 +30,17 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, public spec
 
-# Algorithmic bytes per node of each solver kernel (FP32 Krylov vectors, zero
-# Coriolis, zero centre term): every distinct array read once and written once.
-#   k1: p-update + matvec + rt.v : read r, p_old, v_old, rt, Wxx, Wyy, dinv; write p, v  -> 9 words
-#   k2: s + matvec + 3 dots       : read r, v, Wxx, Wyy, dinv; write t                  -> 6 words
-#   k3: x/r update + 2 dots       : read x, r, rt, p, v, t; write x, r                   -> 8 words
-KERNEL_WORDS = {0: 5, 1: 9, 2: 6, 3: 8}
-KERNEL_NAMES = {0: "matvec", 1: "bicg_pupdate_matvec_dot", 2: "bicg_s_matvec_dots", 3: "bicg_xr_update_dots"}
+# Algorithmic bytes per node of each solver kernel (zero Coriolis, zero centre
+# term; cell arrays counted as one word per node): every distinct array read
+# once and written once, in words of the Krylov precision, plus FP64 x bytes.
+#   0 matvec:                read p, Wxx, Wyy, dinv; write y                         -> 5 words
+#   1 A launch (fast):       read r, p, v, t, rt, Wxx, Wyy, dinv; write r, p, v      -> 11 words
+#                            + x read and written in FP64                            -> +16 B
+#   2 B launch:              read r, v, rt, Wxx, Wyy, dinv; write t                  -> 7 words
+#   3 FP64 true residual:    read x, b, Wxx, Wyy, dinv; write r                      -> 6 words
+KERNEL_WORDS = {0: 5, 1: 11, 2: 7, 3: 6}
+KERNEL_XBYTES = {1: 16}
+KERNEL_NAMES = {0: "matvec", 1: "bicg_fast_A_pupdate_matvec", 2: "bicg_B_s_matvec_dots", 3: "true_residual_f64"}
 
 
 def log(*a):
@@ -101,6 +105,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=300)
     ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--kernel-only", type=int, default=None,
+                    help="only time solver kernel K (0 matvec, 1 fast A launch, 2 B launch, 3 FP64 true "
+                         "residual) on the workload, for profiler runs")
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--tile", type=int, default=4, help="rows per thread for --kernel-only")
+    ap.add_argument("--copy-probe-mib", type=int, default=None,
+                    help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -109,6 +120,16 @@ def main():
 
     import torch
     from pybella_amd import second_projection as sp
+
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 8, "launches": 10}))
+        return
 
     log("building workload 1025x513 ...")
     state, rhs = build_workload()
@@ -123,6 +144,19 @@ def main():
     b = torch.empty(grid.ny, grid.nx, dtype=torch.float64, device="cuda")
     x = torch.empty_like(b)
     torch.cuda.synchronize()
+
+    if args.kernel_only is not None:
+        op.assemble(phys, dt, rho, rhoY, rho, rhoY, dS)
+        op.set_tile(args.tile)
+        us = op.time_kernel(args.kernel_only, args.precision, args.kernel_reps)
+        n = grid.nx * grid.ny
+        words = KERNEL_WORDS[args.kernel_only]
+        wb = 8 if args.precision == "f64" else 4
+        alg = words * wb * n + (KERNEL_XBYTES.get(args.kernel_only, 0) * n)
+        print(json.dumps({"kernel": KERNEL_NAMES[args.kernel_only], "precision": args.precision,
+                          "tile_rows_per_thread": args.tile, "avg_us": us, "alg_bytes": alg,
+                          "alg_GBps": alg / (us * 1e-6) / 1e9}))
+        return
 
     def step(mode):
         op.assemble(phys, dt, rho, rhoY, rho, rhoY, dS)
@@ -168,14 +202,23 @@ def main():
     prec = "f32" if mode == "mixed" else "f64"
     wbytes = 4 if prec == "f32" else 8
     n = grid.nx * grid.ny
-    kt = {k: op.time_kernel(k, prec, args.kernel_reps) for k in (0, 1, 2, 3)}
-    per_iter = {k: kt[k] for k in (1, 2, 3)}
+    kt = {k: op.time_kernel(k, prec, args.kernel_reps) for k in (0, 1, 2, 3)}   # at the tuned tile
+    per_iter = {k: kt[k] for k in (1, 2)}
     dom = max(per_iter, key=per_iter.get)
-    alg = KERNEL_WORDS[dom] * wbytes * n
+    alg = KERNEL_WORDS[dom] * wbytes * n + KERNEL_XBYTES.get(dom, 0) * n
     achieved = alg / (kt[dom] * 1e-6) / 1e9
+    traffic = None
+    try:   # HBM bytes per launch from the committed rocprofv3 counter summary
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_counters.json")) as f:
+            hc = json.load(f)
+        rec = hc["kernels"].get(f"{KERNEL_NAMES[dom]}/{prec}/{tiles[mode]['rows_per_thread']}")
+        if rec:
+            traffic = int(1024 * (rec["fetch_kib"] * hc["fetch_scale"] + rec["write_kib"] * hc["write_scale"]))
+    except (OSError, KeyError, ValueError):
+        traffic = None
     roof = {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "avg_us": round(kt[dom], 3), "alg_bytes": alg,
+            "traffic": traffic, "avg_us": round(kt[dom], 3), "alg_bytes": alg,
             "kernels_us": {KERNEL_NAMES[k]: round(v, 3) for k, v in kt.items()}}
 
     cpu = None

@@ -37,24 +37,30 @@ __device__ __forceinline__ int nblocks() { return gridDim.x * gridDim.y; }
 __device__ __forceinline__ int thread_id() { return threadIdx.x + threadIdx.y * blockDim.x; }
 
 // ---------------------------------------------------------------------------
-// init: r0 = b - A x0, rt = r0, p = v = t = 0, x = x0 (or 0); partials (r.r, b.b)
+// init: r0 = b - A x0, rt = r0, p = v = t = 0, x = x0 (or 0); records
+// (r.r, b.b, 0...) of NREC values
 // ---------------------------------------------------------------------------
+struct Empty {};
+
 template <typename T>
 struct InitIn {
+  using Raw = double;
   const double* x0;
   int given;
-  __device__ T operator()(long g) const { return given ? (T)x0[g] : T(0); }
+  __device__ Raw load(long g) const { return given ? x0[g] : 0.0; }
+  __device__ T combine(const Raw& r) const { return (T)r; }
 };
 template <typename T>
 struct InitOut {
+  using Pre = T;
   const T* b;
   T *r, *rt, *p, *v, *t;
   double* x;
   const double* x0;
   int given;
   double rr, bb;
-  __device__ void operator()(long g, T y, T) {
-    const T bg = b[g];
+  __device__ Pre prefetch(long g) const { return b[g]; }
+  __device__ void operator()(long g, T y, T, const Pre& bg) {
     const T rg = bg - y;
     r[g] = rg;
     rt[g] = rg;
@@ -67,67 +73,33 @@ struct InitOut {
   }
 };
 
-template <typename T, bool CROSS, bool CENTER, int RY>
-__global__ __launch_bounds__(NTHR) void k_init(OpDev<T> op, InitIn<T> in, InitOut<T> out, double* PB) {
+template <int NREC, typename T, bool CROSS, bool CENTER, int RY>
+__device__ __forceinline__ void init_body(OpDev<T>& op, InitIn<T>& in, InitOut<T>& out, double* P) {
   __shared__ TileLds<T, RY> L;
   __shared__ double sred[8 * (NTHR / kWave)];
   out.rr = 0.0;
   out.bb = 0.0;
-  stencil_tile<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L);
-  double v[2] = {out.rr, out.bb};
-  write_partials<2>(v, sred, PB, nblocks(), block_id());
+  stencil_tile2<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, [] { return true; });
+  double v[NREC];
+#pragma unroll
+  for (int k = 0; k < NREC; ++k) v[k] = 0.0;
+  v[0] = out.rr;
+  v[1] = out.bb;
+  write_partials<NREC>(v, sred, P, nblocks(), block_id());
+}
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_init5(OpDev<T> op, InitIn<T> in, InitOut<T> out, double* P) {
+  init_body<5, T, CROSS, CENTER, RY>(op, in, out, P);
+}
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_init2(OpDev<T> op, InitIn<T> in, InitOut<T> out, double* P) {
+  init_body<2, T, CROSS, CENTER, RY>(op, in, out, P);
 }
 
 // ---------------------------------------------------------------------------
-// launch A(k)
+// scalar steps shared by every block
 // ---------------------------------------------------------------------------
-template <typename T>
-struct IterPtrs {
-  const T* r_prev;
-  const T* p_prev;
-  const T* v_prev;
-  const T* t;
-  const T* rt;
-  T* r_new;
-  T* p_new;
-  T* v_new;
-  double* x;
-};
-
-template <typename T>
-struct AIn {
-  const T *r, *p, *v, *t;
-  T alpha, omega, beta;
-  __device__ T operator()(long g) const {
-    const T s = r[g] - alpha * v[g];
-    const T rn = s - omega * t[g];
-    return rn + beta * (p[g] - omega * v[g]);
-  }
-};
-
-template <typename T>
-struct AOut {
-  IterPtrs<T> q;
-  T alpha, omega;
-  double alpha_d, omega_d;
-  int xupd;
-  double rv;
-  __device__ void operator()(long g, T y, T pn) {
-    const T s = q.r_prev[g] - alpha * q.v_prev[g];
-    const T rn = s - omega * q.t[g];
-    if (xupd) {
-      const double xg = q.x[g] + alpha_d * (double)q.p_prev[g];
-      q.x[g] = xg + omega_d * (double)s;
-    }
-    q.r_new[g] = rn;
-    q.p_new[g] = pn;
-    q.v_new[g] = y;
-    rv += (double)q.rt[g] * (double)y;
-  }
-};
-
-// The scalar step shared by every block of A: returns the new state from the
-// previous state and the previous launch's totals.
 __device__ __forceinline__ KState scalars_after_init(KState st, double rr, double bb) {
   st.bnrm2 = sqrt(bb);
   st.rnorm0 = sqrt(rr);
@@ -188,135 +160,6 @@ __device__ __forceinline__ KState scalars_after_b(KState st, const double (&tot)
   return st;
 }
 
-__device__ __forceinline__ int finalize(int pending) {
-  switch (pending) {
-    case ST_P_CONV: return ST_CONV;
-    case ST_P_HALF: return ST_CONV;
-    case ST_P_MAXIT: return ST_MAXIT;
-    case ST_P_BRK_RHO: return ST_BRK_RHO;
-    case ST_P_BRK_OMEGA: return ST_BRK_OMEGA;
-    default: return pending;
-  }
-}
-
-template <typename T, bool CROSS, bool CENTER, int RY>
-__global__ __launch_bounds__(NTHR) void k_iter_a(OpDev<T> op, IterPtrs<T> q, const KState* Sin, KState* Sout,
-                                                 const double* PB, double* PA) {
-  __shared__ TileLds<T, RY> L;
-  __shared__ double sred[8 * (NTHR / kWave)];
-  const int tid = thread_id(), bid = block_id(), nb = nblocks();
-  KState st = *Sin;
-  if (st_final(st.status)) {
-    if (bid == 0 && tid == 0) *Sout = st;
-    return;
-  }
-  if (st.phase == PH_INIT) {
-    double tot[2];
-    sum_partials<2>(PB, nb, tot, sred);
-    st = scalars_after_init(st, tot[0], tot[1]);
-  } else {
-    double tot[5];
-    sum_partials<5>(PB, nb, tot, sred);
-    st = scalars_after_b(st, tot);
-  }
-  st.phase = PH_A;
-  if (st.status != ST_RUN) {
-    if (st.status >= ST_P_CONV) {   // final x update of a finished run
-      const bool half = st.status == ST_P_HALF;
-      const T alpha = (T)st.alpha;
-      constexpr int TY = TYT * RY;
-#pragma unroll
-      for (int r = 0; r < RY; ++r) {
-        const int j = blockIdx.y * TY + threadIdx.y + TYT * r, i = blockIdx.x * TX + threadIdx.x;
-        if (j < op.ny && i < op.nx) {
-          const long g = (long)j * op.nx + i;
-          const double xg = q.x[g] + st.alpha * (double)q.p_prev[g];
-          if (half) {
-            q.x[g] = xg;
-          } else {
-            const T s = q.r_prev[g] - alpha * q.v_prev[g];
-            q.x[g] = xg + st.omega * (double)s;
-          }
-        }
-      }
-      if (half) st.half = 1;
-      st.status = finalize(st.status);
-    }
-    if (bid == 0 && tid == 0) *Sout = st;
-    return;
-  }
-  const T alpha = (T)st.alpha, omega = (T)st.omega, beta = (T)st.beta;
-  AIn<T> in{q.r_prev, q.p_prev, q.v_prev, q.t, alpha, omega, beta};
-  AOut<T> out{q, alpha, omega, st.alpha, st.omega, st.x_upd, 0.0};
-  stencil_tile<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L);
-  double v[1] = {out.rv};
-  write_partials<1>(v, sred, PA, nb, bid);
-  if (bid == 0 && tid == 0) *Sout = st;
-}
-
-// ---------------------------------------------------------------------------
-// launch B(k)
-// ---------------------------------------------------------------------------
-template <typename T>
-struct BIn {
-  const T *r, *v;
-  T alpha;
-  __device__ T operator()(long g) const { return r[g] - alpha * v[g]; }
-};
-template <typename T>
-struct BOut {
-  T* t;
-  const T* rt;
-  double ss, ts, tt, rts, rtt;
-  __device__ void operator()(long g, T y, T s) {
-    t[g] = y;
-    const double sd = (double)s, yd = (double)y, rd = (double)rt[g];
-    ss += sd * sd;
-    ts += yd * sd;
-    tt += yd * yd;
-    rts += rd * sd;
-    rtt += rd * yd;
-  }
-};
-
-template <typename T, bool CROSS, bool CENTER, int RY>
-__global__ __launch_bounds__(NTHR) void k_iter_b(OpDev<T> op, const T* r, const T* v, const T* rt, T* t,
-                                                 const KState* Sin, KState* Sout, const double* PA, double* PB) {
-  __shared__ TileLds<T, RY> L;
-  __shared__ double sred[8 * (NTHR / kWave)];
-  const int tid = thread_id(), bid = block_id(), nb = nblocks();
-  KState st = *Sin;
-  if (st.status != ST_RUN) {
-    if (bid == 0 && tid == 0) *Sout = st;
-    return;
-  }
-  double tot[1];
-  sum_partials<1>(PA, nb, tot, sred);
-  const double a = st.rho / tot[0];
-  st.phase = PH_B;
-  if (tot[0] == 0.0 || !isfinite(a)) {
-    st.status = ST_BRK_OMEGA;
-    if (bid == 0 && tid == 0) *Sout = st;
-    return;
-  }
-  st.alpha = a;
-  BIn<T> in{r, v, (T)a};
-  BOut<T> out{t, rt, 0.0, 0.0, 0.0, 0.0, 0.0};
-  stencil_tile<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L);
-  double vals[5] = {out.ss, out.ts, out.tt, out.rts, out.rtt};
-  write_partials<5>(vals, sred, PB, nb, bid);
-  if (bid == 0 && tid == 0) *Sout = st;
-}
-
-// ---------------------------------------------------------------------------
-// exact variant: three launches per iteration with rho = rt.r and ||r||
-// formed from the stored r as the reference does.
-//   L1(k): beta from L3's partials (or init); p_k = beta (p - omega v) + r,
-//          v_k = A p_k; partials rt.v
-//   L2(k): the B launch (alpha; s = r - alpha v, t = A s; dots)
-//   L3(k): ||s|| test, omega; x += alpha p + omega s; r = s - omega t;
-//          partials rt.r, r.r
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ KState scalars_after_l3(KState st, double rtr, double rr) {
   const double rnrm = sqrt(rr);
   st.iter += 1;
@@ -339,21 +182,231 @@ __device__ __forceinline__ KState scalars_after_l3(KState st, double rtr, double
   return st;
 }
 
+__device__ __forceinline__ int finalize(int pending) {
+  switch (pending) {
+    case ST_P_CONV: return ST_CONV;
+    case ST_P_HALF: return ST_CONV;
+    case ST_P_MAXIT: return ST_MAXIT;
+    case ST_P_BRK_RHO: return ST_BRK_RHO;
+    case ST_P_BRK_OMEGA: return ST_BRK_OMEGA;
+    default: return pending;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fast variant, launch A(k)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct IterPtrs {
+  const T* r_prev;
+  const T* p_prev;
+  const T* v_prev;
+  const T* t;
+  const T* rt;
+  T* r_new;
+  T* p_new;
+  T* v_new;
+  double* x;
+};
+
+template <typename T>
+struct AIn {
+  struct Raw {
+    T r, p, v, t;
+  };
+  const T *r, *p, *v, *t;
+  T alpha, omega, beta;
+  __device__ Raw load(long g) const { return Raw{r[g], p[g], v[g], t[g]}; }
+  __device__ T combine(const Raw& q) const {
+    const T s = q.r - alpha * q.v;
+    const T rn = s - omega * q.t;
+    return rn + beta * (q.p - omega * q.v);
+  }
+};
+
+template <typename T>
+struct AOut {
+  struct Pre {
+    T r, p, v, t, rt;
+    double x;
+  };
+  IterPtrs<T> q;
+  T alpha, omega;
+  double alpha_d, omega_d;
+  int xupd;
+  double rv;
+  __device__ Pre prefetch(long g) const { return Pre{q.r_prev[g], q.p_prev[g], q.v_prev[g], q.t[g], q.rt[g], q.x[g]}; }
+  __device__ void operator()(long g, T y, T pn, const Pre& e) {
+    const T s = e.r - alpha * e.v;
+    const T rn = s - omega * e.t;
+    if (xupd) {
+      const double xg = e.x + alpha_d * (double)e.p;
+      q.x[g] = xg + omega_d * (double)s;
+    }
+    q.r_new[g] = rn;
+    q.p_new[g] = pn;
+    q.v_new[g] = y;
+    rv += (double)e.rt * (double)y;
+  }
+};
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_iter_a(OpDev<T> op, IterPtrs<T> q, const KState* Sin, KState* Sout,
+                                                 const double* PB, double* PA) {
+  __shared__ TileLds<T, RY> L;
+  __shared__ double sred[8 * (NTHR / kWave)];
+  const int tid = thread_id(), bid = block_id(), nb = nblocks();
+  AIn<T> in{q.r_prev, q.p_prev, q.v_prev, q.t, T(0), T(0), T(0)};
+  AOut<T> out{q, T(0), T(0), 0.0, 0.0, 0, 0.0};
+  KState st;
+  auto pro = [&]() -> bool {
+    st = *Sin;
+    double tot[5];
+    sum_partials<5>(PB, nb, tot, sred);
+    if (st_final(st.status)) {
+      if (bid == 0 && tid == 0) *Sout = st;
+      return false;
+    }
+    st = (st.phase == PH_INIT) ? scalars_after_init(st, tot[0], tot[1]) : scalars_after_b(st, tot);
+    st.phase = PH_A;
+    if (st.status != ST_RUN) {
+      if (st.status >= ST_P_CONV) {   // final x update of a finished run
+        const bool half = st.status == ST_P_HALF;
+        const T alpha = (T)st.alpha;
+        constexpr int TY = TYT * RY;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+          const int j = blockIdx.y * TY + threadIdx.y + TYT * r, i = blockIdx.x * TX + threadIdx.x;
+          if (j < op.ny && i < op.nx) {
+            const long g = (long)j * op.nx + i;
+            const double xg = q.x[g] + st.alpha * (double)q.p_prev[g];
+            if (half) {
+              q.x[g] = xg;
+            } else {
+              const T s = q.r_prev[g] - alpha * q.v_prev[g];
+              q.x[g] = xg + st.omega * (double)s;
+            }
+          }
+        }
+        if (half) st.half = 1;
+        st.status = finalize(st.status);
+      }
+      if (bid == 0 && tid == 0) *Sout = st;
+      return false;
+    }
+    in.alpha = out.alpha = (T)st.alpha;
+    in.omega = out.omega = (T)st.omega;
+    in.beta = (T)st.beta;
+    out.alpha_d = st.alpha;
+    out.omega_d = st.omega;
+    out.xupd = st.x_upd;
+    return true;
+  };
+  stencil_tile2<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, pro);
+  if (st.status != ST_RUN || st_final(st.status)) return;
+  double v[1] = {out.rv};
+  write_partials<1>(v, sred, PA, nb, bid);
+  if (bid == 0 && tid == 0) *Sout = st;
+}
+
+// ---------------------------------------------------------------------------
+// launch B(k) (shared by both variants)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct BIn {
+  struct Raw {
+    T r, v;
+  };
+  const T *r, *v;
+  T alpha;
+  __device__ Raw load(long g) const { return Raw{r[g], v[g]}; }
+  __device__ T combine(const Raw& q) const { return q.r - alpha * q.v; }
+};
+template <typename T>
+struct BOut {
+  using Pre = T;
+  T* t;
+  const T* rt;
+  double ss, ts, tt, rts, rtt;
+  __device__ Pre prefetch(long g) const { return rt[g]; }
+  __device__ void operator()(long g, T y, T s, const Pre& rtg) {
+    t[g] = y;
+    const double sd = (double)s, yd = (double)y, rd = (double)rtg;
+    ss += sd * sd;
+    ts += yd * sd;
+    tt += yd * yd;
+    rts += rd * sd;
+    rtt += rd * yd;
+  }
+};
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_iter_b(OpDev<T> op, const T* r, const T* v, const T* rt, T* t,
+                                                 const KState* Sin, KState* Sout, const double* PA, double* PB) {
+  __shared__ TileLds<T, RY> L;
+  __shared__ double sred[8 * (NTHR / kWave)];
+  const int tid = thread_id(), bid = block_id(), nb = nblocks();
+  BIn<T> in{r, v, T(0)};
+  BOut<T> out{t, rt, 0.0, 0.0, 0.0, 0.0, 0.0};
+  KState st;
+  bool run = false;
+  auto pro = [&]() -> bool {
+    st = *Sin;
+    double tot[1];
+    sum_partials<1>(PA, nb, tot, sred);
+    if (st.status != ST_RUN) {
+      if (bid == 0 && tid == 0) *Sout = st;
+      return false;
+    }
+    const double a = st.rho / tot[0];
+    st.phase = PH_B;
+    if (tot[0] == 0.0 || !isfinite(a)) {
+      st.status = ST_BRK_OMEGA;
+      if (bid == 0 && tid == 0) *Sout = st;
+      return false;
+    }
+    st.alpha = a;
+    in.alpha = (T)a;
+    run = true;
+    return true;
+  };
+  stencil_tile2<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, pro);
+  if (!run) return;
+  double vals[5] = {out.ss, out.ts, out.tt, out.rts, out.rtt};
+  write_partials<5>(vals, sred, PB, nb, bid);
+  if (bid == 0 && tid == 0) *Sout = st;
+}
+
+// ---------------------------------------------------------------------------
+// exact variant: three launches per iteration with rho = rt.r and ||r||
+// formed from the stored r as the reference does.
+//   L1(k): beta from L3's partials (or init); p_k = beta (p - omega v) + r,
+//          v_k = A p_k; partials rt.v
+//   L2(k): the B launch (alpha; s = r - alpha v, t = A s; dots)
+//   L3(k): ||s|| test, omega; x += alpha p + omega s; r = s - omega t;
+//          partials rt.r, r.r
+// ---------------------------------------------------------------------------
 template <typename T>
 struct L1In {
+  struct Raw {
+    T r, p, v;
+  };
   const T *r, *p, *v;
   T omega, beta;
-  __device__ T operator()(long g) const { return beta * (p[g] - omega * v[g]) + r[g]; }
+  __device__ Raw load(long g) const { return Raw{r[g], p[g], v[g]}; }
+  __device__ T combine(const Raw& q) const { return beta * (q.p - omega * q.v) + q.r; }
 };
 template <typename T>
 struct L1Out {
+  using Pre = T;
   T *p_new, *v_new;
   const T* rt;
   double rv;
-  __device__ void operator()(long g, T y, T pn) {
+  __device__ Pre prefetch(long g) const { return rt[g]; }
+  __device__ void operator()(long g, T y, T pn, const Pre& rtg) {
     p_new[g] = pn;
     v_new[g] = y;
-    rv += (double)rt[g] * (double)y;
+    rv += (double)rtg * (double)y;
   }
 };
 
@@ -364,22 +417,31 @@ __global__ __launch_bounds__(NTHR) void k_ex_p(OpDev<T> op, const T* r, const T*
   __shared__ TileLds<T, RY> L;
   __shared__ double sred[8 * (NTHR / kWave)];
   const int tid = thread_id(), bid = block_id(), nb = nblocks();
-  KState st = *Sin;
-  if (st_final(st.status)) {
-    if (bid == 0 && tid == 0) *Sout = st;
-    return;
-  }
-  double tot[2];
-  sum_partials<2>(PC, nb, tot, sred);
-  st = (st.phase == PH_INIT) ? scalars_after_init(st, tot[0], tot[1]) : scalars_after_l3(st, tot[0], tot[1]);
-  st.phase = PH_A;
-  if (st.status != ST_RUN) {
-    if (bid == 0 && tid == 0) *Sout = st;
-    return;
-  }
-  L1In<T> in{r, p_prev, v_prev, (T)st.omega, (T)st.beta};
+  L1In<T> in{r, p_prev, v_prev, T(0), T(0)};
   L1Out<T> out{p_new, v_new, rt, 0.0};
-  stencil_tile<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L);
+  KState st;
+  bool run = false;
+  auto pro = [&]() -> bool {
+    st = *Sin;
+    double tot[2];
+    sum_partials<2>(PC, nb, tot, sred);
+    if (st_final(st.status)) {
+      if (bid == 0 && tid == 0) *Sout = st;
+      return false;
+    }
+    st = (st.phase == PH_INIT) ? scalars_after_init(st, tot[0], tot[1]) : scalars_after_l3(st, tot[0], tot[1]);
+    st.phase = PH_A;
+    if (st.status != ST_RUN) {
+      if (bid == 0 && tid == 0) *Sout = st;
+      return false;
+    }
+    in.omega = (T)st.omega;
+    in.beta = (T)st.beta;
+    run = true;
+    return true;
+  };
+  stencil_tile2<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, pro);
+  if (!run) return;
   double v[1] = {out.rv};
   write_partials<1>(v, sred, PA, nb, bid);
   if (bid == 0 && tid == 0) *Sout = st;
@@ -391,13 +453,32 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
                                                     const double* PB, double* PC) {
   __shared__ double sred[8 * (NTHR / kWave)];
   const int tid = thread_id(), bid = block_id(), nb = nblocks();
+  constexpr int TY = TYT * RY;
+  T er[RY], ep[RY], ev[RY], et[RY], ert[RY];
+  double ex[RY];
+  bool ok[RY];
+  long gg[RY];
+#pragma unroll
+  for (int q = 0; q < RY; ++q) {
+    const int j = blockIdx.y * TY + threadIdx.y + TYT * q, i = blockIdx.x * TX + threadIdx.x;
+    ok[q] = j < ny && i < nx;
+    gg[q] = (long)j * nx + i;
+    if (ok[q]) {
+      er[q] = r[gg[q]];
+      ep[q] = p[gg[q]];
+      ev[q] = v[gg[q]];
+      et[q] = t[gg[q]];
+      ert[q] = rt[gg[q]];
+      ex[q] = x[gg[q]];
+    }
+  }
   KState st = *Sin;
+  double tot[5];
+  sum_partials<5>(PB, nb, tot, sred);
   if (st.status != ST_RUN) {
     if (bid == 0 && tid == 0) *Sout = st;
     return;
   }
-  double tot[5];
-  sum_partials<5>(PB, nb, tot, sred);
   const double snrm = sqrt(tot[0]);
   bool half = false;
   if (snrm <= st.atol) {
@@ -417,21 +498,19 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
   st.phase = PH_L3;
   const T alpha = (T)st.alpha, omega = (T)st.omega;
   double rtr = 0.0, rr = 0.0;
-  constexpr int TY = TYT * RY;
 #pragma unroll
   for (int q = 0; q < RY; ++q) {
-    const int j = blockIdx.y * TY + threadIdx.y + TYT * q, i = blockIdx.x * TX + threadIdx.x;
-    if (j < ny && i < nx) {
-      const long g = (long)j * nx + i;
-      const double xg = x[g] + st.alpha * (double)p[g];
+    if (ok[q]) {
+      const long g = gg[q];
+      const double xg = ex[q] + st.alpha * (double)ep[q];
       if (half) {
         x[g] = xg;
       } else {
-        const T s = r[g] - alpha * v[g];
+        const T s = er[q] - alpha * ev[q];
         x[g] = xg + st.omega * (double)s;
-        const T rn = s - omega * t[g];
+        const T rn = s - omega * et[q];
         r[g] = rn;
-        rtr += (double)rt[g] * (double)rn;
+        rtr += (double)ert[q] * (double)rn;
         rr += (double)rn * (double)rn;
       }
     }
@@ -444,12 +523,20 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
 // ---------------------------------------------------------------------------
 // FP64 true residual r = b - A x with partials of r.r, and a one-block sum
 // ---------------------------------------------------------------------------
+struct TrIn {
+  using Raw = double;
+  const double* x;
+  __device__ Raw load(long g) const { return x[g]; }
+  __device__ double combine(const Raw& v) const { return v; }
+};
 struct TrOut {
+  using Pre = double;
   const double* b;
   double* r;
   double rr;
-  __device__ void operator()(long g, double y, double) {
-    const double rg = b[g] - y;
+  __device__ Pre prefetch(long g) const { return b[g]; }
+  __device__ void operator()(long g, double y, double, const Pre& bg) {
+    const double rg = bg - y;
     r[g] = rg;
     rr += rg * rg;
   }
@@ -460,12 +547,9 @@ __global__ __launch_bounds__(NTHR) void k_true_res(OpDev<double> op, const doubl
                                                    double* PT) {
   __shared__ TileLds<double, RY> L;
   __shared__ double sred[8 * (NTHR / kWave)];
-  struct In {
-    const double* x;
-    __device__ double operator()(long g) const { return x[g]; }
-  } in{x};
+  TrIn in{x};
   TrOut out{b, r, 0.0};
-  stencil_tile<double, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L);
+  stencil_tile2<double, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, [] { return true; });
   double v[1] = {out.rr};
   write_partials<1>(v, sred, PT, nblocks(), block_id());
 }
@@ -689,7 +773,8 @@ int krylov_run(pb_op2d* op, Work<T>& w, const T* b, const double* x0, double tol
     const OpDev<T> d = opdev<T>(op);
     InitIn<T> in{x0, x0 != nullptr};
     InitOut<T> o{b, w.r[0], w.rt, w.p[0], w.v[0], w.t, w.x, x0, x0 != nullptr, 0.0, 0.0};
-    PB_DISPATCH(k_init, T, op, grd, s, d, in, o, exact ? w.PC : w.PB);
+    if (exact) PB_DISPATCH(k_init2, T, op, grd, s, d, in, o, w.PC);
+    else PB_DISPATCH(k_init5, T, op, grd, s, d, in, o, w.PB);
   }
   if (exact) launch_l1<T>(op, w, 1, s);
   else launch_a<T>(op, w, 1, s);
@@ -853,7 +938,7 @@ int time_kernel_t(pb_op2d* op, int kernel, int reps, hipStream_t s, double* avg_
   for (int k = 0; k < 9; ++k) hipLaunchKernelGGL(k_fill_pattern<T>, dim3(nb), dim3(256), 0, s, n, vecs[k], 0.7 * k);
   hipLaunchKernelGGL(k_fill_pattern<double>, dim3(nb), dim3(256), 0, s, n, w.x, 0.1);
   // partials that keep the state running: B totals (1, .5, 1, .3, .1), A total 1
-  std::vector<double> pb(5L * nbs), pa(nbs, 1.0 / nbs);
+  std::vector<double> pb(5L * nbs), pa(nbs, 1.0 / nbs);   // record layout [block][value]
   const double tb[5] = {1.0, 0.5, 1.0, 0.3, 0.1};
   for (int k = 0; k < 5; ++k)
     for (int b = 0; b < nbs; ++b) pb[(long)b * 5 + k] = tb[k] / nbs;

@@ -23,20 +23,25 @@ __global__ void k_scale_rhs(int nx, int ny, const double* rhs_isc, const double*
 
 template <typename T>
 struct InVec {
+  using Raw = T;
   const T* x;
-  __device__ T operator()(long g) const { return x[g]; }
+  __device__ Raw load(long g) const { return x[g]; }
+  __device__ T combine(const Raw& v) const { return v; }
 };
 template <typename T>
 struct OutStore {
+  struct Pre {};
   T* y;
-  __device__ void operator()(long g, T v, T) { y[g] = v; }
+  __device__ Pre prefetch(long) const { return Pre{}; }
+  __device__ void operator()(long g, T v, T, const Pre&) { y[g] = v; }
 };
 
 template <typename T, bool CROSS, bool CENTER, int RY>
 __global__ __launch_bounds__(NTHR) void k_matvec(OpDev<T> op, const T* x, T* y) {
   __shared__ TileLds<T, RY> L;
+  InVec<T> in{x};
   OutStore<T> out{y};
-  stencil_tile<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, InVec<T>{x}, out, L);
+  stencil_tile2<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, [] { return true; });
 }
 
 template <typename T>

@@ -131,6 +131,117 @@ __device__ __forceinline__ void stencil_tile(const OpDev<T>& op, int bx, int by,
   }
 }
 
+// Second form with every global load issued up front: raw halo values
+// (In::load), this thread's cell weights and its nodes' epilogue operands
+// (Out::prefetch, diag_inv, hcenter) are requested before the first barrier,
+// then ``prologue()`` runs (it may reduce the previous launch's partials and
+// set the scalars that In::combine and Out use; returning false ends the
+// block), so the latencies of all of them overlap.
+template <typename T, bool CROSS, bool CENTER, int RY, class In, class Out, class Pro>
+__device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by, In& in, Out& out,
+                                              TileLds<T, RY>& L, Pro&& prologue) {
+  constexpr int TY = TYT * RY;
+  constexpr int HR = TY + 2;
+  constexpr int NPASS = (HR + TYT - 1) / TYT;
+  constexpr int NCELLS = (TY + 1) * (TX + 1);
+  constexpr int NCP = (NCELLS + NTHR - 1) / NTHR;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int tid = tx + TX * ty;
+  const int i0 = bx * TX, j0 = by * TY;
+  const int nx = op.nx, ny = op.ny;
+  int ic = i0 + tx;
+  ic = (ic == nx) ? 1 : ic;
+  const bool col_ok = ic < nx;
+  int ie = (tx == 0) ? i0 - 1 : i0 + TX;
+  ie = (ie < 0) ? nx - 2 : (ie == nx ? 1 : ie);
+  const bool edge_ok = tx < 2 && ie < nx;
+  typename In::Raw raw[NPASS], eraw[NPASS];
+  bool rok[NPASS];
+#pragma unroll
+  for (int q = 0; q < NPASS; ++q) {
+    const int ly = ty + TYT * q;
+    int j = j0 - 1 + ly;
+    j = (j < 0) ? ny - 2 : (j == ny ? 1 : j);
+    rok[q] = ly < HR && j < ny;
+    if (rok[q] && col_ok) raw[q] = in.load((long)j * nx + ic);
+    if (rok[q] && edge_ok) eraw[q] = in.load((long)j * nx + ie);
+  }
+  T wa[NCP], wb[NCP], wc[NCP], wd[NCP];
+#pragma unroll
+  for (int c = 0; c < NCP; ++c) {
+    const int k = tid + c * NTHR;
+    const int cy = k / (TX + 1), cx = k - cy * (TX + 1);
+    const int j = j0 + cy, i = i0 + cx;
+    wa[c] = wb[c] = wc[c] = wd[c] = T(0);
+    if (k < NCELLS && j <= ny && i <= nx) {
+      const long cc = (long)j * (nx + 1) + i;
+      wa[c] = op.wxx[cc];
+      wb[c] = op.wyy[cc];
+      if (CROSS) {
+        wc[c] = op.wyx[cc];
+        wd[c] = op.wxy[cc];
+      }
+    }
+  }
+  typename Out::Pre pre[RY];
+  T dv[RY], hv[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int j = j0 + ty + TYT * r, i = i0 + tx;
+    dv[r] = T(0);
+    hv[r] = T(0);
+    if (j < ny && i < nx) {
+      const long g = (long)j * nx + i;
+      pre[r] = out.prefetch(g);
+      dv[r] = op.dinv[g];
+      if (CENTER) hv[r] = op.hc[g];
+    }
+  }
+  if (!prologue()) return;
+#pragma unroll
+  for (int q = 0; q < NPASS; ++q) {
+    const int ly = ty + TYT * q;
+    if (ly < HR) {
+      L.sp[ly * (TX + 2) + tx + 1] = (rok[q] && col_ok) ? in.combine(raw[q]) : T(0);
+      if (tx < 2) L.sp[ly * (TX + 2) + (tx == 0 ? 0 : TX + 1)] = (rok[q] && edge_ok) ? in.combine(eraw[q]) : T(0);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NCP; ++c) {
+    const int k = tid + c * NTHR;
+    if (k < NCELLS) {
+      const int cy = k / (TX + 1), cx = k - cy * (TX + 1);
+      const T* sp = L.sp + cy * (TX + 2) + cx;
+      const T tl = sp[0], tr = sp[1], bl = sp[TX + 2], br = sp[TX + 3];
+      const T gx = (tr - tl) + (br - bl);
+      const T gy = (bl - tl) + (br - tr);
+      T fx = wa[c] * gx, fy = wb[c] * gy;
+      if (CROSS) {
+        fx += wc[c] * gy;
+        fy += wd[c] * gx;
+      }
+      L.sfx[k] = fx;
+      L.sfy[k] = fy;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int ly = ty + TYT * r, lx = tx;
+    const int j = j0 + ly, i = i0 + lx;
+    if (j < ny && i < nx) {
+      const long g = (long)j * nx + i;
+      const int c0 = ly * (TX + 1) + lx, c1 = c0 + (TX + 1);
+      T acc = ((L.sfx[c0 + 1] + L.sfx[c1 + 1]) - (L.sfx[c0] + L.sfx[c1])) +
+              ((L.sfy[c1] + L.sfy[c1 + 1]) - (L.sfy[c0] + L.sfy[c0 + 1]));
+      const T pc = L.sp[(ly + 1) * (TX + 2) + lx + 1];
+      if (CENTER) acc += hv[r] * pc;
+      out(g, dv[r] * acc, pc, pre[r]);
+    }
+  }
+}
+
 // Sum ``nb`` block partials of NV values (record layout [b * NV + k]) in a
 // fixed order; the result is valid in every thread.  sred: NV * (NTHR/64).
 template <int NV>
