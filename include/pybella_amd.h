@@ -59,6 +59,7 @@ extern "C" {
 typedef struct pb_ctx pb_ctx;
 typedef struct pb_op2d pb_op2d;
 typedef struct pb_op3d pb_op3d;
+typedef struct pb_nn pb_nn;
 
 typedef struct pb_grid2d {
   int nx, ny;          /* inner node counts (ud.inx, ud.iny)            */
@@ -80,11 +81,13 @@ typedef struct pb_phys {
 typedef struct pb_solve_opts {
   int mode;            /* PB_SOLVE_*                                          */
   double tol;          /* ud.tol (legacy semantics in PB_SOLVE_F64)           */
-  int maxiter;         /* ud.max_iterations (per Krylov run)                  */
+  int maxiter;         /* ud.max_iterations (per Krylov run); PB_SOLVE_NN: the
+                          flexible-GMRES iteration budget before the fallback */
   double inner_tol;    /* PB_SOLVE_MIXED / PB_SOLVE_NN: inner relative tol    */
   int max_outer;       /* PB_SOLVE_MIXED / PB_SOLVE_NN: refinement cap        */
   int check_every;     /* iterations between host convergence checks (0=auto) */
   int x0_given;        /* 0: x0 = 0 as the reference; 1: use x as the guess   */
+  int fallback_maxiter; /* PB_SOLVE_NN: BiCGStab iterations after the budget  */
 } pb_solve_opts;
 
 typedef struct pb_solve_info {
@@ -95,6 +98,7 @@ typedef struct pb_solve_info {
   double resid;        /* last recursive residual norm                             */
   double bnrm2;        /* ||b||                                                    */
   double true_rel_resid; /* ||b - A x|| / ||b|| recomputed in FP64                 */
+  int fallback;        /* PB_SOLVE_NN: 1 if plain BiCGStab had to finish the solve  */
 } pb_solve_info;
 
 int pb_version(void);
@@ -174,6 +178,32 @@ int pb_correction_nodes(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys,
  * p2_full (nodes) receives the increment. */
 int pb_scatter_update_p2(pb_ctx* ctx, const pb_grid2d* grid, const double* x,
                          double* p2_full, double* p2_nodes);
+
+/* ---- learned preconditioner (no reference counterpart: pyBELLA has none) ----
+ * Fixed architecture: 4-level convolutional encoder-decoder, 3x3 convolutions,
+ * 16-32-64-64 channels, GELU, 15 layers, BF16 MFMA kernels with FP32
+ * accumulation and FP32 master parameters (layout per layer: W[9][cin][cout]
+ * then bias[cout]).  Grids must have odd node counts (2^k + 1). */
+int pb_nn_create(pb_ctx* ctx, unsigned long long seed, pb_nn** out);
+int pb_nn_destroy(pb_nn* nn);
+long pb_nn_num_params(const pb_nn* nn);
+/* host copies of the parameters and / or the last computed gradients */
+int pb_nn_get_params(pb_nn* nn, float* host_params, float* host_grads);
+int pb_nn_set_params(pb_nn* nn, const float* host_params);
+/* One batch: inputs float[nimg][h*w][3] (normalised r, feature 1, feature 2),
+ * target float[nimg][h*w] or NULL (forward only), out_z float[nimg][h*w] or
+ * NULL (device pointers).  With a target: gradients of the mean squared error
+ * of the output (constant and checkerboard components removed) and, if lr > 0,
+ * one Adam step.  Synchronous. */
+int pb_nn_train_step(pb_nn* nn, int nimg, int h, int w, const float* inputs, const float* target, double lr,
+                     float* out_z, double* loss);
+/* z = rms(r) * net(r / rms(r), features(op)); r, z: FP64 device vectors. */
+int pb_nn_apply(pb_nn* nn, const pb_op2d* op, const double* r, double* z);
+/* the two coefficient features (float device arrays of nx*ny) of an operator */
+int pb_nn_features(pb_nn* nn, const pb_op2d* op, float* f1, float* f2);
+/* attach a network to an operator: PB_SOLVE_NN then uses it as the initial
+ * guess and the preconditioner of flexible GMRES (NULL detaches) */
+int pb_op2d_attach_nn(pb_op2d* op, pb_nn* nn);
 
 #ifdef __cplusplus
 }

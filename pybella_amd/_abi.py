@@ -36,12 +36,13 @@ class Phys(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("tol", C.c_double), ("maxiter", C.c_int),
                 ("inner_tol", C.c_double), ("max_outer", C.c_int), ("check_every", C.c_int),
-                ("x0_given", C.c_int)]
+                ("x0_given", C.c_int), ("fallback_maxiter", C.c_int)]
 
 
 class SolveInfo(C.Structure):
     _fields_ = [("info", C.c_int), ("iters", C.c_int), ("outer", C.c_int), ("half_step", C.c_int),
-                ("resid", C.c_double), ("bnrm2", C.c_double), ("true_rel_resid", C.c_double)]
+                ("resid", C.c_double), ("bnrm2", C.c_double), ("true_rel_resid", C.c_double),
+                ("fallback", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -73,6 +74,15 @@ SIGNATURES = {
     "pb_correction_nodes": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), D, D, VP, VP, VP, VP,
                                 VP, VP, VP, VP, VP, VP]),
     "pb_scatter_update_p2": (I, [VP, C.POINTER(Grid2D), VP, VP, VP]),
+    "pb_nn_create": (I, [VP, C.c_ulonglong, C.POINTER(VP)]),
+    "pb_nn_destroy": (I, [VP]),
+    "pb_nn_num_params": (C.c_long, [VP]),
+    "pb_nn_get_params": (I, [VP, VP, VP]),
+    "pb_nn_set_params": (I, [VP, VP]),
+    "pb_nn_train_step": (I, [VP, I, I, I, VP, VP, D, VP, C.POINTER(D)]),
+    "pb_nn_apply": (I, [VP, VP, VP, VP]),
+    "pb_nn_features": (I, [VP, VP, VP, VP]),
+    "pb_op2d_attach_nn": (I, [VP, VP]),
 }
 
 _lib = None

@@ -825,6 +825,20 @@ int info_of(const KState& h) {
   }
 }
 
+}  // namespace
+
+// FP64 true residual norm for other translation units (op->rtrue = b - A x)
+double pb2d_true_residual(pb_op2d* op, const double* b, const double* x, hipStream_t s, int* rc) {
+  double nrm = 0.0;
+  *rc = true_residual(op, b, x, s, &nrm);
+  return nrm;
+}
+
+int pb_nn_solve(pb_op2d* op, const double* b, double* x, const pb_solve_opts* opts, pb_solve_info* info,
+                double (*true_res)(pb_op2d*, const double*, const double*, hipStream_t, int*), hipStream_t s);
+
+namespace {
+
 // the stream solves run on: the caller's when it can be captured, else our own
 hipStream_t solve_stream(pb_ctx* ctx) {
   if (ctx->stream != nullptr) return ctx->stream;
@@ -914,6 +928,19 @@ extern "C" int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x, const 
     out.outer = outer;
     out.half_step = h.half;
     out.resid = rn;
+  } else if (opts->mode == PB_SOLVE_NN) {
+    rc = pb_nn_solve(op, b, x, opts, &out, pb2d_true_residual, s);
+    if (rc) return rc;
+    if (out.info != 0) {   // budget exhausted: plain BiCGStab from the current iterate
+      KState h{};
+      const int fmax = opts->fallback_maxiter > 0 ? opts->fallback_maxiter : 6000;
+      rc = krylov_run<double>(op, op->w64, b, x, opts->tol, fmax, 0, opts->check_every, s, &h, false);
+      if (rc) return rc;
+      PB_HIP(ctx, hipMemcpyAsync(x, op->w64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      out.iters += h.iter;
+      out.fallback = 1;
+      out.info = info_of(h);
+    }
   } else {
     return pb_fail(ctx, PB_ERR_ARG, "pb_bicgstab_solve: mode %d not available on this handle", opts->mode);
   }

@@ -1,0 +1,1044 @@
+// Learned preconditioner network on gfx950: a 4-level convolutional
+// encoder-decoder (3x3 convolutions, 16-32-64-64 channels, GELU) with
+// hand-written BF16 MFMA kernels and FP32 accumulation, for forward, backward
+// and training.  No reference counterpart (pyBELLA has no learned component).
+//
+// Tensors are NHWC bf16 with a channel stride ``ctot`` and offset ``coff`` so
+// that concatenations are views.  Grid levels are node-based 2:1 coarsenings,
+// H_{l+1} = (H_l + 1) / 2, matching 2^k + 1 grids (513 -> 257 -> 129 -> 65).
+// Down-sampling is injection (coarse(i, j) = fine(2i, 2j)); up-sampling is
+// linear interpolation (fine(2i) = coarse(i), fine(2i+1) = mean of neighbours).
+//
+// Convolution as implicit GEMM: out[p][co] = sum_{tap, ci} in[p + off(tap)][ci]
+// W[tap][ci][co] (zero padding).  One wavefront computes a 16-pixel x 16-channel
+// tile per v_mfma_f32_16x16x16bf16: lane l holds A[l%16][4(l/16)..+3] (pixel,
+// 4 input channels), B[4(l/16)..+3][l%16] (4 input channels, output channel)
+// and D[4(l/16)..+3][l%16] (4 pixels, output channel).  The input-gradient
+// (dgrad) is the same kernel on the output gradient with taps flipped and the
+// weights in [tap][ci][co] layout; the weight gradient reduces over pixels
+// (the GEMM K dimension) in per-chunk partials summed in a fixed order.
+#include <vector>
+
+#include "pb_op2d.hpp"
+
+using namespace pb2d;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short bf16;
+
+namespace {
+
+__device__ __forceinline__ float bf2f(bf16 b) { return __uint_as_float(((unsigned)b) << 16); }
+__device__ __forceinline__ bf16 f2bf(float f) {   // round to nearest even
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (bf16)(u >> 16);   // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16)(u >> 16);
+}
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+  return cdf + x * pdf;
+}
+
+struct View {
+  bf16* p;
+  int ctot, coff, c;   // channel stride, offset, channel count (multiple of 16)
+};
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution (forward or input-gradient), NT output tiles of 16 channels
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, int in_ctot, int in_coff, int cin,
+                                                 const bf16* __restrict__ wg, int flip, const float* __restrict__ bias,
+                                                 int act, bf16* out, int out_ctot, int out_coff, int cout_valid,
+                                                 bf16* pre, int pre_ctot, float* out_f32, int nimg, int h, int w) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long npix = (long)nimg * h * w;
+  const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
+  if (p0 >= npix) return;
+  const long pa = p0 + (lane & 15);
+  const int kg = lane >> 4;
+  const bool va = pa < npix;
+  const int xa = (int)(pa % w);
+  const long t1 = pa / w;
+  const int ya = (int)(t1 % h);
+  const long ba = t1 / h;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int nout = NT * 16;
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int ys = ya + dy, xs = xa + dx;
+    const bool ok = va && ys >= 0 && ys < h && xs >= 0 && xs < w;
+    const long src = ok ? (((ba * h + ys) * w + xs) * in_ctot + in_coff + 4 * kg) : 0;
+    const int wt = flip ? 8 - tap : tap;
+    const bf16* wrow = wg + ((long)wt * nout + (lane & 15)) * cin + 4 * kg;
+    for (int ci0 = 0; ci0 < cin; ci0 += 16) {
+      bf16x4 a = {0, 0, 0, 0};
+      if (ok) a = *(const bf16x4*)(in + src + ci0);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const bf16x4 b = *(const bf16x4*)(wrow + (long)t * 16 * cin + ci0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = t * 16 + (lane & 15);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const long p = p0 + 4 * kg + e;
+      if (p < npix && co < cout_valid) {
+        float v = acc[t][e] + (bias ? bias[co] : 0.f);
+        const long o = p * out_ctot + out_coff + co;
+        if (pre) pre[p * pre_ctot + co] = f2bf(v);   // pre-activations: own buffer, cout channels
+        if (act) v = gelu(v);
+        if (out) out[o] = f2bf(v);
+        if (out_f32 && co == 0) out_f32[p] = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient: dW[tap][ci][co] = sum_p in[p + off(tap)][ci] dz[p][co].
+// Block (co tile, ci tile) x pixel chunk; wave-strided 16-pixel steps; MFMA
+// with A[co][pixel] = dz, B[pixel][ci] = shifted input.  Partials per chunk.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ in, int in_ctot, int in_coff, int cin,
+                                               const bf16* __restrict__ dz, int dz_ctot, int dz_coff, int cout,
+                                               int nimg, int h, int w, long chunk, float* partial) {
+  __shared__ float red[4][9][16][17];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nct = cout / 16;
+  const int cot = blockIdx.y % nct, cit = blockIdx.y / nct;
+  const long npix = (long)nimg * h * w;
+  const long pbeg = (long)blockIdx.x * chunk;
+  const long pend = pbeg + chunk < npix ? pbeg + chunk : npix;
+  const int kg = lane >> 4, l16 = lane & 15;
+  f32x4 acc[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (long p0 = pbeg + wave * 16; p0 < pend; p0 += 64) {
+    bf16x4 a;
+    long pp[4];
+    int xx[4], yy[4];
+    long bb[4];
+    bool vv[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const long p = p0 + 4 * kg + e;
+      vv[e] = p < pend;
+      pp[e] = p;
+      xx[e] = (int)(p % w);
+      const long t1 = p / w;
+      yy[e] = (int)(t1 % h);
+      bb[e] = t1 / h;
+      a[e] = vv[e] ? (short)dz[p * dz_ctot + dz_coff + cot * 16 + l16] : (short)0;
+    }
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      bf16x4 b;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int ys = yy[e] + dy, xs = xx[e] + dx;
+        const bool ok = vv[e] && ys >= 0 && ys < h && xs >= 0 && xs < w;
+        b[e] = ok ? (short)in[((bb[e] * h + ys) * w + xs) * in_ctot + in_coff + cit * 16 + l16] : (short)0;
+      }
+      acc[tap] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tap], 0, 0, 0);
+    }
+    (void)pp;
+  }
+  // acc[tap][e] = D[co = 4 kg + e][ci = l16]
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[wave][tap][4 * kg + e][l16] = acc[tap][e];
+  __syncthreads();
+  for (int k = threadIdx.x; k < 9 * 256; k += 256) {
+    const int tap = k / 256, rem = k % 256, co = rem / 16, ci = rem % 16;
+    const float s = (red[0][tap][co][ci] + red[1][tap][co][ci]) + (red[2][tap][co][ci] + red[3][tap][co][ci]);
+    // partial layout [chunk][tap][ci][co] over the padded cin x cout
+    partial[(((long)blockIdx.x * 9 + tap) * cin + cit * 16 + ci) * cout + cot * 16 + co] = s;
+  }
+}
+
+// sum the chunk partials into the master gradient [9][cin_v][cout_v] (true dims)
+__global__ void k_wgrad_reduce(const float* partial, int nchunk, int cin, int cout, int cin_v, int cout_v, float* g) {
+  const long n = 9L * cin_v * cout_v;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const int co = (int)(k % cout_v);
+    const long t1 = k / cout_v;
+    const int ci = (int)(t1 % cin_v);
+    const int tap = (int)(t1 / cin_v);
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += partial[(((long)c * 9 + tap) * cin + ci) * cout + co];
+    g[k] = s;
+  }
+}
+
+// bias gradient: per output channel sum over pixels (chunk partials, then reduce)
+__global__ void k_bgrad(const bf16* dz, int ctot, int coff, int cout_v, long npix, long chunk, float* partial) {
+  const int co = threadIdx.x & 63;
+  const long pbeg = (long)blockIdx.x * chunk;
+  const long pend = pbeg + chunk < npix ? pbeg + chunk : npix;
+  float s = 0.f;
+  if (co < cout_v)
+    for (long p = pbeg + (threadIdx.x >> 6); p < pend; p += blockDim.x >> 6) s += bf2f(dz[p * ctot + coff + co]);
+  __shared__ float sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < 64 && co < cout_v) {
+    float t = 0.f;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k * 64 + threadIdx.x];
+    partial[(long)blockIdx.x * 64 + co] = t;
+  }
+}
+__global__ void k_bgrad_reduce(const float* partial, int nchunk, int cout_v, float* g) {
+  const int co = threadIdx.x;
+  if (co >= cout_v) return;
+  float s = 0.f;
+  for (int c = 0; c < nchunk; ++c) s += partial[(long)c * 64 + co];
+  g[co] = s;
+}
+
+// dz *= gelu'(pre) on a view
+__global__ void k_act_bwd(bf16* d, int ctot, int coff, int c, const bf16* pre, int pctot, int pcoff, long npix) {
+  const long n = npix * c;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const long p = k / c;
+    const int ch = (int)(k % c);
+    const long o = p * ctot + coff + ch;
+    d[o] = f2bf(bf2f(d[o]) * gelu_grad(bf2f(pre[p * pctot + pcoff + ch])));
+  }
+}
+
+// down-sampling by injection and its adjoint (accumulating)
+__global__ void k_down(const bf16* f, int fct, int fco, bf16* c, int cct, int cco, int ch, int nimg, int hc, int wc,
+                       int hf, int wf) {
+  const long n = (long)nimg * hc * wc * ch;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const int cc = (int)(k % ch);
+    long t = k / ch;
+    const int x = (int)(t % wc);
+    t /= wc;
+    const int y = (int)(t % hc);
+    const long b = t / hc;
+    c[((b * hc + y) * wc + x) * cct + cco + cc] = f[((b * hf + 2 * y) * wf + 2 * x) * fct + fco + cc];
+  }
+}
+__global__ void k_down_bwd(const bf16* dc, int cct, int cco, bf16* df, int fct, int fco, int ch, int nimg, int hc,
+                           int wc, int hf, int wf) {
+  const long n = (long)nimg * hc * wc * ch;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const int cc = (int)(k % ch);
+    long t = k / ch;
+    const int x = (int)(t % wc);
+    t /= wc;
+    const int y = (int)(t % hc);
+    const long b = t / hc;
+    const long o = ((b * hf + 2 * y) * wf + 2 * x) * fct + fco + cc;
+    df[o] = f2bf(bf2f(df[o]) + bf2f(dc[((b * hc + y) * wc + x) * cct + cco + cc]));
+  }
+}
+
+// linear up-sampling (fine = 2 coarse - 1 per axis) and its adjoint (gather)
+__global__ void k_up(const bf16* c, int cct, int cco, bf16* f, int fct, int fco, int ch, int nimg, int hc, int wc,
+                     int hf, int wf) {
+  const long n = (long)nimg * hf * wf * ch;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const int cc = (int)(k % ch);
+    long t = k / ch;
+    const int x = (int)(t % wf);
+    t /= wf;
+    const int y = (int)(t % hf);
+    const long b = t / hf;
+    const int y0 = y >> 1, x0 = x >> 1;
+    const int y1 = (y & 1) ? y0 + 1 : y0, x1 = (x & 1) ? x0 + 1 : x0;
+    auto at = [&](int yy, int xx) { return bf2f(c[((b * hc + yy) * wc + xx) * cct + cco + cc]); };
+    const float v = 0.25f * ((at(y0, x0) + at(y0, x1)) + (at(y1, x0) + at(y1, x1)));
+    f[((b * hf + y) * wf + x) * fct + fco + cc] = f2bf(v);
+  }
+}
+__global__ void k_up_bwd(const bf16* df, int fct, int fco, bf16* dc, int cct, int cco, int ch, int nimg, int hc,
+                         int wc, int hf, int wf) {
+  const long n = (long)nimg * hc * wc * ch;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const int cc = (int)(k % ch);
+    long t = k / ch;
+    const int x = (int)(t % wc);
+    t /= wc;
+    const int y = (int)(t % hc);
+    const long b = t / hc;
+    float s = 0.f;
+    for (int ay = -1; ay <= 1; ++ay) {
+      const int fy = 2 * y + ay;
+      if (fy < 0 || fy >= hf) continue;
+      const float wy = ay == 0 ? 1.f : 0.5f;
+      for (int ax = -1; ax <= 1; ++ax) {
+        const int fx = 2 * x + ax;
+        if (fx < 0 || fx >= wf) continue;
+        const float wx = ax == 0 ? 1.f : 0.5f;
+        s += wy * wx * bf2f(df[((b * hf + fy) * wf + fx) * fct + fco + cc]);
+      }
+    }
+    dc[((b * hc + y) * wc + x) * cct + cco + cc] = f2bf(s);
+  }
+}
+
+// network input: ch0 = r / scale, ch1, ch2 = coefficient features, rest 0
+template <typename Tr>
+__global__ void k_input(const Tr* r, const float* scale, const float* f1, const float* f2, int feat_per_img,
+                        long npix_img, int nimg, bf16* x0) {
+  const long n = npix_img * nimg;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const long b = k / npix_img, p = k % npix_img;
+    const long fo = feat_per_img ? k : p;
+    bf16* o = x0 + k * 16;
+    o[0] = f2bf((float)((double)r[k] / (double)scale[b]));
+    o[1] = f2bf(f1[fo]);
+    o[2] = f2bf(f2[fo]);
+#pragma unroll
+    for (int c = 3; c < 16; ++c) o[c] = 0;
+  }
+}
+
+// per-image sums: sum r^2 (-> rms scale), or sum z, sum c z for the null-space fit
+template <typename Tr>
+__global__ void k_img_sums(const Tr* v, long npix_img, int w, int mode, double* part, int nchunk) {
+  const int b = blockIdx.y;
+  const long chunk = (npix_img + nchunk - 1) / nchunk;
+  const long beg = blockIdx.x * chunk, end = beg + chunk < npix_img ? beg + chunk : npix_img;
+  double s0 = 0.0, s1 = 0.0;
+  for (long p = beg + threadIdx.x; p < end; p += blockDim.x) {
+    const double x = (double)v[b * npix_img + p];
+    if (mode == 0) {
+      s0 += x * x;
+    } else {
+      const long y = p / w, xx = p % w;
+      s0 += x;
+      s1 += ((y + xx) & 1) ? -x : x;
+    }
+  }
+  __shared__ double sh[2][256];
+  sh[0][threadIdx.x] = s0;
+  sh[1][threadIdx.x] = s1;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[((long)b * nchunk + blockIdx.x) * 2] = sh[0][0];
+    part[((long)b * nchunk + blockIdx.x) * 2 + 1] = sh[1][0];
+  }
+}
+
+// scale[b] = sqrt(mean r^2) (floored)
+__global__ void k_scale_from(const double* part, int nchunk, long npix_img, float* scale) {
+  const int b = threadIdx.x;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += part[((long)b * nchunk + c) * 2];
+  const double rms = sqrt(s / (double)npix_img);
+  scale[b] = (float)(rms > 1e-300 ? rms : 1.0);
+}
+
+// null-space coefficients (mu, kappa) of each image: least squares on {1, c}
+__global__ void k_null_coef(const double* part, int nchunk, long npix_img, int w, int h, double* coef) {
+  const int b = threadIdx.x;
+  double sz = 0.0, scz = 0.0;
+  for (int c = 0; c < nchunk; ++c) {
+    sz += part[((long)b * nchunk + c) * 2];
+    scz += part[((long)b * nchunk + c) * 2 + 1];
+  }
+  // sum of the checkerboard over the grid
+  const long ne = ((long)h * w + 1) / 2, no = (long)h * w / 2;
+  const double sc = (double)(ne - no), n = (double)npix_img;
+  const double det = n * n - sc * sc;
+  coef[2 * b] = (n * sz - sc * scz) / det;
+  coef[2 * b + 1] = (n * scz - sc * sz) / det;
+}
+
+// loss: z projected off the null space vs target; gradient into channel 0 of dout
+__global__ void k_loss(const float* z, const float* target, const double* coef, long npix_img, int w, int nimg,
+                       float inv_n, bf16* dout, double* lpart) {
+  const long n = npix_img * nimg;
+  double l = 0.0;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const long b = k / npix_img, p = k % npix_img;
+    const long y = p / w, x = p % w;
+    const double c = ((y + x) & 1) ? -1.0 : 1.0;
+    const double zp = (double)z[k] - coef[2 * b] - coef[2 * b + 1] * c;
+    const double d = zp - (double)target[k];
+    l += d * d;
+    bf16* o = dout + k * 16;
+    o[0] = f2bf((float)(2.0 * d) * inv_n);
+#pragma unroll
+    for (int ch = 1; ch < 16; ++ch) o[ch] = 0;
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = l;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lpart[blockIdx.x] = sh[0];
+}
+
+// z_out = scale * z (optionally projected off the null space)
+template <typename To>
+__global__ void k_output(const float* z, const float* scale, long npix_img, int nimg, To* out) {
+  const long n = npix_img * nimg;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+    out[k] = (To)((double)z[k] * (double)scale[k / npix_img]);
+}
+
+__global__ void k_adam(long n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2) {
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.f - b1) * gk;
+    const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    w[k] -= lr * (mk / bc1) / (sqrtf(vk / bc2) + eps);
+  }
+}
+
+// master [9][cin_v][cout_v] fp32 -> forward [9][cout_p][cin_p] and backward
+// [9][cin_p][cout_p] bf16, zero padded
+__global__ void k_pack_w(const float* wm, int cin_v, int cout_v, int cin_p, int cout_p, bf16* wf, bf16* wb) {
+  const long n = 9L * cin_p * cout_p;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const int co = (int)(k % cout_p);
+    const long t1 = k / cout_p;
+    const int ci = (int)(t1 % cin_p);
+    const int tap = (int)(t1 / cin_p);
+    const float v = (ci < cin_v && co < cout_v) ? wm[((long)tap * cin_v + ci) * cout_v + co] : 0.f;
+    const bf16 b = f2bf(v);
+    wb[k] = b;
+    wf[((long)tap * cout_p + co) * cin_p + ci] = b;
+  }
+}
+
+__global__ void k_split3(const float* in, long n, bf16* x0) {
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    bf16* o = x0 + k * 16;
+    o[0] = f2bf(in[3 * k]);
+    o[1] = f2bf(in[3 * k + 1]);
+    o[2] = f2bf(in[3 * k + 2]);
+#pragma unroll
+    for (int c = 3; c < 16; ++c) o[c] = 0;
+  }
+}
+
+__global__ void k_fill_bf16(long n, bf16* a) {
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) a[k] = 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side: fixed architecture
+// ===========================================================================
+namespace {
+
+struct LayerSpec {
+  int cin_v, cout_v;   // true channel counts
+  int act;
+};
+
+// 15 convolutions in forward order (see the table in the file header)
+const LayerSpec kLayers[15] = {
+    {3, 16, 1},  {16, 16, 1},  {16, 32, 1}, {32, 32, 1}, {32, 64, 1},  {64, 64, 1}, {64, 64, 1}, {64, 64, 1},
+    {128, 64, 1}, {64, 64, 1}, {96, 32, 1}, {32, 32, 1}, {48, 16, 1}, {16, 16, 1}, {16, 1, 0}};
+
+inline int pad16(int c) { return (c + 15) / 16 * 16; }
+
+struct Level {
+  int h, w;
+};
+
+struct Acts {   // activation / gradient buffers of one (nimg, h, w) shape
+  int nimg = 0, h = 0, w = 0;
+  Level lv[4];
+  std::vector<void*> allocs;
+  // activations
+  bf16 *X0, *E0a, *CAT0, *D0a, *D0b, *OUT16;
+  bf16 *P1, *E1a, *CAT1, *D1a, *D1b;
+  bf16 *P2, *E2a, *CAT2, *D2a, *D2b;
+  bf16 *P3, *B3a, *B3b;
+  // pre-activations (training)
+  bf16 *pE0a, *pE0b, *pD0a, *pD0b, *pE1a, *pE1b, *pD1a, *pD1b, *pE2a, *pE2b, *pD2a, *pD2b, *pB3a, *pB3b;
+  // gradients
+  bf16 *dOUT, *dD0b, *dD0a, *dCAT0, *dD1b, *dD1a, *dCAT1, *dD2b, *dD2a, *dCAT2, *dB3b, *dB3a, *dP3, *dP2, *dP1,
+      *dE2a, *dE1a, *dE0a;
+  float* zout;       // fp32 network output (channel 0)
+  float* scale;      // per image
+  double* part;      // per image partial sums
+  double* coef;      // per image null-space coefficients
+  double* lpart;     // loss partials
+  float* feat1;      // per-image features when training (nimg * npix)
+  float* feat2;
+  bool train = false;
+};
+
+}  // namespace
+
+struct pb_nn {
+  pb_ctx* ctx = nullptr;
+  long nparam = 0;
+  long woff[15], boff[15];     // offsets into the master parameter vector
+  float* params = nullptr;     // master fp32
+  float* grads = nullptr;
+  float* adam_m = nullptr;
+  float* adam_v = nullptr;
+  long adam_t = 0;
+  bf16* wf[15];                // forward layout
+  bf16* wb[15];                // backward layout
+  float* wpart = nullptr;      // wgrad partials
+  long wpart_cap = 0;
+  float* bpart = nullptr;
+  std::vector<void*> allocs;
+  std::vector<Acts*> shapes;
+  // features of the last operator used for inference
+  const pb_op2d* feat_op = nullptr;
+  float* f1 = nullptr;
+  float* f2 = nullptr;
+  long feat_n = 0;
+};
+
+namespace {
+
+template <typename T>
+int nn_malloc(pb_nn* nn, std::vector<void*>& list, T** p, long count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)std::max<long>(count, 1));
+  if (e != hipSuccess)
+    return pb_fail(nn->ctx, PB_ERR_HIP, "hipMalloc(%ld bytes) failed: %s", (long)(sizeof(T) * count),
+                   hipGetErrorString(e));
+  list.push_back(q);
+  *p = (T*)q;
+  return PB_OK;
+}
+
+inline int gs(long n, int per = 256) { return (int)std::min<long>((n + per - 1) / per, 65535); }
+
+int pack_weights(pb_nn* nn, hipStream_t s) {
+  for (int l = 0; l < 15; ++l) {
+    const LayerSpec& L = kLayers[l];
+    const int cin_p = pad16(L.cin_v), cout_p = pad16(L.cout_v);
+    hipLaunchKernelGGL(k_pack_w, dim3(gs(9L * cin_p * cout_p)), dim3(256), 0, s, nn->params + nn->woff[l], L.cin_v,
+                       L.cout_v, cin_p, cout_p, nn->wf[l], nn->wb[l]);
+  }
+  PB_LAUNCH_CHECK(nn->ctx);
+  return PB_OK;
+}
+
+Acts* get_shape(pb_nn* nn, int nimg, int h, int w, bool train, int* rc) {
+  for (Acts* a : nn->shapes)
+    if (a->nimg == nimg && a->h == h && a->w == w && (a->train || !train)) return a;
+  Acts* a = new Acts();
+  a->nimg = nimg;
+  a->h = h;
+  a->w = w;
+  a->train = train;
+  a->lv[0] = {h, w};
+  for (int l = 1; l < 4; ++l) a->lv[l] = {(a->lv[l - 1].h + 1) / 2, (a->lv[l - 1].w + 1) / 2};
+  auto np = [&](int l) { return (long)nimg * a->lv[l].h * a->lv[l].w; };
+  int r = 0;
+  auto T = [&](bf16** p, int l, int c) { r |= nn_malloc(nn, a->allocs, p, np(l) * c); };
+  T(&a->X0, 0, 16);
+  T(&a->E0a, 0, 16);
+  T(&a->CAT0, 0, 48);
+  T(&a->D0a, 0, 16);
+  T(&a->D0b, 0, 16);
+  T(&a->OUT16, 0, 16);
+  T(&a->P1, 1, 16);
+  T(&a->E1a, 1, 32);
+  T(&a->CAT1, 1, 96);
+  T(&a->D1a, 1, 32);
+  T(&a->D1b, 1, 32);
+  T(&a->P2, 2, 32);
+  T(&a->E2a, 2, 64);
+  T(&a->CAT2, 2, 128);
+  T(&a->D2a, 2, 64);
+  T(&a->D2b, 2, 64);
+  T(&a->P3, 3, 64);
+  T(&a->B3a, 3, 64);
+  T(&a->B3b, 3, 64);
+  if (train) {
+    T(&a->pE0a, 0, 16);
+    T(&a->pE0b, 0, 16);
+    T(&a->pD0a, 0, 16);
+    T(&a->pD0b, 0, 16);
+    T(&a->pE1a, 1, 32);
+    T(&a->pE1b, 1, 32);
+    T(&a->pD1a, 1, 32);
+    T(&a->pD1b, 1, 32);
+    T(&a->pE2a, 2, 64);
+    T(&a->pE2b, 2, 64);
+    T(&a->pD2a, 2, 64);
+    T(&a->pD2b, 2, 64);
+    T(&a->pB3a, 3, 64);
+    T(&a->pB3b, 3, 64);
+    T(&a->dOUT, 0, 16);
+    T(&a->dD0b, 0, 16);
+    T(&a->dD0a, 0, 16);
+    T(&a->dCAT0, 0, 48);
+    T(&a->dD1b, 1, 32);
+    T(&a->dD1a, 1, 32);
+    T(&a->dCAT1, 1, 96);
+    T(&a->dD2b, 2, 64);
+    T(&a->dD2a, 2, 64);
+    T(&a->dCAT2, 2, 128);
+    T(&a->dB3b, 3, 64);
+    T(&a->dB3a, 3, 64);
+    T(&a->dP3, 3, 64);
+    T(&a->dP2, 2, 32);
+    T(&a->dP1, 1, 16);
+    T(&a->dE2a, 2, 64);
+    T(&a->dE1a, 1, 32);
+    T(&a->dE0a, 0, 16);
+    r |= nn_malloc(nn, a->allocs, &a->feat1, np(0));
+    r |= nn_malloc(nn, a->allocs, &a->feat2, np(0));
+  }
+  r |= nn_malloc(nn, a->allocs, &a->zout, np(0));
+  r |= nn_malloc(nn, a->allocs, &a->scale, nimg);
+  r |= nn_malloc(nn, a->allocs, &a->part, 2L * nimg * 64);
+  r |= nn_malloc(nn, a->allocs, &a->coef, 2L * nimg);
+  r |= nn_malloc(nn, a->allocs, &a->lpart, 1024);
+  if (r) {
+    for (void* p : a->allocs) (void)hipFree(p);
+    delete a;
+    *rc = PB_ERR_HIP;
+    return nullptr;
+  }
+  nn->shapes.push_back(a);
+  *rc = PB_OK;
+  return a;
+}
+
+void conv(pb_nn* nn, Acts* a, int l, int level, const bf16* in, int in_ct, int in_co, bf16* out, int out_ct,
+          int out_co, bf16* pre, float* out_f32, hipStream_t s) {
+  const LayerSpec& L = kLayers[l];
+  const int cin = pad16(L.cin_v), nt = pad16(L.cout_v) / 16;
+  const long npix = (long)a->nimg * a->lv[level].h * a->lv[level].w;
+  const dim3 grd((unsigned)((npix + 63) / 64)), blk(256);
+  const float* bias = nn->params + nn->boff[l];
+  const int h = a->lv[level].h, w = a->lv[level].w;
+#define PB_CONV(NT)                                                                                              \
+  hipLaunchKernelGGL(k_conv3x3<NT>, grd, blk, 0, s, in, in_ct, in_co, cin, nn->wf[l], 0, bias, L.act, out, out_ct, \
+                     out_co, L.cout_v, pre, pad16(L.cout_v), out_f32, a->nimg, h, w)
+  switch (nt) {
+    case 1: PB_CONV(1); break;
+    case 2: PB_CONV(2); break;
+    default: PB_CONV(4); break;
+  }
+#undef PB_CONV
+}
+
+// input gradient of layer l: din (cin_p channels) from dz (cout_p channels)
+void dgrad(pb_nn* nn, Acts* a, int l, int level, const bf16* dz, int dz_ct, int dz_co, bf16* din, int din_ct,
+           int din_co, hipStream_t s) {
+  const LayerSpec& L = kLayers[l];
+  const int cout_p = pad16(L.cout_v), cin_p = pad16(L.cin_v);
+  const long npix = (long)a->nimg * a->lv[level].h * a->lv[level].w;
+  const dim3 grd((unsigned)((npix + 63) / 64)), blk(256);
+  const int h = a->lv[level].h, w = a->lv[level].w;
+#define PB_DG(NT)                                                                                                 \
+  hipLaunchKernelGGL(k_conv3x3<NT>, grd, blk, 0, s, dz, dz_ct, dz_co, cout_p, nn->wb[l], 1, nullptr, 0, din, din_ct, \
+                     din_co, cin_p, nullptr, 0, nullptr, a->nimg, h, w)
+  switch (cin_p / 16) {
+    case 1: PB_DG(1); break;
+    case 2: PB_DG(2); break;
+    case 3: PB_DG(3); break;
+    case 4: PB_DG(4); break;
+    case 6: PB_DG(6); break;
+    default: PB_DG(8); break;
+  }
+#undef PB_DG
+}
+
+int wgrad(pb_nn* nn, Acts* a, int l, int level, const bf16* in, int in_ct, int in_co, const bf16* dz, int dz_ct,
+          int dz_co, hipStream_t s) {
+  const LayerSpec& L = kLayers[l];
+  const int cin_p = pad16(L.cin_v), cout_p = pad16(L.cout_v);
+  const long npix = (long)a->nimg * a->lv[level].h * a->lv[level].w;
+  const int h = a->lv[level].h, w = a->lv[level].w;
+  const int ntiles = (cin_p / 16) * (cout_p / 16);
+  int nchunk = (int)std::max<long>(1, std::min<long>(2048 / ntiles, (npix + 1023) / 1024));
+  const long chunk = (npix + nchunk - 1) / nchunk;
+  nchunk = (int)((npix + chunk - 1) / chunk);
+  const long need = (long)nchunk * 9 * cin_p * cout_p;
+  if (need > nn->wpart_cap) return pb_fail(nn->ctx, PB_ERR_STATE, "wgrad partial buffer too small");
+  hipLaunchKernelGGL(k_wgrad, dim3(nchunk, ntiles), dim3(256), 0, s, in, in_ct, in_co, cin_p, dz, dz_ct, dz_co, cout_p,
+                     a->nimg, h, w, chunk, nn->wpart);
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(gs(9L * L.cin_v * L.cout_v)), dim3(256), 0, s, (const float*)nn->wpart,
+                     nchunk, cin_p, cout_p, L.cin_v, L.cout_v, nn->grads + nn->woff[l]);
+  const int nbc = (int)std::min<long>(1024, (npix + 255) / 256);
+  const long bchunk = (npix + nbc - 1) / nbc;
+  hipLaunchKernelGGL(k_bgrad, dim3(nbc), dim3(256), 0, s, dz, dz_ct, dz_co, L.cout_v, npix, bchunk, nn->bpart);
+  hipLaunchKernelGGL(k_bgrad_reduce, dim3(1), dim3(64), 0, s, (const float*)nn->bpart, nbc, L.cout_v,
+                     nn->grads + nn->boff[l]);
+  return PB_OK;
+}
+
+void act_bwd(Acts* a, int level, bf16* d, int ct, int co, int c, const bf16* pre, int pct, int pco, hipStream_t s) {
+  const long npix = (long)a->nimg * a->lv[level].h * a->lv[level].w;
+  hipLaunchKernelGGL(k_act_bwd, dim3(gs(npix * c)), dim3(256), 0, s, d, ct, co, c, pre, pct, pco, npix);
+}
+
+void down(Acts* a, int lf, const bf16* f, int fct, int fco, bf16* c, int cct, int cco, int ch, hipStream_t s) {
+  const Level F = a->lv[lf], Cc = a->lv[lf + 1];
+  const long n = (long)a->nimg * Cc.h * Cc.w * ch;
+  hipLaunchKernelGGL(k_down, dim3(gs(n)), dim3(256), 0, s, f, fct, fco, c, cct, cco, ch, a->nimg, Cc.h, Cc.w, F.h, F.w);
+}
+void down_bwd(Acts* a, int lf, const bf16* dc, int cct, int cco, bf16* df, int fct, int fco, int ch, hipStream_t s) {
+  const Level F = a->lv[lf], Cc = a->lv[lf + 1];
+  const long n = (long)a->nimg * Cc.h * Cc.w * ch;
+  hipLaunchKernelGGL(k_down_bwd, dim3(gs(n)), dim3(256), 0, s, dc, cct, cco, df, fct, fco, ch, a->nimg, Cc.h, Cc.w, F.h,
+                     F.w);
+}
+void up(Acts* a, int lf, const bf16* c, int cct, int cco, bf16* f, int fct, int fco, int ch, hipStream_t s) {
+  const Level F = a->lv[lf], Cc = a->lv[lf + 1];
+  const long n = (long)a->nimg * F.h * F.w * ch;
+  hipLaunchKernelGGL(k_up, dim3(gs(n)), dim3(256), 0, s, c, cct, cco, f, fct, fco, ch, a->nimg, Cc.h, Cc.w, F.h, F.w);
+}
+void up_bwd(Acts* a, int lf, const bf16* df, int fct, int fco, bf16* dc, int cct, int cco, int ch, hipStream_t s) {
+  const Level F = a->lv[lf], Cc = a->lv[lf + 1];
+  const long n = (long)a->nimg * Cc.h * Cc.w * ch;
+  hipLaunchKernelGGL(k_up_bwd, dim3(gs(n)), dim3(256), 0, s, df, fct, fco, dc, cct, cco, ch, a->nimg, Cc.h, Cc.w, F.h,
+                     F.w);
+}
+
+// forward from a->X0 to a->zout (fp32 channel 0); train: keep pre-activations
+void forward(pb_nn* nn, Acts* a, hipStream_t s) {
+  const bool tr = a->train;
+  auto P = [&](bf16* p) { return tr ? p : nullptr; };
+  conv(nn, a, 0, 0, a->X0, 16, 0, a->E0a, 16, 0, P(a->pE0a), nullptr, s);
+  conv(nn, a, 1, 0, a->E0a, 16, 0, a->CAT0, 48, 32, P(a->pE0b), nullptr, s);
+  down(a, 0, a->CAT0, 48, 32, a->P1, 16, 0, 16, s);
+  conv(nn, a, 2, 1, a->P1, 16, 0, a->E1a, 32, 0, P(a->pE1a), nullptr, s);
+  conv(nn, a, 3, 1, a->E1a, 32, 0, a->CAT1, 96, 64, P(a->pE1b), nullptr, s);
+  down(a, 1, a->CAT1, 96, 64, a->P2, 32, 0, 32, s);
+  conv(nn, a, 4, 2, a->P2, 32, 0, a->E2a, 64, 0, P(a->pE2a), nullptr, s);
+  conv(nn, a, 5, 2, a->E2a, 64, 0, a->CAT2, 128, 64, P(a->pE2b), nullptr, s);
+  down(a, 2, a->CAT2, 128, 64, a->P3, 64, 0, 64, s);
+  conv(nn, a, 6, 3, a->P3, 64, 0, a->B3a, 64, 0, P(a->pB3a), nullptr, s);
+  conv(nn, a, 7, 3, a->B3a, 64, 0, a->B3b, 64, 0, P(a->pB3b), nullptr, s);
+  up(a, 2, a->B3b, 64, 0, a->CAT2, 128, 0, 64, s);
+  conv(nn, a, 8, 2, a->CAT2, 128, 0, a->D2a, 64, 0, P(a->pD2a), nullptr, s);
+  conv(nn, a, 9, 2, a->D2a, 64, 0, a->D2b, 64, 0, P(a->pD2b), nullptr, s);
+  up(a, 1, a->D2b, 64, 0, a->CAT1, 96, 0, 64, s);
+  conv(nn, a, 10, 1, a->CAT1, 96, 0, a->D1a, 32, 0, P(a->pD1a), nullptr, s);
+  conv(nn, a, 11, 1, a->D1a, 32, 0, a->D1b, 32, 0, P(a->pD1b), nullptr, s);
+  up(a, 0, a->D1b, 32, 0, a->CAT0, 48, 0, 32, s);
+  conv(nn, a, 12, 0, a->CAT0, 48, 0, a->D0a, 16, 0, P(a->pD0a), nullptr, s);
+  conv(nn, a, 13, 0, a->D0a, 16, 0, a->D0b, 16, 0, P(a->pD0b), nullptr, s);
+  conv(nn, a, 14, 0, a->D0b, 16, 0, nullptr, 16, 0, nullptr, a->zout, s);
+}
+
+// backward from a->dOUT (channel 0 = dL/dz); fills nn->grads
+int backward(pb_nn* nn, Acts* a, hipStream_t s) {
+  int rc = 0;
+  rc |= wgrad(nn, a, 14, 0, a->D0b, 16, 0, a->dOUT, 16, 0, s);
+  dgrad(nn, a, 14, 0, a->dOUT, 16, 0, a->dD0b, 16, 0, s);
+  act_bwd(a, 0, a->dD0b, 16, 0, 16, a->pD0b, 16, 0, s);
+  rc |= wgrad(nn, a, 13, 0, a->D0a, 16, 0, a->dD0b, 16, 0, s);
+  dgrad(nn, a, 13, 0, a->dD0b, 16, 0, a->dD0a, 16, 0, s);
+  act_bwd(a, 0, a->dD0a, 16, 0, 16, a->pD0a, 16, 0, s);
+  rc |= wgrad(nn, a, 12, 0, a->CAT0, 48, 0, a->dD0a, 16, 0, s);
+  dgrad(nn, a, 12, 0, a->dD0a, 16, 0, a->dCAT0, 48, 0, s);
+  up_bwd(a, 0, a->dCAT0, 48, 0, a->dD1b, 32, 0, 32, s);
+  act_bwd(a, 1, a->dD1b, 32, 0, 32, a->pD1b, 32, 0, s);
+  rc |= wgrad(nn, a, 11, 1, a->D1a, 32, 0, a->dD1b, 32, 0, s);
+  dgrad(nn, a, 11, 1, a->dD1b, 32, 0, a->dD1a, 32, 0, s);
+  act_bwd(a, 1, a->dD1a, 32, 0, 32, a->pD1a, 32, 0, s);
+  rc |= wgrad(nn, a, 10, 1, a->CAT1, 96, 0, a->dD1a, 32, 0, s);
+  dgrad(nn, a, 10, 1, a->dD1a, 32, 0, a->dCAT1, 96, 0, s);
+  up_bwd(a, 1, a->dCAT1, 96, 0, a->dD2b, 64, 0, 64, s);
+  act_bwd(a, 2, a->dD2b, 64, 0, 64, a->pD2b, 64, 0, s);
+  rc |= wgrad(nn, a, 9, 2, a->D2a, 64, 0, a->dD2b, 64, 0, s);
+  dgrad(nn, a, 9, 2, a->dD2b, 64, 0, a->dD2a, 64, 0, s);
+  act_bwd(a, 2, a->dD2a, 64, 0, 64, a->pD2a, 64, 0, s);
+  rc |= wgrad(nn, a, 8, 2, a->CAT2, 128, 0, a->dD2a, 64, 0, s);
+  dgrad(nn, a, 8, 2, a->dD2a, 64, 0, a->dCAT2, 128, 0, s);
+  up_bwd(a, 2, a->dCAT2, 128, 0, a->dB3b, 64, 0, 64, s);
+  act_bwd(a, 3, a->dB3b, 64, 0, 64, a->pB3b, 64, 0, s);
+  rc |= wgrad(nn, a, 7, 3, a->B3a, 64, 0, a->dB3b, 64, 0, s);
+  dgrad(nn, a, 7, 3, a->dB3b, 64, 0, a->dB3a, 64, 0, s);
+  act_bwd(a, 3, a->dB3a, 64, 0, 64, a->pB3a, 64, 0, s);
+  rc |= wgrad(nn, a, 6, 3, a->P3, 64, 0, a->dB3a, 64, 0, s);
+  dgrad(nn, a, 6, 3, a->dB3a, 64, 0, a->dP3, 64, 0, s);
+  down_bwd(a, 2, a->dP3, 64, 0, a->dCAT2, 128, 64, 64, s);
+  act_bwd(a, 2, a->dCAT2, 128, 64, 64, a->pE2b, 64, 0, s);
+  rc |= wgrad(nn, a, 5, 2, a->E2a, 64, 0, a->dCAT2, 128, 64, s);
+  dgrad(nn, a, 5, 2, a->dCAT2, 128, 64, a->dE2a, 64, 0, s);
+  act_bwd(a, 2, a->dE2a, 64, 0, 64, a->pE2a, 64, 0, s);
+  rc |= wgrad(nn, a, 4, 2, a->P2, 32, 0, a->dE2a, 64, 0, s);
+  dgrad(nn, a, 4, 2, a->dE2a, 64, 0, a->dP2, 32, 0, s);
+  down_bwd(a, 1, a->dP2, 32, 0, a->dCAT1, 96, 64, 32, s);
+  act_bwd(a, 1, a->dCAT1, 96, 64, 32, a->pE1b, 32, 0, s);
+  rc |= wgrad(nn, a, 3, 1, a->E1a, 32, 0, a->dCAT1, 96, 64, s);
+  dgrad(nn, a, 3, 1, a->dCAT1, 96, 64, a->dE1a, 32, 0, s);
+  act_bwd(a, 1, a->dE1a, 32, 0, 32, a->pE1a, 32, 0, s);
+  rc |= wgrad(nn, a, 2, 1, a->P1, 16, 0, a->dE1a, 32, 0, s);
+  dgrad(nn, a, 2, 1, a->dE1a, 32, 0, a->dP1, 16, 0, s);
+  down_bwd(a, 0, a->dP1, 16, 0, a->dCAT0, 48, 32, 16, s);
+  act_bwd(a, 0, a->dCAT0, 48, 32, 16, a->pE0b, 16, 0, s);
+  rc |= wgrad(nn, a, 1, 0, a->E0a, 16, 0, a->dCAT0, 48, 32, s);
+  dgrad(nn, a, 1, 0, a->dCAT0, 48, 32, a->dE0a, 16, 0, s);
+  act_bwd(a, 0, a->dE0a, 16, 0, 16, a->pE0a, 16, 0, s);
+  rc |= wgrad(nn, a, 0, 0, a->X0, 16, 0, a->dE0a, 16, 0, s);
+  PB_LAUNCH_CHECK(nn->ctx);
+  return rc;
+}
+
+// per-image rms of v -> a->scale
+template <typename Tv>
+void rms_scale(Acts* a, const Tv* v, hipStream_t s) {
+  const long np = (long)a->h * a->w;
+  hipLaunchKernelGGL(k_img_sums<Tv>, dim3(64, a->nimg), dim3(256), 0, s, v, np, a->w, 0, a->part, 64);
+  hipLaunchKernelGGL(k_scale_from, dim3(1), dim3(a->nimg), 0, s, (const double*)a->part, 64, np, a->scale);
+}
+
+// coefficient features of an operator at its nodes
+__global__ void k_op_features(int nx, int ny, const double* wxx, const double* wyy, const double* dinv, float* f1,
+                              float* f2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nx || j >= ny) return;
+  const long c00 = (long)j * (nx + 1) + i, c01 = c00 + 1, c10 = c00 + nx + 1, c11 = c10 + 1;
+  const double d = fabs(dinv[(long)j * nx + i]);
+  f1[(long)j * nx + i] = (float)(d * ((wxx[c00] + wxx[c01]) + (wxx[c10] + wxx[c11])));
+  f2[(long)j * nx + i] = (float)(d * ((wyy[c00] + wyy[c01]) + (wyy[c10] + wyy[c11])));
+}
+
+int op_features(pb_nn* nn, const pb_op2d* op, hipStream_t s) {
+  if (nn->feat_op == op && nn->feat_n == op->n) return PB_OK;
+  if (nn->feat_n < op->n) {
+    int rc = nn_malloc(nn, nn->allocs, &nn->f1, op->n);
+    rc |= nn_malloc(nn, nn->allocs, &nn->f2, op->n);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_op_features, dim3(ceil_div(op->g.nx, 256), op->g.ny), dim3(256), 0, s, op->g.nx, op->g.ny,
+                     op->wxx, op->wyy, op->dinv, nn->f1, nn->f2);
+  PB_LAUNCH_CHECK(nn->ctx);
+  nn->feat_op = op;
+  nn->feat_n = op->n;
+  return PB_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// internal entry used by the flexible Krylov solver: z = M(r) for one image
+// on the operator's grid (r, z FP32 vectors of op->n values)
+// ---------------------------------------------------------------------------
+int pb_nn_apply_f32(pb_nn* nn, const pb_op2d* op, const float* r, float* z, hipStream_t s) {
+  if ((op->g.nx & 1) == 0 || (op->g.ny & 1) == 0)
+    return pb_fail(nn->ctx, PB_ERR_ARG, "network preconditioner: grid dimensions must be odd (2^k + 1 nodes)");
+  int rc = op_features(nn, op, s);
+  if (rc) return rc;
+  Acts* a = get_shape(nn, 1, op->g.ny, op->g.nx, false, &rc);
+  if (!a) return rc;
+  const long np = op->n;
+  rms_scale<float>(a, r, s);
+  hipLaunchKernelGGL(k_input<float>, dim3(gs(np)), dim3(256), 0, s, r, (const float*)a->scale, (const float*)nn->f1,
+                     (const float*)nn->f2, 0, np, 1, a->X0);
+  forward(nn, a, s);
+  hipLaunchKernelGGL(k_output<float>, dim3(gs(np)), dim3(256), 0, s, (const float*)a->zout, (const float*)a->scale, np,
+                     1, z);
+  PB_LAUNCH_CHECK(nn->ctx);
+  return PB_OK;
+}
+
+extern "C" int pb_nn_create(pb_ctx* ctx, unsigned long long seed, pb_nn** out) {
+  if (!ctx || !out) return pb_fail(ctx, PB_ERR_ARG, "pb_nn_create: null argument");
+  pb_nn* nn = new pb_nn();
+  nn->ctx = ctx;
+  long off = 0;
+  for (int l = 0; l < 15; ++l) {
+    nn->woff[l] = off;
+    off += 9L * kLayers[l].cin_v * kLayers[l].cout_v;
+    nn->boff[l] = off;
+    off += kLayers[l].cout_v;
+  }
+  nn->nparam = off;
+  int rc = nn_malloc(nn, nn->allocs, &nn->params, off);
+  rc |= nn_malloc(nn, nn->allocs, &nn->grads, off);
+  rc |= nn_malloc(nn, nn->allocs, &nn->adam_m, off);
+  rc |= nn_malloc(nn, nn->allocs, &nn->adam_v, off);
+  for (int l = 0; l < 15; ++l) {
+    const long sz = 9L * pad16(kLayers[l].cin_v) * pad16(kLayers[l].cout_v);
+    rc |= nn_malloc(nn, nn->allocs, &nn->wf[l], sz);
+    rc |= nn_malloc(nn, nn->allocs, &nn->wb[l], sz);
+  }
+  nn->wpart_cap = 2048L * 9 * 16 * 16 + 9L * 128 * 64;
+  rc |= nn_malloc(nn, nn->allocs, &nn->wpart, nn->wpart_cap);
+  rc |= nn_malloc(nn, nn->allocs, &nn->bpart, 1024L * 64);
+  if (rc) {
+    pb_nn_destroy(nn);
+    return PB_ERR_HIP;
+  }
+  // He-style initialisation (fixed-seed LCG on the host), zero biases
+  std::vector<float> h(off, 0.f);
+  unsigned long long st = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+  auto uni = [&]() {
+    st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+    return (double)(st >> 11) / 9007199254740992.0;
+  };
+  for (int l = 0; l < 15; ++l) {
+    const double fan_in = 9.0 * kLayers[l].cin_v;
+    const double bound = sqrt(6.0 / fan_in) * (l == 14 ? 0.1 : 1.0);
+    for (long k = 0; k < 9L * kLayers[l].cin_v * kLayers[l].cout_v; ++k)
+      h[nn->woff[l] + k] = (float)((2.0 * uni() - 1.0) * bound);
+  }
+  hipStream_t s = ctx->own_stream;
+  if (hipMemcpyAsync(nn->params, h.data(), sizeof(float) * off, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemsetAsync(nn->adam_m, 0, sizeof(float) * off, s) != hipSuccess ||
+      hipMemsetAsync(nn->adam_v, 0, sizeof(float) * off, s) != hipSuccess) {
+    pb_nn_destroy(nn);
+    return pb_fail(ctx, PB_ERR_HIP, "pb_nn_create: initial copy failed");
+  }
+  rc = pack_weights(nn, s);
+  if (rc || hipStreamSynchronize(s) != hipSuccess) {
+    pb_nn_destroy(nn);
+    return PB_ERR_HIP;
+  }
+  *out = nn;
+  return PB_OK;
+}
+
+extern "C" int pb_nn_destroy(pb_nn* nn) {
+  if (!nn) return PB_OK;
+  if (nn->ctx) (void)hipStreamSynchronize(nn->ctx->own_stream);
+  for (Acts* a : nn->shapes) {
+    for (void* p : a->allocs) (void)hipFree(p);
+    delete a;
+  }
+  for (void* p : nn->allocs) (void)hipFree(p);
+  delete nn;
+  return PB_OK;
+}
+
+extern "C" long pb_nn_num_params(const pb_nn* nn) { return nn ? nn->nparam : -1; }
+
+extern "C" int pb_nn_get_params(pb_nn* nn, float* host, float* grads_host) {
+  if (!nn || (!host && !grads_host)) return PB_ERR_ARG;
+  hipStream_t s = nn->ctx->own_stream;
+  if (host) PB_HIP(nn->ctx, hipMemcpyAsync(host, nn->params, sizeof(float) * nn->nparam, hipMemcpyDeviceToHost, s));
+  if (grads_host)
+    PB_HIP(nn->ctx, hipMemcpyAsync(grads_host, nn->grads, sizeof(float) * nn->nparam, hipMemcpyDeviceToHost, s));
+  PB_HIP(nn->ctx, hipStreamSynchronize(s));
+  return PB_OK;
+}
+
+extern "C" int pb_nn_set_params(pb_nn* nn, const float* host) {
+  if (!nn || !host) return PB_ERR_ARG;
+  hipStream_t s = nn->ctx->own_stream;
+  PB_HIP(nn->ctx, hipMemcpyAsync(nn->params, host, sizeof(float) * nn->nparam, hipMemcpyHostToDevice, s));
+  int rc = pack_weights(nn, s);
+  if (rc) return rc;
+  PB_HIP(nn->ctx, hipStreamSynchronize(s));
+  nn->adam_t = 0;
+  return PB_OK;
+}
+
+// Forward (and optionally backward + Adam step) on a batch of nimg images of
+// h x w nodes.  inputs: float [nimg][h*w][3] (r, feature 1, feature 2);
+// target: float [nimg][h*w] (solution / rms(r), null-space free) or NULL.
+// out_z: float [nimg][h*w] network output (unscaled) or NULL.  lr > 0 applies
+// an Adam step; lr == 0 only computes gradients.  loss: mean squared error.
+extern "C" int pb_nn_train_step(pb_nn* nn, int nimg, int h, int w, const float* inputs, const float* target,
+                                double lr, float* out_z, double* loss) {
+  if (!nn || !inputs || nimg < 1 || nimg > 256 || h < 9 || w < 9)
+    return pb_fail(nn ? nn->ctx : nullptr, PB_ERR_ARG, "pb_nn_train_step: bad argument");
+  if ((h & 1) == 0 || (w & 1) == 0) return pb_fail(nn->ctx, PB_ERR_ARG, "pb_nn_train_step: h and w must be odd");
+  hipStream_t s = nn->ctx->own_stream;
+  const bool train = target != nullptr;
+  int rc = 0;
+  Acts* a = get_shape(nn, nimg, h, w, train, &rc);
+  if (!a) return rc;
+  const long np = (long)h * w, n = np * nimg;
+  // inputs (already normalised by the caller) into the 16-channel X0
+  hipLaunchKernelGGL(k_split3, dim3(gs(n)), dim3(256), 0, s, inputs, n, a->X0);
+  forward(nn, a, s);
+  if (out_z) PB_HIP(nn->ctx, hipMemcpyAsync(out_z, a->zout, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+  if (train) {
+    hipLaunchKernelGGL(k_img_sums<float>, dim3(64, nimg), dim3(256), 0, s, (const float*)a->zout, np, w, 1, a->part, 64);
+    hipLaunchKernelGGL(k_null_coef, dim3(1), dim3(nimg), 0, s, (const double*)a->part, 64, np, w, h, a->coef);
+    const int nbl = 1024;
+    hipLaunchKernelGGL(k_loss, dim3(nbl), dim3(256), 0, s, (const float*)a->zout, target, (const double*)a->coef, np,
+                       w, nimg, (float)(1.0 / (double)n), a->dOUT, a->lpart);
+    rc = backward(nn, a, s);
+    if (rc) return rc;
+    if (lr > 0) {
+      nn->adam_t += 1;
+      const float b1 = 0.9f, b2 = 0.999f;
+      const float bc1 = 1.f - powf(b1, (float)nn->adam_t), bc2 = 1.f - powf(b2, (float)nn->adam_t);
+      hipLaunchKernelGGL(k_adam, dim3(gs(nn->nparam)), dim3(256), 0, s, nn->nparam, nn->params, (const float*)nn->grads,
+                         nn->adam_m, nn->adam_v, (float)lr, b1, b2, 1e-8f, bc1, bc2);
+      rc = pack_weights(nn, s);
+      if (rc) return rc;
+    }
+    if (loss) {
+      std::vector<double> lp(nbl);
+      PB_HIP(nn->ctx, hipMemcpyAsync(lp.data(), a->lpart, sizeof(double) * nbl, hipMemcpyDeviceToHost, s));
+      PB_HIP(nn->ctx, hipStreamSynchronize(s));
+      double t = 0.0;
+      for (double v : lp) t += v;
+      *loss = t / (double)n;
+    }
+  }
+  PB_LAUNCH_CHECK(nn->ctx);
+  PB_HIP(nn->ctx, hipStreamSynchronize(s));
+  return PB_OK;
+}
+
+// Network preconditioner on an operator's grid: z = rms(r) * net(r / rms(r),
+// features of op).  r, z: device FP64 vectors of op->n values.
+extern "C" int pb_nn_apply(pb_nn* nn, const pb_op2d* op, const double* r, double* z) {
+  if (!nn || !op || !r || !z) return pb_fail(nn ? nn->ctx : nullptr, PB_ERR_ARG, "pb_nn_apply: null argument");
+  if ((op->g.nx & 1) == 0 || (op->g.ny & 1) == 0)
+    return pb_fail(nn->ctx, PB_ERR_ARG, "pb_nn_apply: grid dimensions must be odd (2^k + 1 nodes)");
+  hipStream_t s = nn->ctx->own_stream;
+  int rc = op_features(nn, op, s);
+  if (rc) return rc;
+  Acts* a = get_shape(nn, 1, op->g.ny, op->g.nx, false, &rc);
+  if (!a) return rc;
+  const long np = op->n;
+  rms_scale<double>(a, r, s);
+  hipLaunchKernelGGL(k_input<double>, dim3(gs(np)), dim3(256), 0, s, r, (const float*)a->scale, (const float*)nn->f1,
+                     (const float*)nn->f2, 0, np, 1, a->X0);
+  forward(nn, a, s);
+  hipLaunchKernelGGL(k_output<double>, dim3(gs(np)), dim3(256), 0, s, (const float*)a->zout, (const float*)a->scale,
+                     np, 1, z);
+  PB_LAUNCH_CHECK(nn->ctx);
+  PB_HIP(nn->ctx, hipStreamSynchronize(s));
+  return PB_OK;
+}
+
+// Coefficient features (two float arrays of op->n values) of an operator.
+extern "C" int pb_nn_features(pb_nn* nn, const pb_op2d* op, float* f1, float* f2) {
+  if (!nn || !op || !f1 || !f2) return pb_fail(nn ? nn->ctx : nullptr, PB_ERR_ARG, "pb_nn_features: null argument");
+  hipStream_t s = nn->ctx->own_stream;
+  int rc = op_features(nn, op, s);
+  if (rc) return rc;
+  PB_HIP(nn->ctx, hipMemcpyAsync(f1, nn->f1, sizeof(float) * op->n, hipMemcpyDeviceToDevice, s));
+  PB_HIP(nn->ctx, hipMemcpyAsync(f2, nn->f2, sizeof(float) * op->n, hipMemcpyDeviceToDevice, s));
+  PB_HIP(nn->ctx, hipStreamSynchronize(s));
+  return PB_OK;
+}

@@ -123,7 +123,7 @@ class Operator2D:
         return b
 
     def solve(self, b, x=None, tol=1e-8, maxiter=6000, mode="f64", inner_tol=1e-3, max_outer=12,
-              check_every=32, x0=None):
+              check_every=32, x0=None, fallback_maxiter=6000):
         torch = _torch()
         if x is None:
             x = torch.empty(self.ny, self.nx, dtype=torch.float64, device="cuda")
@@ -132,13 +132,19 @@ class Operator2D:
             x.copy_(x0)
             given = 1
         opts = _abi.SolveOpts(_MODES[mode], float(tol), int(maxiter), float(inner_tol), int(max_outer),
-                              int(check_every), given)
+                              int(check_every), given, int(fallback_maxiter))
         info = _abi.SolveInfo()
         self.ctx.sync_stream()
         rc = self.ctx.lib.pb_bicgstab_solve(self.h, _abi.ptr(b), _abi.ptr(x), C.byref(opts), C.byref(info))
         if rc not in (-10, -11):   # breakdown codes are solver info, not errors
             self.ctx.check(rc, "pb_bicgstab_solve")
         return x, info.as_dict()
+
+    def attach_nn(self, net):
+        """Use ``net`` (pybella_amd.nn.Network) for mode="nn" solves (None detaches)."""
+        self.ctx.check(self.ctx.lib.pb_op2d_attach_nn(self.h, net.h if net is not None else None),
+                       "pb_op2d_attach_nn")
+        self._net = net
 
     def set_tile(self, rows_per_thread):
         self.ctx.check(self.ctx.lib.pb_op2d_set_tile(self.h, int(rows_per_thread)), "pb_op2d_set_tile")
