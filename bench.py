@@ -114,8 +114,9 @@ def main():
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(a solve shards by domain decomposition; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path "
+              "(a solve would shard by domain decomposition into row slabs). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

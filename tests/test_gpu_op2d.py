@@ -14,46 +14,14 @@ from oracle import laplacian as olp
 from oracle import projection as opj
 from pybella_amd import workloads as W
 
+from _cases import make_case, oracle_op, rel
+
 pytestmark = pytest.mark.gpu
 
 
 def _sp():
     from pybella_amd import second_projection as sp
     return sp
-
-
-def rel(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
-    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
-
-
-def make_case(kind, nx, ny):
-    if kind == "strat_psinc":
-        Sol, mpv, elem, node, ud, th = W.random_momentum_state(nx, ny, seed=1)
-    elif kind == "strat_comp":
-        Sol, mpv, elem, node, ud, th = W.random_momentum_state(nx, ny, seed=2, compressible=True)
-    elif kind == "coriolis":
-        Sol, mpv, elem, node, ud, th = W.random_momentum_state(nx, ny, seed=3, compressible=True)
-        ud.omega = 7.292e-5 * 50
-        ud.derive()
-    elif kind == "xwall":
-        Sol, mpv, elem, node, ud, th = W.random_momentum_state(nx, ny, seed=4, compressible=True)
-        ud.bdry_type[0] = W.BdryType.WALL
-    else:
-        raise ValueError(kind)
-    return Sol, mpv, elem, node, ud, th
-
-
-def oracle_op(Sol, mpv, elem, node, ud, th, dt):
-    """Oracle assembly on deep copies (no regime handling beyond the reference's)."""
-    Sol, mpv = copy.deepcopy(Sol), copy.deepcopy(mpv)
-    opj.operator_coefficients_nodes(elem, node, Sol, mpv, ud, th, dt)
-    if ud.is_compressible == 0:
-        mpv.wcenter[...] *= ud.compressibility
-    cor = opj.multiply_inverse_coriolis(mpv, Sol, mpv, ud, elem, node, dt, attrs=("u", "v", "w"), get_coeffs=True)
-    dinv = olp.precon_diag_prepare(mpv, elem, node, ud)
-    return olp.op2d_arrays(mpv, node, cor, dinv, ud), mpv, dinv
 
 
 @pytest.mark.parametrize("kind", ["strat_psinc", "strat_comp", "coriolis", "xwall"])
