@@ -43,6 +43,15 @@ KERNEL_XBYTES = {1: 16}
 KERNEL_NAMES = {0: "matvec", 1: "bicg_fast_A_pupdate_matvec", 2: "bicg_B_s_matvec_dots", 3: "true_residual_f64"}
 
 
+SOLVER_NAMES = {
+    "mixed": "plain BiCGStab, FP32 Krylov (FP64 iterate) in FP64 iterative refinement",
+    "f64": "plain BiCGStab FP64 (reference semantics, 3 launches/iteration)",
+    "f64fast": "plain BiCGStab FP64 (rho, ||r|| from s/t dots, 2 launches/iteration)",
+    "nn": "learned preconditioner: BF16 network initial guess + damped Jacobi, flexible GMRES budget, "
+          "then plain BiCGStab FP64 to the same true-residual tolerance",
+}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -101,7 +110,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="auto", choices=["auto", "f64", "f64fast", "mixed"])
+    ap.add_argument("--mode", default="auto", choices=["auto", "nn", "f64", "f64fast", "mixed"],
+                    help="auto = the learned-preconditioned solve timed beside the fastest plain mode")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=300)
     ap.add_argument("--kernel-reps", type=int, default=200)
@@ -159,45 +169,85 @@ def main():
                           "alg_GBps": alg / (us * 1e-6) / 1e9}))
         return
 
-    def step(mode):
+    nn_net = None
+    if args.mode in ("auto", "nn"):
+        from pybella_amd.nn import Network, default_weights_path
+        nn_net = Network(seed=0)
+        nn_net.load(default_weights_path())
+        op.attach_nn(nn_net)
+
+    def step(mode, nn_cfg=None):
         op.assemble(phys, dt, rho, rhoY, rho, rhoY, dS)
         op.scale_rhs(rhs_d, b)
-        _, info = op.solve(b, x, tol=1e-8, maxiter=6000, mode=mode, inner_tol=1e-3)
+        if mode == "nn":
+            _, info = op.solve(b, x, tol=1e-8, mode="nn", fallback_maxiter=6000, **nn_cfg)
+        else:
+            _, info = op.solve(b, x, tol=1e-8, maxiter=6000, mode=mode, inner_tol=1e-3)
         return info
+
+    def rate_of(fn, reps=2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return reps / (time.perf_counter() - t0)
 
     from pybella_amd.autotune import tune_tile
     # plain FP64 reference-semantics solve (iteration count, parity anchor)
     info64 = step("f64")
     log("f64 (exact) solve:", info64)
-    # calibrate every plain mode (tile height autotuned per mode), pick the fastest
+    # calibrate every plain mode (tile height autotuned per mode)
     rates, tiles, infos_cal = {}, {}, {}
-    modes = ["f64", "f64fast", "mixed"] if args.mode == "auto" else [args.mode]
-    for m in modes:
+    plain_modes = ["f64", "f64fast", "mixed"] if args.mode in ("auto", "nn") else [args.mode]
+    for m in plain_modes:
         best, table = tune_tile(op, lambda: step(m), key=(grid.nx, grid.ny, m))
         infos_cal[m] = step(m)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        step(m)
-        torch.cuda.synchronize()
-        rates[m] = 1.0 / (time.perf_counter() - t0)
+        rates[m] = rate_of(lambda: step(m))
         tiles[m] = {"rows_per_thread": best, "seconds": {str(k): round(v, 5) for k, v in table.items()}}
         log(f"mode {m}: {rates[m]:.2f} solves/s, tile rows/thread {best}, {infos_cal[m]}")
-    mode = max(rates, key=rates.get)
-    op.set_tile(tiles[mode]["rows_per_thread"])
-    for _ in range(args.warmup):
-        step(mode)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    infos = []
-    for _ in range(args.steps):
-        infos.append(step(mode))
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    log("timed:", mode, infos[-1])
-    for inf in infos:
-        if inf["info"] != 0 or inf["true_rel_resid"] > 1e-8:
-            raise SystemExit(f"solve did not reach the tolerance: {inf}")
+    plain = max(rates, key=rates.get)
+    # the learned-preconditioned solve: network initial guess (+ damped Jacobi),
+    # flexible GMRES for ``maxiter`` iterations, then plain BiCGStab (two-launch
+    # iteration at the fast mode's tile) to the same true-residual tolerance
+    nn_cfg, nn_table = None, {}
+    if nn_net is not None:
+        op.set_tile(tiles.get("f64fast", tiles[plain])["rows_per_thread"])
+        cands = [dict(maxiter=0, sweeps=2), dict(maxiter=0, sweeps=0), dict(maxiter=10, sweeps=2),
+                 dict(maxiter=40, sweeps=1)]
+        for c in cands:
+            step("nn", c)
+            r = rate_of(lambda: step("nn", c))
+            nn_table[json.dumps(c, sort_keys=True)] = round(r, 3)
+            log(f"mode nn {c}: {r:.2f} solves/s")
+        nn_cfg = json.loads(max(nn_table, key=nn_table.get))
+    mode = "nn" if nn_cfg is not None else plain
+    if mode != "nn":
+        op.set_tile(tiles[mode]["rows_per_thread"])
+
+    def timed(m, cfg):
+        for _ in range(args.warmup):
+            step(m, cfg)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infs = [step(m, cfg) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        for inf in infs:
+            if inf["info"] != 0 or inf["true_rel_resid"] > 1e-8:
+                raise SystemExit(f"solve did not reach the tolerance: {inf}")
+        return el, infs
+
+    el, infos = timed(mode, nn_cfg)
+    log("timed:", mode, nn_cfg, infos[-1])
     value = args.steps / el
+    plain_rate, plain_info = None, None
+    if mode == "nn":   # the plain BiCGStab it is compared with, timed the same way
+        op.set_tile(tiles[plain]["rows_per_thread"])
+        el_p, infos_p = timed(plain, None)
+        plain_rate, plain_info = args.steps / el_p, infos_p[-1]
+        log("timed plain:", plain, plain_info)
+        op.set_tile(tiles.get("f64fast", tiles[plain])["rows_per_thread"])
 
     # roofline of the solver kernels (HIP events, same stream)
     prec = "f32" if mode == "mixed" else "f64"
@@ -212,7 +262,8 @@ def main():
     try:   # HBM bytes per launch from the committed rocprofv3 counter summary
         with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_counters.json")) as f:
             hc = json.load(f)
-        rec = hc["kernels"].get(f"{KERNEL_NAMES[dom]}/{prec}/{tiles[mode]['rows_per_thread']}")
+        tile_now = tiles["f64fast" if mode == "nn" and "f64fast" in tiles else (plain if mode == "nn" else mode)]
+        rec = hc["kernels"].get(f"{KERNEL_NAMES[dom]}/{prec}/{tile_now['rows_per_thread']}")
         if rec:
             traffic = int(1024 * (rec["fetch_kib"] * hc["fetch_scale"] + rec["write_kib"] * hc["write_scale"]))
     except (OSError, KeyError, ValueError):
@@ -238,17 +289,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if mode == "mixed" else "f64",
+        "dtype": {"mixed": "f32", "nn": "f64+bf16"}.get(mode, "f64"),
         "data": "synthetic",
         "config": {"workload": "isolated 2D nodal 9-point solve, 1025x513 nodes, stratified theta 300->330 K, "
                                "psinc, seeded 64-mode RHS, true rel. resid <= 1e-8",
-                   "solver": {"mixed": "plain BiCGStab, FP32 Krylov (FP64 iterate) in FP64 iterative refinement",
-                              "f64": "plain BiCGStab FP64 (reference semantics, 3 launches/iteration)",
-                              "f64fast": "plain BiCGStab FP64 (rho, ||r|| from s/t dots, 2 launches/iteration)"}[mode],
+                   "solver": SOLVER_NAMES[mode],
+                   "nn_config": nn_cfg,
                    "iterations": infos[-1]["iters"], "refinements": infos[-1]["outer"],
                    "true_rel_resid": infos[-1]["true_rel_resid"],
                    "f64_iterations": info64["iters"],
-                   "plain_modes_solves_per_s": {k: round(v, 3) for k, v in rates.items()},
+                   "plain_bicgstab": None if plain_rate is None else {
+                       "solver": SOLVER_NAMES[plain], "solves_per_s": round(plain_rate, 3),
+                       "iterations": plain_info["iters"], "true_rel_resid": plain_info["true_rel_resid"]},
+                   "nn_over_plain": None if plain_rate is None else round(value / plain_rate, 3),
+                   "calibration_solves_per_s": {**{k: round(v, 3) for k, v in rates.items()},
+                                                **{"nn " + k: v for k, v in nn_table.items()}},
                    "tiles": tiles},
         "roofline": roof,
         "cpu_baseline": cpu,

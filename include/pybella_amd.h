@@ -88,6 +88,12 @@ typedef struct pb_solve_opts {
   int check_every;     /* iterations between host convergence checks (0=auto) */
   int x0_given;        /* 0: x0 = 0 as the reference; 1: use x as the guess   */
   int fallback_maxiter; /* PB_SOLVE_NN: BiCGStab iterations after the budget  */
+  int krylov_m;        /* PB_SOLVE_NN: FGMRES restart length (<=0: 40)         */
+  int sweeps;          /* PB_SOLVE_NN: damped-Jacobi sweeps after the network
+                          (<0: default 1)                                       */
+  double omega;        /* PB_SOLVE_NN: Jacobi damping (<=0: 0.6)               */
+  int use_network;     /* PB_SOLVE_NN: 1 network + Jacobi; 0 Jacobi only (the
+                          same flexible GMRES without the network, for A/B)    */
 } pb_solve_opts;
 
 typedef struct pb_solve_info {
@@ -183,7 +189,8 @@ int pb_scatter_update_p2(pb_ctx* ctx, const pb_grid2d* grid, const double* x,
  * Fixed architecture: 4-level convolutional encoder-decoder, 3x3 convolutions,
  * 16-32-64-64 channels, GELU, 15 layers, BF16 MFMA kernels with FP32
  * accumulation and FP32 master parameters (layout per layer: W[9][cin][cout]
- * then bias[cout]).  Grids must have odd node counts (2^k + 1). */
+ * then bias[cout]).  Grids must have odd node counts (2^k + 1).  Every
+ * network call below runs on the context's stream (pb_ctx_set_stream). */
 int pb_nn_create(pb_ctx* ctx, unsigned long long seed, pb_nn** out);
 int pb_nn_destroy(pb_nn* nn);
 long pb_nn_num_params(const pb_nn* nn);
@@ -191,12 +198,13 @@ long pb_nn_num_params(const pb_nn* nn);
 int pb_nn_get_params(pb_nn* nn, float* host_params, float* host_grads);
 int pb_nn_set_params(pb_nn* nn, const float* host_params);
 /* One batch: inputs float[nimg][h*w][3] (normalised r, feature 1, feature 2),
- * target float[nimg][h*w] or NULL (forward only), out_z float[nimg][h*w] or
- * NULL (device pointers).  With a target: gradients of the mean squared error
- * of the output (constant and checkerboard components removed) and, if lr > 0,
- * one Adam step.  Synchronous. */
-int pb_nn_train_step(pb_nn* nn, int nimg, int h, int w, const float* inputs, const float* target, double lr,
-                     float* out_z, double* loss);
+ * target float[nimg][h*w] or NULL (forward only), img_weight float[nimg] or
+ * NULL (all 1), out_z float[nimg][h*w] or NULL (device pointers).  With a
+ * target: gradients of the per-image weighted mean squared error of the output
+ * (constant and checkerboard components removed) and, if lr > 0, one Adam
+ * step.  Synchronous. */
+int pb_nn_train_step(pb_nn* nn, int nimg, int h, int w, const float* inputs, const float* target,
+                     const float* img_weight, double lr, float* out_z, double* loss);
 /* z = rms(r) * net(r / rms(r), features(op)); r, z: FP64 device vectors. */
 int pb_nn_apply(pb_nn* nn, const pb_op2d* op, const double* r, double* z);
 /* the two coefficient features (float device arrays of nx*ny) of an operator */

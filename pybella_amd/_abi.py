@@ -36,7 +36,8 @@ class Phys(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("tol", C.c_double), ("maxiter", C.c_int),
                 ("inner_tol", C.c_double), ("max_outer", C.c_int), ("check_every", C.c_int),
-                ("x0_given", C.c_int), ("fallback_maxiter", C.c_int)]
+                ("x0_given", C.c_int), ("fallback_maxiter", C.c_int), ("krylov_m", C.c_int),
+                ("sweeps", C.c_int), ("omega", C.c_double), ("use_network", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -79,7 +80,7 @@ SIGNATURES = {
     "pb_nn_num_params": (C.c_long, [VP]),
     "pb_nn_get_params": (I, [VP, VP, VP]),
     "pb_nn_set_params": (I, [VP, VP]),
-    "pb_nn_train_step": (I, [VP, I, I, I, VP, VP, D, VP, C.POINTER(D)]),
+    "pb_nn_train_step": (I, [VP, I, I, I, VP, VP, VP, D, VP, C.POINTER(D)]),
     "pb_nn_apply": (I, [VP, VP, VP, VP]),
     "pb_nn_features": (I, [VP, VP, VP, VP]),
     "pb_op2d_attach_nn": (I, [VP, VP]),

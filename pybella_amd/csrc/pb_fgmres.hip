@@ -12,7 +12,6 @@
 // starts from x0 = M(b) (the network's initial guess).  If the iteration
 // budget runs out before the tolerance, plain BiCGStab finishes from the
 // current iterate.
-#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -31,6 +30,7 @@ struct FgWork {
   float* Z = nullptr;     // m x n
   float* w = nullptr;
   float* t = nullptr;
+  float* r32 = nullptr;   // FP32 copy of the outer residual
   float* diag = nullptr;  // exact diagonal of A (FP32)
   double* part = nullptr; // dot partials
   double* h = nullptr;    // device dot results (m+2)
@@ -127,6 +127,57 @@ __global__ void k_jacobi(long n, const float* v, const float* t, const float* d,
     z[g] += omega * (v[g] - t[g]) / d[g];
 }
 
+// z = omega v / diag (first Jacobi step from z = 0)
+__global__ void k_jacobi0(long n, const float* v, const float* d, float omega, float* z) {
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
+    z[g] = omega * v[g] / d[g];
+}
+
+// partial sums of v.t and t.t (one pair per block)
+__global__ void k_dot2(long n, const float* v, const float* t, double* part) {
+  double a = 0.0, b = 0.0;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+    const double tv = (double)t[g];
+    a += (double)v[g] * tv;
+    b += tv * tv;
+  }
+  __shared__ double sh[2][256];
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0][0];
+    part[2 * blockIdx.x + 1] = sh[1][0];
+  }
+}
+
+// alpha = (v.t) / (t.t) minimises ||v - alpha A z|| (every block re-reduces the
+// partials in the same order, so all blocks see the same alpha); then
+// z = alpha z + omega (v - alpha t) / diag
+__global__ void k_scale_jacobi(long n, const double* part, int nparts, const float* v, const float* t, const float* d,
+                               float omega, float* z) {
+  __shared__ float alpha_s;
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < nparts; ++k) {
+      a += part[2 * k];
+      b += part[2 * k + 1];
+    }
+    alpha_s = b > 0.0 ? (float)(a / b) : 1.f;
+  }
+  __syncthreads();
+  const float alpha = alpha_s;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
+    z[g] = alpha * z[g] + omega * (v[g] - alpha * t[g]) / d[g];
+}
+
 __global__ void k_cast_df(long n, const double* a, float* b) {
   for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) b[g] = (float)a[g];
 }
@@ -164,6 +215,7 @@ int fg_alloc(pb_op2d* op, int m, FgWork** out) {
   rc |= A(&f->Z, (long)m * op->n);
   rc |= A(&f->w, op->n);
   rc |= A(&f->t, op->n);
+  rc |= A(&f->r32, op->n);
   rc |= A(&f->diag, op->n);
   rc |= A(&f->part, (long)(m + 2) * kDotBlocks);
   rc |= A(&f->h, m + 2);
@@ -189,19 +241,47 @@ int fnorm2(pb_op2d* op, FgWork* f, const float* v, hipStream_t s, double* out) {
   return PB_OK;
 }
 
-// z = M(v): network then ``sweeps`` damped-Jacobi steps on A z = v
-int precond(pb_op2d* op, pb_nn* nn, FgWork* f, const float* v, float* z, int sweeps, float omega, hipStream_t s) {
+// z = M(v): the network, then ``sweeps`` damped-Jacobi steps on A z = v; the
+// first sweep also rescales the network output by the alpha that minimises
+// ||v - alpha A z|| (computed on the device, no host round trip).  v, z and
+// f->t must be distinct buffers.  With ``rescale`` the alpha step is done even
+// when sweeps == 0 (the initial guess needs the right magnitude; flexible GMRES
+// directions do not).
+int precond(pb_op2d* op, pb_nn* nn, FgWork* f, const float* v, float* z, int sweeps, float omega, bool rescale,
+            hipStream_t s) {
+  const int nb = elem_blocks(op->n);
+  if (!nn) {   // Jacobi only: z = omega v / diag, then the remaining sweeps
+    hipLaunchKernelGGL(k_jacobi0, dim3(nb), dim3(256), 0, s, op->n, v, (const float*)f->diag, omega, z);
+    for (int k = 1; k < sweeps; ++k) {
+      void* saved = op->ctx->stream;
+      op->ctx->stream = s;
+      int rc = pb_op2d_matvec(op, z, f->t, PB_F32);
+      op->ctx->stream = (hipStream_t)saved;
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(256), 0, s, op->n, v, (const float*)f->t, (const float*)f->diag,
+                         omega, z);
+    }
+    PB_LAUNCH_CHECK(op->ctx);
+    return PB_OK;
+  }
   int rc = pb_nn_apply_f32(nn, op, v, z, s);
   if (rc) return rc;
-  const int nb = elem_blocks(op->n);
-  for (int k = 0; k < sweeps; ++k) {
+  const int steps = (sweeps == 0 && rescale) ? 1 : sweeps;
+  if (sweeps == 0) omega = 0.f;
+  for (int k = 0; k < steps; ++k) {
     void* saved = op->ctx->stream;
     op->ctx->stream = s;
     rc = pb_op2d_matvec(op, z, f->t, PB_F32);
     op->ctx->stream = (hipStream_t)saved;
     if (rc) return rc;
-    hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(256), 0, s, op->n, v, (const float*)f->t, (const float*)f->diag, omega,
-                       z);
+    if (k == 0) {
+      hipLaunchKernelGGL(k_dot2, dim3(kDotBlocks), dim3(256), 0, s, op->n, v, (const float*)f->t, f->part);
+      hipLaunchKernelGGL(k_scale_jacobi, dim3(nb), dim3(256), 0, s, op->n, (const double*)f->part, kDotBlocks, v,
+                         (const float*)f->t, (const float*)f->diag, omega, z);
+    } else {
+      hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(256), 0, s, op->n, v, (const float*)f->t, (const float*)f->diag,
+                         omega, z);
+    }
   }
   PB_LAUNCH_CHECK(op->ctx);
   return PB_OK;
@@ -229,7 +309,7 @@ int fgmres_cycle(pb_op2d* op, pb_nn* nn, FgWork* f, const float* r32, double* x,
   for (; j < m; ++j) {
     float* vj = f->V + (long)j * n;
     float* zj = f->Z + (long)j * n;
-    rc = precond(op, nn, f, vj, zj, sweeps, omega, s);
+    rc = precond(op, nn, f, vj, zj, sweeps, omega, false, s);
     if (rc) return rc;
     void* saved = op->ctx->stream;
     op->ctx->stream = s;
@@ -296,6 +376,18 @@ int fgmres_cycle(pb_op2d* op, pb_nn* nn, FgWork* f, const float* r32, double* x,
 
 }  // namespace
 
+// frees the flexible-GMRES workspace of an operator (called on its destroy)
+void pb_fgmres_release(const pb_op2d* op) {
+  auto it = g_work.find(op);
+  if (it != g_work.end()) {
+    for (void* p : it->second->allocs) (void)hipFree(p);
+    if (it->second->hh) (void)hipHostFree(it->second->hh);
+    delete it->second;
+    g_work.erase(it);
+  }
+  g_attached.erase(op);
+}
+
 extern "C" int pb_op2d_attach_nn(pb_op2d* op, pb_nn* nn) {
   if (!op) return PB_ERR_ARG;
   if (nn) g_attached[op] = nn;
@@ -309,15 +401,17 @@ extern "C" int pb_op2d_attach_nn(pb_op2d* op, pb_nn* nn) {
 int pb_nn_solve(pb_op2d* op, const double* b, double* x, const pb_solve_opts* opts, pb_solve_info* info,
                 double (*true_res)(pb_op2d*, const double*, const double*, hipStream_t, int*), hipStream_t s) {
   pb_ctx* ctx = op->ctx;
-  auto it = g_attached.find(op);
-  if (it == g_attached.end()) return pb_fail(ctx, PB_ERR_STATE, "PB_SOLVE_NN: no network attached to the operator");
-  pb_nn* nn = it->second;
-  const char* em = std::getenv("PYBELLA_AMD_FGMRES_M");
-  const char* es = std::getenv("PYBELLA_AMD_NN_SWEEPS");
-  const char* eo = std::getenv("PYBELLA_AMD_NN_OMEGA");
-  const int m = em ? std::max(4, std::min(200, atoi(em))) : 40;
-  const int sweeps = es ? std::max(0, atoi(es)) : 2;
-  const float omega = eo ? (float)atof(eo) : 0.6f;
+  pb_nn* nn = nullptr;
+  if (opts->use_network) {
+    auto it = g_attached.find(op);
+    if (it == g_attached.end()) return pb_fail(ctx, PB_ERR_STATE, "PB_SOLVE_NN: no network attached to the operator");
+    nn = it->second;
+  }
+  const int m = opts->krylov_m > 0 ? std::max(4, std::min(200, opts->krylov_m)) : 40;
+  const int sweeps = opts->sweeps >= 0 ? std::min(opts->sweeps, 16) : 1;
+  const float omega = opts->omega > 0 ? (float)opts->omega : 0.6f;
+  if (!opts->use_network && sweeps == 0)
+    return pb_fail(ctx, PB_ERR_ARG, "PB_SOLVE_NN without the network needs at least one Jacobi sweep");
   FgWork* f = nullptr;
   int rc = fg_alloc(op, m, &f);
   if (rc) return rc;
@@ -332,23 +426,22 @@ int pb_nn_solve(pb_op2d* op, const double* b, double* x, const pb_solve_opts* op
   // ||b|| and the initial guess x0 = M(b) (or the caller's x0)
   const double bn = true_res(op, b, op->zero, s, &rcode);
   if (rcode) return rcode;
-  if (!opts->x0_given) {
-    hipLaunchKernelGGL(k_cast_df, dim3(nb), dim3(256), 0, s, n, b, f->w);
-    rc = precond(op, nn, f, f->w, f->t, sweeps, omega, s);
+  if (!opts->x0_given) {   // V[0], Z[0] are free until the first cycle
+    hipLaunchKernelGGL(k_cast_df, dim3(nb), dim3(256), 0, s, n, b, f->V);
+    rc = precond(op, nn, f, f->V, f->Z, sweeps, omega, true, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_cast_fd, dim3(nb), dim3(256), 0, s, n, (const float*)f->t, x);
+    hipLaunchKernelGGL(k_cast_fd, dim3(nb), dim3(256), 0, s, n, (const float*)f->Z, x);
   }
   double rn = true_res(op, b, x, s, &rcode);   // op->rtrue = b - A x
   if (rcode) return rcode;
   const double target = opts->tol * bn;
   int total = 0, outer = 0;
   while (rn > target && total < budget && outer < 50) {
-    hipLaunchKernelGGL(k_cast_df, dim3(nb), dim3(256), 0, s, n, (const double*)op->rtrue, f->w);
-    PB_HIP(ctx, hipMemcpyAsync(f->t, f->w, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_cast_df, dim3(nb), dim3(256), 0, s, n, (const double*)op->rtrue, f->r32);
     int its = 0;
     double est = 0.0;
     const double tol_rel = std::max(inner, 0.5 * target / rn);
-    rc = fgmres_cycle(op, nn, f, f->t, x, std::min(m, budget - total), tol_rel, sweeps, omega, s, &its, &est);
+    rc = fgmres_cycle(op, nn, f, f->r32, x, std::min(m, budget - total), tol_rel, sweeps, omega, s, &its, &est);
     if (rc) return rc;
     total += its;
     ++outer;

@@ -854,7 +854,8 @@ extern "C" int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x, const 
   pb_ctx* ctx = op->ctx;
   if (!b || !x || !opts || !info) return pb_fail(ctx, PB_ERR_ARG, "pb_bicgstab_solve: null argument");
   if (!op->assembled) return pb_fail(ctx, PB_ERR_STATE, "pb_bicgstab_solve: operator not assembled");
-  if (!(opts->tol >= 0) || opts->maxiter < 1)
+  // PB_SOLVE_NN accepts a zero flexible-GMRES budget: initial guess M(b), then BiCGStab
+  if (!(opts->tol >= 0) || opts->maxiter < (opts->mode == PB_SOLVE_NN ? 0 : 1))
     return pb_fail(ctx, PB_ERR_ARG, "pb_bicgstab_solve: bad tol/maxiter");
   int rc = alloc_work(op);
   if (rc) return rc;
@@ -931,15 +932,24 @@ extern "C" int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x, const 
   } else if (opts->mode == PB_SOLVE_NN) {
     rc = pb_nn_solve(op, b, x, opts, &out, pb2d_true_residual, s);
     if (rc) return rc;
-    if (out.info != 0) {   // budget exhausted: plain BiCGStab from the current iterate
-      KState h{};
+    if (out.info != 0) {
+      // budget exhausted: plain BiCGStab (two-launch iteration) from the
+      // current iterate, restarted from the new iterate after a breakdown
+      // (near convergence rho = r~.r can fall below the absolute eps^2 test)
       const int fmax = opts->fallback_maxiter > 0 ? opts->fallback_maxiter : 6000;
-      rc = krylov_run<double>(op, op->w64, b, x, opts->tol, fmax, 0, opts->check_every, s, &h, false);
-      if (rc) return rc;
-      PB_HIP(ctx, hipMemcpyAsync(x, op->w64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-      out.iters += h.iter;
       out.fallback = 1;
-      out.info = info_of(h);
+      int left = fmax;
+      for (int restart = 0; restart < 8 && left > 0; ++restart) {
+        KState h{};
+        rc = krylov_run<double>(op, op->w64, b, x, opts->tol, left, 0, opts->check_every, s, &h, false);
+        if (rc) return rc;
+        PB_HIP(ctx, hipMemcpyAsync(x, op->w64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        out.iters += h.iter;
+        left -= std::max(h.iter, 1);
+        out.info = info_of(h);
+        out.resid = h.resid;
+        if (out.info >= 0) break;
+      }
     }
   } else {
     return pb_fail(ctx, PB_ERR_ARG, "pb_bicgstab_solve: mode %d not available on this handle", opts->mode);

@@ -371,8 +371,8 @@ __global__ void k_null_coef(const double* part, int nchunk, long npix_img, int w
 }
 
 // loss: z projected off the null space vs target; gradient into channel 0 of dout
-__global__ void k_loss(const float* z, const float* target, const double* coef, long npix_img, int w, int nimg,
-                       float inv_n, bf16* dout, double* lpart) {
+__global__ void k_loss(const float* z, const float* target, const double* coef, const float* wimg, long npix_img,
+                       int w, int nimg, float inv_n, bf16* dout, double* lpart) {
   const long n = npix_img * nimg;
   double l = 0.0;
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
@@ -381,9 +381,10 @@ __global__ void k_loss(const float* z, const float* target, const double* coef, 
     const double c = ((y + x) & 1) ? -1.0 : 1.0;
     const double zp = (double)z[k] - coef[2 * b] - coef[2 * b + 1] * c;
     const double d = zp - (double)target[k];
-    l += d * d;
+    const double wb = wimg ? (double)wimg[b] : 1.0;
+    l += wb * d * d;
     bf16* o = dout + k * 16;
-    o[0] = f2bf((float)(2.0 * d) * inv_n);
+    o[0] = f2bf((float)(2.0 * wb * d) * inv_n);
 #pragma unroll
     for (int ch = 1; ch < 16; ++ch) o[ch] = 0;
   }
@@ -517,7 +518,7 @@ struct pb_nn {
   const pb_op2d* feat_op = nullptr;
   float* f1 = nullptr;
   float* f2 = nullptr;
-  long feat_n = 0;
+  long feat_n = 0, feat_cap = 0, feat_version = -1;
 };
 
 namespace {
@@ -827,16 +828,18 @@ __global__ void k_op_features(int nx, int ny, const double* wxx, const double* w
 }
 
 int op_features(pb_nn* nn, const pb_op2d* op, hipStream_t s) {
-  if (nn->feat_op == op && nn->feat_n == op->n) return PB_OK;
-  if (nn->feat_n < op->n) {
+  if (nn->feat_op == op && nn->feat_version == op->version && nn->feat_n == op->n) return PB_OK;
+  if (nn->feat_cap < op->n) {
     int rc = nn_malloc(nn, nn->allocs, &nn->f1, op->n);
     rc |= nn_malloc(nn, nn->allocs, &nn->f2, op->n);
     if (rc) return rc;
+    nn->feat_cap = op->n;
   }
   hipLaunchKernelGGL(k_op_features, dim3(ceil_div(op->g.nx, 256), op->g.ny), dim3(256), 0, s, op->g.nx, op->g.ny,
                      op->wxx, op->wyy, op->dinv, nn->f1, nn->f2);
   PB_LAUNCH_CHECK(nn->ctx);
   nn->feat_op = op;
+  nn->feat_version = op->version;
   nn->feat_n = op->n;
   return PB_OK;
 }
@@ -924,7 +927,10 @@ extern "C" int pb_nn_create(pb_ctx* ctx, unsigned long long seed, pb_nn** out) {
 
 extern "C" int pb_nn_destroy(pb_nn* nn) {
   if (!nn) return PB_OK;
-  if (nn->ctx) (void)hipStreamSynchronize(nn->ctx->own_stream);
+  if (nn->ctx) {
+    (void)hipStreamSynchronize(nn->ctx->stream);
+    (void)hipStreamSynchronize(nn->ctx->own_stream);
+  }
   for (Acts* a : nn->shapes) {
     for (void* p : a->allocs) (void)hipFree(p);
     delete a;
@@ -938,7 +944,7 @@ extern "C" long pb_nn_num_params(const pb_nn* nn) { return nn ? nn->nparam : -1;
 
 extern "C" int pb_nn_get_params(pb_nn* nn, float* host, float* grads_host) {
   if (!nn || (!host && !grads_host)) return PB_ERR_ARG;
-  hipStream_t s = nn->ctx->own_stream;
+  hipStream_t s = nn->ctx->stream;
   if (host) PB_HIP(nn->ctx, hipMemcpyAsync(host, nn->params, sizeof(float) * nn->nparam, hipMemcpyDeviceToHost, s));
   if (grads_host)
     PB_HIP(nn->ctx, hipMemcpyAsync(grads_host, nn->grads, sizeof(float) * nn->nparam, hipMemcpyDeviceToHost, s));
@@ -948,7 +954,7 @@ extern "C" int pb_nn_get_params(pb_nn* nn, float* host, float* grads_host) {
 
 extern "C" int pb_nn_set_params(pb_nn* nn, const float* host) {
   if (!nn || !host) return PB_ERR_ARG;
-  hipStream_t s = nn->ctx->own_stream;
+  hipStream_t s = nn->ctx->stream;
   PB_HIP(nn->ctx, hipMemcpyAsync(nn->params, host, sizeof(float) * nn->nparam, hipMemcpyHostToDevice, s));
   int rc = pack_weights(nn, s);
   if (rc) return rc;
@@ -960,14 +966,16 @@ extern "C" int pb_nn_set_params(pb_nn* nn, const float* host) {
 // Forward (and optionally backward + Adam step) on a batch of nimg images of
 // h x w nodes.  inputs: float [nimg][h*w][3] (r, feature 1, feature 2);
 // target: float [nimg][h*w] (solution / rms(r), null-space free) or NULL.
+// img_weight: float [nimg] per-image loss weights or NULL (all 1).
 // out_z: float [nimg][h*w] network output (unscaled) or NULL.  lr > 0 applies
-// an Adam step; lr == 0 only computes gradients.  loss: mean squared error.
+// an Adam step; lr == 0 only computes gradients.  loss: weighted mean squared
+// error sum_b w_b sum_p d^2 / (nimg h w).
 extern "C" int pb_nn_train_step(pb_nn* nn, int nimg, int h, int w, const float* inputs, const float* target,
-                                double lr, float* out_z, double* loss) {
+                                const float* img_weight, double lr, float* out_z, double* loss) {
   if (!nn || !inputs || nimg < 1 || nimg > 256 || h < 9 || w < 9)
     return pb_fail(nn ? nn->ctx : nullptr, PB_ERR_ARG, "pb_nn_train_step: bad argument");
   if ((h & 1) == 0 || (w & 1) == 0) return pb_fail(nn->ctx, PB_ERR_ARG, "pb_nn_train_step: h and w must be odd");
-  hipStream_t s = nn->ctx->own_stream;
+  hipStream_t s = nn->ctx->stream;
   const bool train = target != nullptr;
   int rc = 0;
   Acts* a = get_shape(nn, nimg, h, w, train, &rc);
@@ -981,8 +989,8 @@ extern "C" int pb_nn_train_step(pb_nn* nn, int nimg, int h, int w, const float* 
     hipLaunchKernelGGL(k_img_sums<float>, dim3(64, nimg), dim3(256), 0, s, (const float*)a->zout, np, w, 1, a->part, 64);
     hipLaunchKernelGGL(k_null_coef, dim3(1), dim3(nimg), 0, s, (const double*)a->part, 64, np, w, h, a->coef);
     const int nbl = 1024;
-    hipLaunchKernelGGL(k_loss, dim3(nbl), dim3(256), 0, s, (const float*)a->zout, target, (const double*)a->coef, np,
-                       w, nimg, (float)(1.0 / (double)n), a->dOUT, a->lpart);
+    hipLaunchKernelGGL(k_loss, dim3(nbl), dim3(256), 0, s, (const float*)a->zout, target, (const double*)a->coef,
+                       img_weight, np, w, nimg, (float)(1.0 / (double)n), a->dOUT, a->lpart);
     rc = backward(nn, a, s);
     if (rc) return rc;
     if (lr > 0) {
@@ -1014,7 +1022,7 @@ extern "C" int pb_nn_apply(pb_nn* nn, const pb_op2d* op, const double* r, double
   if (!nn || !op || !r || !z) return pb_fail(nn ? nn->ctx : nullptr, PB_ERR_ARG, "pb_nn_apply: null argument");
   if ((op->g.nx & 1) == 0 || (op->g.ny & 1) == 0)
     return pb_fail(nn->ctx, PB_ERR_ARG, "pb_nn_apply: grid dimensions must be odd (2^k + 1 nodes)");
-  hipStream_t s = nn->ctx->own_stream;
+  hipStream_t s = nn->ctx->stream;
   int rc = op_features(nn, op, s);
   if (rc) return rc;
   Acts* a = get_shape(nn, 1, op->g.ny, op->g.nx, false, &rc);
@@ -1034,7 +1042,7 @@ extern "C" int pb_nn_apply(pb_nn* nn, const pb_op2d* op, const double* r, double
 // Coefficient features (two float arrays of op->n values) of an operator.
 extern "C" int pb_nn_features(pb_nn* nn, const pb_op2d* op, float* f1, float* f2) {
   if (!nn || !op || !f1 || !f2) return pb_fail(nn ? nn->ctx : nullptr, PB_ERR_ARG, "pb_nn_features: null argument");
-  hipStream_t s = nn->ctx->own_stream;
+  hipStream_t s = nn->ctx->stream;
   int rc = op_features(nn, op, s);
   if (rc) return rc;
   PB_HIP(nn->ctx, hipMemcpyAsync(f1, nn->f1, sizeof(float) * op->n, hipMemcpyDeviceToDevice, s));

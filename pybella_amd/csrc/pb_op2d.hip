@@ -198,12 +198,15 @@ extern "C" int pb_op2d_create(pb_ctx* ctx, const pb_grid2d* grid, pb_op2d** out)
   return PB_OK;
 }
 
+void pb_fgmres_release(const pb_op2d* op);   // pb_fgmres.hip
+
 extern "C" int pb_op2d_destroy(pb_op2d* op) {
   if (!op) return PB_OK;
   if (op->ctx) {
     (void)hipStreamSynchronize(op->ctx->stream);
     if (op->ctx->own_stream) (void)hipStreamSynchronize(op->ctx->own_stream);
   }
+  pb_fgmres_release(op);
   for (auto& kv : op->w64.graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto& kv : op->w32.graphs) (void)hipGraphExecDestroy(kv.second);
   for (void* p : op->allocs) (void)hipFree(p);
@@ -264,6 +267,7 @@ extern "C" int pb_op2d_assemble(pb_op2d* op, const pb_phys* phys, double dt, con
                      op->coeff, op->rYpow, wcenter, diag_inv, op->hc, op->dinv, op->hc32, op->dinv32);
   PB_LAUNCH_CHECK(ctx);
   op->assembled = true;
+  op->version++;
   return PB_OK;
 }
 

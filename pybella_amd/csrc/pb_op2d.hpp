@@ -331,6 +331,7 @@ struct pb_op2d {
   long n = 0;      // nodes
   long nc = 0;     // op-cells
   bool cross = false, center = true, assembled = false;
+  long version = 0;   // bumped by every assembly (keys caches derived from the weights)
   int ry = 4;      // tile rows per thread (4 or 8)
   double* wxx = nullptr;
   double* wyy = nullptr;

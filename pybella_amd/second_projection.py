@@ -123,7 +123,12 @@ class Operator2D:
         return b
 
     def solve(self, b, x=None, tol=1e-8, maxiter=6000, mode="f64", inner_tol=1e-3, max_outer=12,
-              check_every=32, x0=None, fallback_maxiter=6000):
+              check_every=32, x0=None, fallback_maxiter=6000, krylov_m=40, sweeps=1, omega=0.6,
+              use_network=True):
+        """mode: "f64" (reference semantics), "f64fast", "mixed" or "nn"
+        (flexible GMRES with the attached network; ``maxiter`` is then its
+        iteration budget before plain BiCGStab takes over for up to
+        ``fallback_maxiter`` iterations)."""
         torch = _torch()
         if x is None:
             x = torch.empty(self.ny, self.nx, dtype=torch.float64, device="cuda")
@@ -132,7 +137,8 @@ class Operator2D:
             x.copy_(x0)
             given = 1
         opts = _abi.SolveOpts(_MODES[mode], float(tol), int(maxiter), float(inner_tol), int(max_outer),
-                              int(check_every), given, int(fallback_maxiter))
+                              int(check_every), given, int(fallback_maxiter), int(krylov_m), int(sweeps),
+                              float(omega), 1 if use_network else 0)
         info = _abi.SolveInfo()
         self.ctx.sync_stream()
         rc = self.ctx.lib.pb_bicgstab_solve(self.h, _abi.ptr(b), _abi.ptr(x), C.byref(opts), C.byref(info))

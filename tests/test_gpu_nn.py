@@ -84,3 +84,97 @@ def test_training_reduces_loss():
         net.step(Xd, Td, lr=2e-3)
     l1 = net.step(Xd, Td, lr=0.0)
     assert l1 < 0.7 * l0, (l0, l1)
+
+
+def test_image_weights_scale_gradients():
+    """Per-image loss weights: loss and gradients of a weighted 2-image batch
+    equal the weighted combination of the single-image ones (each single-image
+    loss is normalised by its own pixel count, the batch by twice that)."""
+    import torch
+    print("case nn image weights", flush=True)
+    net = _net(11)
+    X, T = _inputs(2, 33, 17, seed=21)
+    Xd, Td = torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()
+    la = net.step(Xd[:1].contiguous(), Td[:1].contiguous(), lr=0.0)
+    ga = net.get_grads().astype(np.float64)
+    lb = net.step(Xd[1:].contiguous(), Td[1:].contiguous(), lr=0.0)
+    gb = net.get_grads().astype(np.float64)
+    w = torch.tensor([0.25, 3.0], dtype=torch.float32, device="cuda")
+    lw = net.step(Xd, Td, lr=0.0, img_weight=w)
+    gw = net.get_grads().astype(np.float64)
+    assert abs(lw - 0.5 * (0.25 * la + 3.0 * lb)) < 1e-4 * lw
+    ref = 0.5 * (0.25 * ga + 3.0 * gb)
+    assert np.linalg.norm(gw - ref) < 0.03 * np.linalg.norm(ref)
+
+
+def _stratified_system(nx, ny, seed=20261202):
+    import torch
+    from pybella_amd import second_projection as sp
+    from pybella_amd import workloads as W
+    Sol, mpv, elem, node, ud, th = W.stratified_state(nx, ny)
+    op = sp.Operator2D(sp.grid_of(node, ud))
+    rho, rhoY = sp.to_dev(Sol.rho), sp.to_dev(Sol.rhoY)
+    op.assemble(sp.phys_of(ud, th), 0.05, rho, rhoY, rho, rhoY, sp.to_dev(sp.dSdy_cells(mpv, elem, node)))
+    b = op.scale_rhs(sp.to_dev(W.stratified_rhs(node, seed=seed)))
+    torch.cuda.synchronize()
+    return op, b
+
+
+def _trained():
+    from pybella_amd.nn import Network, default_weights_path
+    net = Network(0)
+    net.load(default_weights_path())
+    return net
+
+
+@pytest.mark.parametrize("budget,sweeps", [(0, 2), (20, 1), (40, 0)])
+def test_nn_solve_reaches_tolerance(budget, sweeps):
+    """The learned-preconditioned solve (network guess, flexible GMRES budget,
+    plain BiCGStab after it) reaches true relative residual 1e-8."""
+    print(f"case nn solve 257x129 budget {budget} sweeps {sweeps}", flush=True)
+    op, b = _stratified_system(257, 129)
+    net = _trained()
+    op.attach_nn(net)
+    x, info = op.solve(b, tol=1e-8, mode="nn", maxiter=budget, sweeps=sweeps, fallback_maxiter=6000)
+    assert info["info"] == 0 and info["true_rel_resid"] <= 1e-8, info
+    # the same solution as plain BiCGStab, modulo the null space {1, checkerboard}
+    xp, infop = op.solve(b, tol=1e-12, mode="f64", maxiter=6000)
+    d = (x - xp).cpu().numpy()
+    d = onn.project_null(d[None])[0]
+    assert np.linalg.norm(d) <= 1e-5 * np.linalg.norm(onn.project_null(xp.cpu().numpy()[None])[0])
+
+
+def test_network_gain_over_jacobi_only():
+    """A/B: the same flexible GMRES (60 iterations, no fallback), with the
+    trained network + one Jacobi sweep vs one Jacobi sweep alone; the network
+    must lower the true residual reached by at least 5x."""
+    print("case nn A/B 513x257", flush=True)
+    op, b = _stratified_system(513, 257)
+    net = _trained()
+    op.attach_nn(net)
+    _, with_nn = op.solve(b, tol=1e-8, mode="nn", maxiter=60, sweeps=1, fallback_maxiter=1)
+    _, without = op.solve(b, tol=1e-8, mode="nn", maxiter=60, sweeps=1, fallback_maxiter=1, use_network=False)
+    print("with network", with_nn, "without", without, flush=True)
+    assert with_nn["true_rel_resid"] * 5 < without["true_rel_resid"]
+
+
+def test_features_follow_reassembly():
+    """The network's cached operator features are refreshed when the same
+    operator handle is re-assembled on a different state."""
+    import torch
+    from pybella_amd import second_projection as sp
+    from pybella_amd import workloads as W
+    print("case nn features cache", flush=True)
+    net = _net(1)
+    Sol, mpv, elem, node, ud, th = W.stratified_state(65, 33)
+    op = sp.Operator2D(sp.grid_of(node, ud))
+    dS = sp.to_dev(sp.dSdy_cells(mpv, elem, node))
+    op.assemble(sp.phys_of(ud, th), 0.05, sp.to_dev(Sol.rho), sp.to_dev(Sol.rhoY), sp.to_dev(Sol.rho),
+                sp.to_dev(Sol.rhoY), dS)
+    f1a = net.features(op)[0].clone()
+    rhoY2 = Sol.rhoY * (1.0 + 0.2 * np.random.default_rng(0).random(Sol.rhoY.shape))
+    op.assemble(sp.phys_of(ud, th), 0.05, sp.to_dev(Sol.rho), sp.to_dev(rhoY2), sp.to_dev(Sol.rho),
+                sp.to_dev(rhoY2), dS)
+    f1b = net.features(op)[0].clone()
+    torch.cuda.synchronize()
+    assert float((f1a - f1b).abs().max()) > 1e-4
