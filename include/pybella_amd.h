@@ -59,6 +59,7 @@ extern "C" {
 typedef struct pb_ctx pb_ctx;
 typedef struct pb_op2d pb_op2d;
 typedef struct pb_op3d pb_op3d;
+typedef struct pb_op3d pb_op3d;
 typedef struct pb_nn pb_nn;
 
 typedef struct pb_grid2d {
@@ -77,6 +78,14 @@ typedef struct pb_phys {
   double compressibility;  /* ud.compressibility                         */
   int is_arakawa_konor;    /* ud.is_ArakawaKonor                         */
 } pb_phys;
+
+/* 3D node grid: nx * ny * nz inner nodes (y the gravity axis), spacings,
+ * boundary type per axis. */
+typedef struct pb_grid3d {
+  int nx, ny, nz;
+  double dx, dy, dz;
+  int bc_x, bc_y, bc_z;
+} pb_grid3d;
 
 typedef struct pb_solve_opts {
   int mode;            /* PB_SOLVE_*                                          */
@@ -212,6 +221,32 @@ int pb_nn_features(pb_nn* nn, const pb_op2d* op, float* f1, float* f2);
 /* attach a network to an operator: PB_SOLVE_NN then uses it as the initial
  * guess and the preconditioner of flexible GMRES (NULL detaches) */
 int pb_op2d_attach_nn(pb_op2d* op, pb_nn* nn);
+
+/* ---- 3D 27-point operator (stencil_27pt / lap3D, laplacian.py:872-1026) ----
+ * Layout: nodes (nz, ny, nx) x fastest; operator cells (nz+1, ny+1, nx+1)
+ * (cell (k, j, i) between nodes k-1..k, j-1..j, i-1..i; the outer layers are
+ * ghost cells).  Periodic axes wrap onto the duplicated end node; wall axes
+ * zero the outside cell layers.  The x-z Coriolis cross term is not built
+ * (coriolis_strength[0] = 0 in every 3D configuration). */
+int pb_op3d_create(pb_ctx* ctx, const pb_grid3d* grid, pb_op3d** out);
+int pb_op3d_destroy(pb_op3d* op);
+/* Coefficients from the cell state (operator_coefficients_nodes,
+ * second_projection.py:308-356, every wplus[d] = Gammainv rhoY Y; the regime
+ * scaling :139-158) and the 3D diagonal (precon_diag_prepare,
+ * laplacian.py:1284-1356).  wcenter / diag_inv: optional node outputs. */
+int pb_op3d_assemble(pb_op3d* op, const pb_phys* phys, double dt, const double* rho_cells,
+                     const double* rhoY_cells, double* wcenter, double* diag_inv);
+/* y = diag_inv * (A x); precision PB_F64 or PB_F32 */
+int pb_op3d_matvec(pb_op3d* op, const void* x, void* y, int precision);
+/* Solve (D^-1 A) x = b with x0 = 0 (the reference's bicgstab call,
+ * second_projection.py:230, on the 27-point operator).  Modes PB_SOLVE_F64
+ * (legacy tolerance) and PB_SOLVE_MIXED (FP32 inner solves in FP64
+ * refinement).  Synchronous; returns the info code as pb_bicgstab_solve. */
+int pb_op3d_solve(pb_op3d* op, const double* b, double* x, const pb_solve_opts* opts, pb_solve_info* info);
+/* Average device time (microseconds) of a 3D kernel: 0 matvec, 1 iteration
+ * launch 1 (x, r, p updates + v = A p + rt.v), 2 launch 2 (t = A s + 5 dots).
+ * Measurement aid; no reference counterpart. */
+int pb_op3d_time_kernel(pb_op3d* op, int kernel, int precision, int reps, double* avg_us);
 
 #ifdef __cplusplus
 }

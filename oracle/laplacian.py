@@ -288,3 +288,37 @@ def lap3d(p, op):
     lap = lap + op["hc"] * P
     lap = lap * op["dinv"]
     return lap.ravel()
+
+
+def assemble3d(st, dt):
+    """3D operator arrays for ``lap3d`` from a workloads.State3D, restating
+    operator_coefficients_nodes (second_projection.py:308-356: coeff =
+    Gammainv rhoY Y on cells for every wplus[d], wcenter = ccenter * 2x2x2
+    mean of rhoY^(2-gamma) at nodes, wall node planes halved with the other
+    axes restricted to their inner nodes, :433-464), the regime scaling
+    (:139-158) and precon_diag_prepare in 3D (laplacian.py:1284-1356: coeff
+    1/16, diag = wcenter - sum of the box sums of wplus times 1/dx^2, 1/dy^2,
+    1/dz^2 and twice 1/(dx dy), on the wall-unzeroed weights)."""
+    th, ud = st.th, st.ud
+    Y = st.rhoY_c / st.rho_c
+    coeff = th.Gammainv * st.rhoY_c * Y
+    ccenter = -ud.Msq * th.gm1inv / dt ** 2
+    r = st.rhoY_c ** (2.0 - th.gamm)
+    box = (r[:-1, :-1, :-1] + r[:-1, :-1, 1:] + r[:-1, 1:, :-1] + r[:-1, 1:, 1:]
+           + r[1:, :-1, :-1] + r[1:, :-1, 1:] + r[1:, 1:, :-1] + r[1:, 1:, 1:])
+    wc = ccenter * box / 8.0
+    per = st.periodic
+    for axis, periodic in ((2, per[0]), (1, per[1]), (0, per[2])):
+        if not periodic:
+            for end in (0, -1):
+                sl = [slice(1, -1)] * 3
+                sl[axis] = end
+                wc[tuple(sl)] *= 0.5
+    if ud.is_compressible == 0:
+        wc = wc * ud.compressibility
+    hb = (coeff[:-1, :-1, :-1] + coeff[:-1, :-1, 1:] + coeff[:-1, 1:, :-1] + coeff[:-1, 1:, 1:]
+          + coeff[1:, :-1, :-1] + coeff[1:, :-1, 1:] + coeff[1:, 1:, :-1] + coeff[1:, 1:, 1:])
+    c = 0.0625
+    diag = wc - c * hb * (1.0 / st.dx ** 2 + 1.0 / st.dy ** 2 + 1.0 / st.dz ** 2 + 2.0 / (st.dx * st.dy))
+    return dict(nx=st.nx, ny=st.ny, nz=st.nz, dx=st.dx, dy=st.dy, dz=st.dz, periodic=per,
+                hx=coeff, hy=coeff, hz=coeff, hc=wc, dinv=1.0 / diag)

@@ -26,6 +26,11 @@ class Grid2D(C.Structure):
                 ("bc_x", C.c_int), ("bc_y", C.c_int)]
 
 
+class Grid3D(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+                ("dz", C.c_double), ("bc_x", C.c_int), ("bc_y", C.c_int), ("bc_z", C.c_int)]
+
+
 class Phys(C.Structure):
     _fields_ = [("gamm", C.c_double), ("Msq", C.c_double), ("g", C.c_double),
                 ("coriolis", C.c_double * 3), ("nonhydro", C.c_double),
@@ -56,6 +61,12 @@ D = C.c_double
 # name -> (restype, argtypes); the list every test checks against the header
 SIGNATURES = {
     "pb_version": (I, []),
+    "pb_op3d_create": (I, [VP, C.POINTER(Grid3D), C.POINTER(VP)]),
+    "pb_op3d_destroy": (I, [VP]),
+    "pb_op3d_assemble": (I, [VP, C.POINTER(Phys), D, VP, VP, VP, VP]),
+    "pb_op3d_matvec": (I, [VP, VP, VP, I]),
+    "pb_op3d_solve": (I, [VP, VP, VP, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+    "pb_op3d_time_kernel": (I, [VP, I, I, I, C.POINTER(D)]),
     "pb_ctx_create": (I, [I, C.POINTER(VP)]),
     "pb_ctx_destroy": (I, [VP]),
     "pb_ctx_set_stream": (I, [VP, VP]),

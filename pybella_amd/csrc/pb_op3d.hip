@@ -1,0 +1,820 @@
+// 3D nodal 27-point operator and its BiCGStab solve (gfx950).
+//
+// Operator (lap3D, laplacian.py:872-1026, restated on a clean layout): nodes
+// (nz, ny, nx) with x fastest; cell weights h on the (nz+1, ny+1, nx+1)
+// operator cells (cell (k, j, i) has corner nodes k-1..k, j-1..j, i-1..i);
+// per cell the differences along each axis summed over the cube's 4 edges,
+// F_x = h g_x / (16 dx^2) (and y, z); node value = sum over the 8 cells of
+// (+F on the cell's high side of the node, -F on its low side) + hc p, times
+// diag_inv.  Periodic axes wrap onto the duplicated end node (i-1 -> n-2 at
+// i = 0, i+1 -> 1 at i = n-1, as lap3D's ghost copy, :927-958); wall axes
+// zero the weights of the outside cell layers (:933-965).  The x-z Coriolis
+// cross term of lap3D (:1017-1022) is zero for every state here
+// (coriolis_strength[0] = 0) and is not built.
+//
+// Kernel shape: a 2.5D march.  A 256-thread block owns a 64 x 4 node tile
+// and a chunk of 8 z planes; two node planes (with a one-node halo) and one
+// plane of cell fluxes live in LDS.  Each cell plane is computed once and
+// used by the node planes below and above it: for node plane k,
+//   y = D(k-1/2) + D(k+1/2) + Z(k+1/2) - Z(k-1/2)
+// with D the in-plane flux divergence and Z the z-flux sum of the 4 cells
+// of a cell plane around the node.
+//
+// BiCGStab: the two-launch iteration of the 2D solver (x and r updates of
+// the previous iteration, p and v = A p in launch 1; s, t = A s and the five
+// dots in launch 2), with one-block scalar kernels between the launches that
+// reduce the block partials in fixed order (deterministic) and advance the
+// solver state.  Mixed precision: FP32 inner solves inside FP64 iterative
+// refinement with the FP64 true residual.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pb_common.hpp"
+
+namespace {
+
+constexpr int X3 = 64, Y3 = 4, Z3 = 8;
+constexpr int kNV = 5;   // max partials per block
+
+enum : int { S3_RUN = 0, S3_CONV = 1, S3_MAXIT = 3, S3_BRK_RHO = -10, S3_BRK_OMEGA = -11,
+             S3_P_CONV = 10, S3_P_HALF = 11, S3_P_MAXIT = 12, S3_P_BRK_RHO = 13, S3_P_BRK_OMEGA = 14 };
+
+struct KS3 {
+  double rho, alpha, omega, beta, atol, bnrm, rnorm, tol;
+  int iter, maxiter, status, legacy;
+};
+
+template <typename T>
+struct Op3 {
+  const T* h;
+  const T* hc;
+  const T* dinv;
+  int nx, ny, nz, px, py, pz, center;
+  T sx, sy, sz;
+};
+
+__device__ __forceinline__ int wrap3(int i, int n, int per) {
+  if (i < 0) return per ? n - 2 : 0;
+  if (i >= n) return (i == n && per) ? 1 : n - 1;
+  return i;
+}
+
+__device__ __forceinline__ int tid3() { return threadIdx.x + threadIdx.y * X3; }
+__device__ __forceinline__ int bid3() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+__device__ __forceinline__ int nblk3() { return gridDim.x * gridDim.y * gridDim.z; }
+__device__ __forceinline__ double eps2() { return 4.930380657631324e-32; }
+
+// The march: ``in(g)`` gives the operand at node g (may combine vectors),
+// ``out(g, y, pc, acc)`` consumes y = (A operand)(g) and the operand's own
+// value pc for the owned nodes; acc collects per-thread partial sums.
+template <typename T, int NV, class In, class Out>
+__device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (&acc)[NV > 0 ? NV : 1]) {
+  __shared__ T P[2][Y3 + 2][X3 + 2];
+  __shared__ T F[3][Y3 + 1][X3 + 1];
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tid3();
+  const int x0 = blockIdx.x * X3, y0 = blockIdx.y * Y3;
+  const int kb = blockIdx.z * Z3, ke = min(kb + Z3, o.nz);
+  const int nx = o.nx, ny = o.ny;
+  const int i = x0 + tx, j = y0 + ty;
+  const bool own = i < nx && j < ny;
+  auto load_plane = [&](int k, int slot) {
+    const long kk = wrap3(k, o.nz, o.pz);
+    for (int q = tid; q < (Y3 + 2) * (X3 + 2); q += X3 * Y3) {
+      const int jj = q / (X3 + 2), ii = q - jj * (X3 + 2);
+      const int gj = wrap3(y0 - 1 + jj, ny, o.py), gi = wrap3(x0 - 1 + ii, nx, o.px);
+      P[slot][jj][ii] = in((kk * ny + gj) * nx + gi);
+    }
+  };
+  auto cells = [&](int k, int s0, int s1) {   // cell plane between node planes k and k+1
+    const long ck = k + 1;
+    for (int q = tid; q < (Y3 + 1) * (X3 + 1); q += X3 * Y3) {
+      const int cj = q / (X3 + 1), ci = q - cj * (X3 + 1);
+      const int cy = y0 + cj, cx = x0 + ci;
+      T h = 0;
+      if (cy <= ny && cx <= nx) h = o.h[(ck * (ny + 1) + cy) * (nx + 1) + cx];
+      const T a00 = P[s0][cj][ci], a01 = P[s0][cj][ci + 1], a10 = P[s0][cj + 1][ci], a11 = P[s0][cj + 1][ci + 1];
+      const T b00 = P[s1][cj][ci], b01 = P[s1][cj][ci + 1], b10 = P[s1][cj + 1][ci], b11 = P[s1][cj + 1][ci + 1];
+      const T gx = ((a01 - a00) + (a11 - a10)) + ((b01 - b00) + (b11 - b10));
+      const T gy = ((a10 - a00) + (a11 - a01)) + ((b10 - b00) + (b11 - b01));
+      const T gz = ((b00 - a00) + (b01 - a01)) + ((b10 - a10) + (b11 - a11));
+      F[0][cj][ci] = h * gx * o.sx;
+      F[1][cj][ci] = h * gy * o.sy;
+      F[2][cj][ci] = h * gz * o.sz;
+    }
+  };
+  auto dz = [&](T& D, T& Z) {
+    D = ((F[0][ty][tx + 1] + F[0][ty + 1][tx + 1]) - (F[0][ty][tx] + F[0][ty + 1][tx])) +
+        ((F[1][ty + 1][tx] + F[1][ty + 1][tx + 1]) - (F[1][ty][tx] + F[1][ty][tx + 1]));
+    Z = (F[2][ty][tx] + F[2][ty][tx + 1]) + (F[2][ty + 1][tx] + F[2][ty + 1][tx + 1]);
+  };
+  load_plane(kb - 1, 0);
+  load_plane(kb, 1);
+  __syncthreads();
+  cells(kb - 1, 0, 1);
+  __syncthreads();
+  T Dp, Zp;
+  dz(Dp, Zp);
+  for (int k = kb; k < ke; ++k) {
+    const int sk = (k - kb + 1) & 1, sn = sk ^ 1;
+    __syncthreads();          // F and the slot of plane k-1 are free
+    load_plane(k + 1, sn);
+    __syncthreads();
+    cells(k, sk, sn);
+    __syncthreads();
+    T Dn, Zn;
+    dz(Dn, Zn);
+    if (own) {
+      const long g = ((long)k * ny + j) * nx + i;
+      const T pc = P[sk][ty + 1][tx + 1];
+      T y = (Dp + Dn) + (Zn - Zp);
+      if (o.center) y += o.hc[g] * pc;
+      y *= o.dinv[g];
+      out(g, y, pc, acc);
+    }
+    Dp = Dn;
+    Zp = Zn;
+  }
+}
+
+template <int NV>
+__device__ __forceinline__ void write_partials3(double (&acc)[NV], double* part) {
+  __shared__ double sh[NV * 4];
+  block_sum<NV>(acc, sh);
+  if (tid3() == 0)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) part[(long)bid3() * NV + v] = acc[v];
+}
+
+// one-block reduction of the partial records [b][NV] in block order
+template <int NV>
+__device__ __forceinline__ void reduce_records(const double* part, int nb, double (&out)[NV]) {
+  __shared__ double sh[NV][256];
+  double s[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s[v] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s[v] += part[(long)b * NV + v];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) sh[v][threadIdx.x] = s[v];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sh[v][threadIdx.x] += sh[v][threadIdx.x + off];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) out[v] = sh[v][0];
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k3_matvec(Op3<T> o, const T* p, T* y) {
+  double acc[1] = {0.0};
+  march3<T, 0>(o, [&](long g) { return p[g]; }, [&](long g, T v, T, double(&)[1]) { y[g] = v; }, acc);
+}
+
+// r = b - A x (FP64), r32 = (float) r, partials of r.r
+__global__ __launch_bounds__(256) void k3_resid(Op3<double> o, const double* x, const double* b, double* r,
+                                                float* r32, double* part) {
+  double acc[1] = {0.0};
+  march3<double, 1>(o, [&](long g) { return x[g]; },
+                    [&](long g, double y, double, double(&a)[1]) {
+                      const double rr = b[g] - y;
+                      r[g] = rr;
+                      if (r32) r32[g] = (float)rr;
+                      a[0] += rr * rr;
+                    },
+                    acc);
+  write_partials3<1>(acc, part);
+}
+
+// init: r = b, rt = b, p = v = t = 0, x = 0; partials of b.b
+template <typename T>
+__global__ void k3_init(long n, const T* b, T* r, T* rt, T* p, T* v, T* t, T* x, double* part) {
+  double acc[1] = {0.0};
+  for (long g = (long)bid3() * 256 + tid3(); g < n; g += (long)nblk3() * 256) {
+    const T bb = b[g];
+    r[g] = bb;
+    rt[g] = bb;
+    p[g] = 0;
+    v[g] = 0;
+    t[g] = 0;
+    x[g] = 0;
+    acc[0] += (double)bb * (double)bb;
+  }
+  write_partials3<1>(acc, part);
+}
+
+__global__ void k3_s0(KS3* st, const double* part, int nb, double tol, int maxiter, int legacy) {
+  double s[1];
+  reduce_records<1>(part, nb, s);
+  if (threadIdx.x == 0) {
+    KS3 k{};
+    k.rho = s[0];
+    k.bnrm = sqrt(s[0]);
+    k.tol = tol;
+    k.atol = tol * k.bnrm;
+    k.maxiter = maxiter;
+    k.legacy = legacy;
+    k.rnorm = k.bnrm;
+    k.alpha = k.omega = k.beta = 0.0;
+    k.iter = 0;
+    // legacy: return x0 = 0 at once if ||b - A x0|| <= tol (absolute); else b = 0
+    k.status = ((legacy && k.bnrm <= tol) || k.bnrm == 0.0) ? S3_CONV : S3_RUN;
+    *st = k;
+  }
+}
+
+// launch 1 of iteration k: r_{k-1} = r - alpha v - omega t, x += alpha p + omega (r - alpha v),
+// p_k = r_{k-1} + beta (p - omega v), v_k = A p_k; partials of rt.v_k.  Reads set a, writes set b.
+template <typename T>
+__global__ __launch_bounds__(256) void k3_iter1(Op3<T> o, const KS3* st, const T* ra, const T* pa, const T* va,
+                                                const T* t, const T* rt, T* rb, T* pb, T* vb, T* x, double* part) {
+  if (st->status != S3_RUN) return;
+  const T al = (T)st->alpha, om = (T)st->omega, be = (T)st->beta;
+  double acc[1] = {0.0};
+  march3<T, 1>(
+      o,
+      [&](long g) {
+        const T v = va[g];
+        const T rn = (ra[g] - al * v) - om * t[g];
+        return rn + be * (pa[g] - om * v);
+      },
+      [&](long g, T y, T pn, double(&a)[1]) {
+        const T v = va[g];
+        const T s = ra[g] - al * v;
+        x[g] += al * pa[g] + om * s;
+        rb[g] = s - om * t[g];
+        pb[g] = pn;
+        vb[g] = y;
+        a[0] += (double)rt[g] * (double)y;
+      },
+      acc);
+  write_partials3<1>(acc, part);
+}
+
+__global__ void k3_s1(KS3* st, const double* part, int nb) {
+  if (st->status != S3_RUN) return;
+  double s[1];
+  reduce_records<1>(part, nb, s);
+  if (threadIdx.x == 0) {
+    KS3 k = *st;
+    if (s[0] == 0.0 || !isfinite(s[0])) {
+      k.status = S3_BRK_OMEGA;
+    } else {
+      k.alpha = k.rho / s[0];
+    }
+    *st = k;
+  }
+}
+
+// launch 2: s = r - alpha v (on the fly), t = A s; partials s.s, t.s, t.t, rt.s, rt.t
+template <typename T>
+__global__ __launch_bounds__(256) void k3_iter2(Op3<T> o, const KS3* st, const T* r, const T* v, const T* rt, T* t,
+                                                double* part) {
+  if (st->status != S3_RUN) return;
+  const T al = (T)st->alpha;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  march3<T, 5>(
+      o, [&](long g) { return r[g] - al * v[g]; },
+      [&](long g, T y, T s, double(&a)[5]) {
+        t[g] = y;
+        const double sd = s, td = y, rd = rt[g];
+        a[0] += sd * sd;
+        a[1] += td * sd;
+        a[2] += td * td;
+        a[3] += rd * sd;
+        a[4] += rd * td;
+      },
+      acc);
+  write_partials3<5>(acc, part);
+}
+
+__global__ void k3_s2(KS3* st, const double* part, int nb) {
+  if (st->status != S3_RUN) return;
+  double s[5];
+  reduce_records<5>(part, nb, s);
+  if (threadIdx.x == 0) {
+    KS3 k = *st;
+    const double ss = s[0], ts = s[1], tt = s[2], rts = s[3], rtt = s[4];
+    if (sqrt(ss) <= k.atol) {
+      k.status = S3_P_HALF;     // x += alpha p
+      k.rnorm = sqrt(ss);
+    } else if (tt == 0.0 || !isfinite(tt)) {
+      k.status = S3_BRK_OMEGA;
+    } else {
+      const double om = ts / tt;
+      const double rho_new = rts - om * rtt;
+      const double rr = fmax(ss - 2.0 * om * ts + om * om * tt, 0.0);
+      k.omega = om;
+      k.iter += 1;
+      k.rnorm = sqrt(rr);
+      if (k.rnorm <= k.atol) k.status = S3_P_CONV;
+      else if (fabs(om) < eps2()) k.status = S3_P_BRK_OMEGA;
+      else if (k.iter >= k.maxiter) k.status = S3_P_MAXIT;
+      else if (fabs(rho_new) < eps2()) k.status = S3_P_BRK_RHO;
+      else {
+        k.beta = (rho_new / k.rho) * (k.alpha / om);
+        k.rho = rho_new;
+      }
+    }
+    *st = k;
+  }
+}
+
+// pending exit: x += alpha p + omega (r - alpha v)  (omega term dropped for a half step)
+template <typename T>
+__global__ void k3_finish(long n, const KS3* st, const T* r, const T* p, const T* v, T* x) {
+  const T al = (T)st->alpha, om = st->status == S3_P_HALF ? (T)0 : (T)st->omega;
+  for (long g = (long)bid3() * 256 + tid3(); g < n; g += (long)nblk3() * 256) x[g] += al * p[g] + om * (r[g] - al * v[g]);
+}
+
+template <typename T>
+__global__ void k3_cast(long n, const T* a, double* b, int add) {
+  for (long g = (long)bid3() * 256 + tid3(); g < n; g += (long)nblk3() * 256) b[g] = add ? b[g] + (double)a[g] : (double)a[g];
+}
+
+// assembly -------------------------------------------------------------------
+// cell weights h = Gammainv rhoY (rhoY/rho), zeroed on the outside layers of wall axes
+__global__ void k3_asm_cells(int nx, int ny, int nz, int px, int py, int pz, double Gammainv, const double* rho,
+                             const double* rhoY, double* h64, float* h32) {
+  const long nc = (long)(nx + 1) * (ny + 1) * (nz + 1);
+  for (long c = (long)bid3() * 256 + tid3(); c < nc; c += (long)nblk3() * 256) {
+    const int cx = (int)(c % (nx + 1));
+    const long t = c / (nx + 1);
+    const int cy = (int)(t % (ny + 1)), cz = (int)(t / (ny + 1));
+    const double Y = rhoY[c] / rho[c];
+    double h = Gammainv * rhoY[c] * Y;
+    if ((!px && (cx == 0 || cx == nx)) || (!py && (cy == 0 || cy == ny)) || (!pz && (cz == 0 || cz == nz))) h = 0.0;
+    h64[c] = h;
+    h32[c] = (float)h;
+  }
+}
+
+// node terms: wcenter (2x2x2 mean of rhoY^(2-gamma), wall planes halved on
+// inner nodes of the other axes, regime scale), diag_inv from the unzeroed
+// box sum of the cell weights (laplacian.py:1284-1356, 3D coefficient 1/16)
+__global__ void k3_asm_nodes(int nx, int ny, int nz, int px, int py, int pz, double Gammainv, double gamm,
+                             double ccenter, double regime, double dx, double dy, double dz, const double* rho,
+                             const double* rhoY, double* hc64, float* hc32, double* dinv64, float* dinv32,
+                             double* wc_out, double* di_out) {
+  const long n = (long)nx * ny * nz;
+  for (long g = (long)bid3() * 256 + tid3(); g < n; g += (long)nblk3() * 256) {
+    const int i = (int)(g % nx);
+    const long t = g / nx;
+    const int j = (int)(t % ny), k = (int)(t / ny);
+    double br = 0.0, bh = 0.0;
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 2; ++c) {
+          const long cc = ((long)(k + a) * (ny + 1) + (j + b)) * (nx + 1) + (i + c);
+          br += pow(rhoY[cc], 2.0 - gamm);
+          bh += Gammainv * rhoY[cc] * (rhoY[cc] / rho[cc]);
+        }
+    double wc = ccenter * br / 8.0;
+    const bool ix = i > 0 && i < nx - 1, iy = j > 0 && j < ny - 1, iz = k > 0 && k < nz - 1;
+    if (!px && !ix && iy && iz) wc *= 0.5;
+    if (!py && !iy && ix && iz) wc *= 0.5;
+    if (!pz && !iz && ix && iy) wc *= 0.5;
+    wc *= regime;
+    const double diag = wc - 0.0625 * bh * (1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz) + 2.0 / (dx * dy));
+    const double di = 1.0 / diag;
+    hc64[g] = wc;
+    hc32[g] = (float)wc;
+    dinv64[g] = di;
+    dinv32[g] = (float)di;
+    if (wc_out) wc_out[g] = wc;
+    if (di_out) di_out[g] = di;
+  }
+}
+
+__global__ void k3_cast_down(long n, const double* a, float* b) {
+  for (long g = (long)bid3() * 256 + tid3(); g < n; g += (long)nblk3() * 256) b[g] = (float)a[g];
+}
+
+__global__ void k3_fill(long n, double* a, double v) {
+  for (long g = (long)bid3() * 256 + tid3(); g < n; g += (long)nblk3() * 256) a[g] = v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct pb_op3d {
+  pb_ctx* ctx = nullptr;
+  pb_grid3d g{};
+  long n = 0, nc = 0;
+  bool assembled = false, center = true;
+  double *h64 = nullptr, *hc64 = nullptr, *dinv64 = nullptr;
+  float *h32 = nullptr, *hc32 = nullptr, *dinv32 = nullptr;
+  // Krylov sets [precision][set]: r, p, v; shared t, rt, x, b
+  void* r[2][2] = {};
+  void* p[2][2] = {};
+  void* v[2][2] = {};
+  void* t[2] = {};
+  void* rt[2] = {};
+  void* x[2] = {};
+  void* b32 = nullptr;
+  double* r64 = nullptr;
+  double* part = nullptr;
+  KS3* st = nullptr;
+  KS3* hst = nullptr;
+  bool have_work = false;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+template <typename T>
+int dalloc3(pb_op3d* op, T** p, long count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)std::max<long>(count, 1));
+  if (e != hipSuccess)
+    return pb_fail(op->ctx, PB_ERR_HIP, "hipMalloc(%ld bytes) failed: %s", (long)(sizeof(T) * count),
+                   hipGetErrorString(e));
+  op->allocs.push_back(q);
+  *p = (T*)q;
+  return PB_OK;
+}
+
+dim3 march_grid(const pb_op3d* op) {
+  return dim3(ceil_div(op->g.nx, X3), ceil_div(op->g.ny, Y3), ceil_div(op->g.nz, Z3));
+}
+int march_blocks(const pb_op3d* op) {
+  const dim3 gr = march_grid(op);
+  return (int)(gr.x * gr.y * gr.z);
+}
+dim3 elem_grid3(long n) { return dim3((unsigned)std::min<long>((n + 255) / 256, 4096)); }
+const dim3 kBlk3(X3, Y3);
+
+template <typename T>
+Op3<T> op3(const pb_op3d* op);
+template <>
+Op3<double> op3<double>(const pb_op3d* op) {
+  const pb_grid3d& g = op->g;
+  return Op3<double>{op->h64, op->hc64, op->dinv64, g.nx, g.ny, g.nz,
+                     g.bc_x == PB_BC_PERIODIC, g.bc_y == PB_BC_PERIODIC, g.bc_z == PB_BC_PERIODIC, op->center ? 1 : 0,
+                     0.0625 / (g.dx * g.dx), 0.0625 / (g.dy * g.dy), 0.0625 / (g.dz * g.dz)};
+}
+template <>
+Op3<float> op3<float>(const pb_op3d* op) {
+  const pb_grid3d& g = op->g;
+  return Op3<float>{op->h32, op->hc32, op->dinv32, g.nx, g.ny, g.nz,
+                    g.bc_x == PB_BC_PERIODIC, g.bc_y == PB_BC_PERIODIC, g.bc_z == PB_BC_PERIODIC, op->center ? 1 : 0,
+                    (float)(0.0625 / (g.dx * g.dx)), (float)(0.0625 / (g.dy * g.dy)), (float)(0.0625 / (g.dz * g.dz))};
+}
+
+int alloc_work3(pb_op3d* op) {
+  if (op->have_work) return PB_OK;
+  const long n = op->n;
+  int rc = 0;
+  for (int s = 0; s < 2; ++s) {
+    rc |= dalloc3(op, (double**)&op->r[0][s], n);
+    rc |= dalloc3(op, (double**)&op->p[0][s], n);
+    rc |= dalloc3(op, (double**)&op->v[0][s], n);
+    rc |= dalloc3(op, (float**)&op->r[1][s], n);
+    rc |= dalloc3(op, (float**)&op->p[1][s], n);
+    rc |= dalloc3(op, (float**)&op->v[1][s], n);
+  }
+  rc |= dalloc3(op, (double**)&op->t[0], n);
+  rc |= dalloc3(op, (double**)&op->rt[0], n);
+  rc |= dalloc3(op, (double**)&op->x[0], n);
+  rc |= dalloc3(op, (float**)&op->t[1], n);
+  rc |= dalloc3(op, (float**)&op->rt[1], n);
+  rc |= dalloc3(op, (float**)&op->x[1], n);
+  rc |= dalloc3(op, (float**)&op->b32, n);
+  rc |= dalloc3(op, &op->r64, n);
+  rc |= dalloc3(op, &op->part, (long)std::max(march_blocks(op), 4096) * kNV);
+  rc |= dalloc3(op, &op->st, 1);
+  if (rc) return rc;
+  void* hp = nullptr;
+  PB_HIP(op->ctx, hipHostMalloc(&hp, sizeof(KS3), hipHostMallocDefault));
+  op->hst = (KS3*)hp;
+  op->have_work = true;
+  return PB_OK;
+}
+
+template <typename T>
+struct Prec;
+template <>
+struct Prec<double> {
+  static constexpr int k = 0;
+};
+template <>
+struct Prec<float> {
+  static constexpr int k = 1;
+};
+
+// BiCGStab on (D^-1 A) x = b with x0 = 0; b, x are the workspace's own
+// vectors of precision T.  Returns PB_OK and the final solver state.
+template <typename T>
+int krylov3(pb_op3d* op, const T* b, double tol, int maxiter, int legacy, int check_every, hipStream_t s,
+            KS3* out) {
+  const int P = Prec<T>::k;
+  const long n = op->n;
+  const Op3<T> o = op3<T>(op);
+  const dim3 mg = march_grid(op);
+  const int nbm = march_blocks(op);
+  const dim3 eg = elem_grid3(n);
+  T* t = (T*)op->t[P];
+  T* rt = (T*)op->rt[P];
+  T* x = (T*)op->x[P];
+  hipLaunchKernelGGL(k3_init<T>, eg, dim3(256), 0, s, n, b, (T*)op->r[P][0], rt, (T*)op->p[P][0], (T*)op->v[P][0], t,
+                     x, op->part);
+  hipLaunchKernelGGL(k3_s0, dim3(1), dim3(256), 0, s, op->st, (const double*)op->part, (int)eg.x, tol, maxiter,
+                     legacy);
+  PB_LAUNCH_CHECK(op->ctx);
+  const int chunk = std::max(1, check_every > 0 ? check_every : 32);
+  long it = 0;
+  for (;;) {
+    for (int c = 0; c < chunk; ++c) {
+      ++it;
+      const int a = (int)((it - 1) & 1), bset = (int)(it & 1);
+      hipLaunchKernelGGL(k3_iter1<T>, mg, kBlk3, 0, s, o, (const KS3*)op->st, (const T*)op->r[P][a],
+                         (const T*)op->p[P][a], (const T*)op->v[P][a], (const T*)t, (const T*)rt, (T*)op->r[P][bset],
+                         (T*)op->p[P][bset], (T*)op->v[P][bset], x, op->part);
+      hipLaunchKernelGGL(k3_s1, dim3(1), dim3(256), 0, s, op->st, (const double*)op->part, nbm);
+      hipLaunchKernelGGL(k3_iter2<T>, mg, kBlk3, 0, s, o, (const KS3*)op->st, (const T*)op->r[P][bset],
+                         (const T*)op->v[P][bset], (const T*)rt, t, op->part);
+      hipLaunchKernelGGL(k3_s2, dim3(1), dim3(256), 0, s, op->st, (const double*)op->part, nbm);
+    }
+    PB_LAUNCH_CHECK(op->ctx);
+    PB_HIP(op->ctx, hipMemcpyAsync(op->hst, op->st, sizeof(KS3), hipMemcpyDeviceToHost, s));
+    PB_HIP(op->ctx, hipStreamSynchronize(s));
+    KS3 h = *op->hst;
+    if (h.status == S3_RUN) continue;
+    if (h.status >= S3_P_CONV) {   // the exit's x update with the vectors of iteration h.iter (or h.iter+1)
+      // the vectors of the iteration that stopped: launch 1 of iteration k wrote set k & 1; a half-step
+      // exit stops inside iteration iter + 1, the others after iteration iter
+      const long k = h.status == S3_P_HALF ? (long)h.iter + 1 : (long)h.iter;
+      const int set = (int)(k & 1);
+      hipLaunchKernelGGL(k3_finish<T>, eg, dim3(256), 0, s, n, (const KS3*)op->st, (const T*)op->r[P][set],
+                         (const T*)op->p[P][set], (const T*)op->v[P][set], x);
+      PB_LAUNCH_CHECK(op->ctx);
+      switch (h.status) {
+        case S3_P_CONV:
+        case S3_P_HALF: h.status = S3_CONV; break;
+        case S3_P_MAXIT: h.status = S3_MAXIT; break;
+        case S3_P_BRK_RHO: h.status = S3_BRK_RHO; break;
+        default: h.status = S3_BRK_OMEGA; break;
+      }
+    }
+    *out = h;
+    return PB_OK;
+  }
+}
+
+int info3(const KS3& h) {
+  switch (h.status) {
+    case S3_CONV: return 0;
+    case S3_BRK_RHO: return -10;
+    case S3_BRK_OMEGA: return -11;
+    default: return std::max(h.iter, 1);
+  }
+}
+
+// ||b - A x|| in FP64 into op->r64 (and op->b32 as FP32)
+int resid3(pb_op3d* op, const double* x, const double* b, hipStream_t s, double* nrm) {
+  const int nbm = march_blocks(op);
+  hipLaunchKernelGGL(k3_resid, march_grid(op), kBlk3, 0, s, op3<double>(op), x, b, op->r64, (float*)op->b32,
+                     op->part);
+  PB_LAUNCH_CHECK(op->ctx);
+  std::vector<double> h((size_t)nbm);
+  PB_HIP(op->ctx, hipMemcpyAsync(h.data(), op->part, sizeof(double) * nbm, hipMemcpyDeviceToHost, s));
+  PB_HIP(op->ctx, hipStreamSynchronize(s));
+  double t = 0.0;
+  for (double v : h) t += v;
+  *nrm = std::sqrt(t);
+  return PB_OK;
+}
+
+}  // namespace
+
+extern "C" int pb_op3d_create(pb_ctx* ctx, const pb_grid3d* grid, pb_op3d** out) {
+  if (!ctx || !grid || !out) return pb_fail(ctx, PB_ERR_ARG, "pb_op3d_create: null argument");
+  const pb_grid3d& g = *grid;
+  if (g.nx < 3 || g.ny < 3 || g.nz < 3 || !(g.dx > 0) || !(g.dy > 0) || !(g.dz > 0))
+    return pb_fail(ctx, PB_ERR_ARG, "pb_op3d_create: bad grid %d x %d x %d", g.nx, g.ny, g.nz);
+  pb_op3d* op = new pb_op3d();
+  op->ctx = ctx;
+  op->g = g;
+  op->n = (long)g.nx * g.ny * g.nz;
+  op->nc = (long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  int rc = dalloc3(op, &op->h64, op->nc);
+  rc |= dalloc3(op, &op->h32, op->nc);
+  rc |= dalloc3(op, &op->hc64, op->n);
+  rc |= dalloc3(op, &op->hc32, op->n);
+  rc |= dalloc3(op, &op->dinv64, op->n);
+  rc |= dalloc3(op, &op->dinv32, op->n);
+  if (rc) {
+    pb_op3d_destroy(op);
+    return PB_ERR_HIP;
+  }
+  *out = op;
+  return PB_OK;
+}
+
+extern "C" int pb_op3d_destroy(pb_op3d* op) {
+  if (!op) return PB_OK;
+  if (op->ctx) (void)hipStreamSynchronize(op->ctx->stream);
+  for (void* p : op->allocs) (void)hipFree(p);
+  if (op->hst) (void)hipHostFree(op->hst);
+  delete op;
+  return PB_OK;
+}
+
+extern "C" int pb_op3d_assemble(pb_op3d* op, const pb_phys* phys, double dt, const double* rho_c,
+                                const double* rhoY_c, double* wcenter, double* diag_inv) {
+  if (!op || !phys || !rho_c || !rhoY_c || !(dt > 0))
+    return pb_fail(op ? op->ctx : nullptr, PB_ERR_ARG, "pb_op3d_assemble: bad argument");
+  const pb_grid3d& g = op->g;
+  hipStream_t s = op->ctx->stream;
+  const double gm = phys->gamm, Gammainv = gm / (gm - 1.0), gm1inv = 1.0 / (gm - 1.0);
+  const double ccenter = -phys->Msq * gm1inv / (dt * dt);
+  // regime (second_projection.py:139-158): compressible -> 1, pseudo-incompressible -> compressibility
+  const double regime = phys->is_compressible ? 1.0 : (phys->is_arakawa_konor ? 0.0 : phys->compressibility);
+  op->center = regime != 0.0;
+  const int px = g.bc_x == PB_BC_PERIODIC, py = g.bc_y == PB_BC_PERIODIC, pz = g.bc_z == PB_BC_PERIODIC;
+  hipLaunchKernelGGL(k3_asm_cells, elem_grid3(op->nc), dim3(256), 0, s, g.nx, g.ny, g.nz, px, py, pz, Gammainv, rho_c,
+                     rhoY_c, op->h64, op->h32);
+  hipLaunchKernelGGL(k3_asm_nodes, elem_grid3(op->n), dim3(256), 0, s, g.nx, g.ny, g.nz, px, py, pz, Gammainv, gm,
+                     ccenter, regime, g.dx, g.dy, g.dz, rho_c, rhoY_c, op->hc64, op->hc32, op->dinv64, op->dinv32,
+                     wcenter, diag_inv);
+  PB_LAUNCH_CHECK(op->ctx);
+  op->assembled = true;
+  return PB_OK;
+}
+
+extern "C" int pb_op3d_matvec(pb_op3d* op, const void* x, void* y, int precision) {
+  if (!op || !x || !y) return pb_fail(op ? op->ctx : nullptr, PB_ERR_ARG, "pb_op3d_matvec: null argument");
+  if (!op->assembled) return pb_fail(op->ctx, PB_ERR_STATE, "pb_op3d_matvec: operator not assembled");
+  hipStream_t s = op->ctx->stream;
+  if (precision == PB_F64)
+    hipLaunchKernelGGL(k3_matvec<double>, march_grid(op), kBlk3, 0, s, op3<double>(op), (const double*)x, (double*)y);
+  else
+    hipLaunchKernelGGL(k3_matvec<float>, march_grid(op), kBlk3, 0, s, op3<float>(op), (const float*)x, (float*)y);
+  PB_LAUNCH_CHECK(op->ctx);
+  return PB_OK;
+}
+
+// Solve (D^-1 A) x = b.  PB_SOLVE_F64 / PB_SOLVE_F64_FAST: FP64 BiCGStab with
+// the reference's legacy tolerance; PB_SOLVE_MIXED: FP32 BiCGStab on the
+// residual inside FP64 iterative refinement until the FP64 true residual
+// ||b - A x|| <= tol ||b|| (a refinement that does not reduce it switches to
+// an FP64 inner solve).  Synchronous.
+extern "C" int pb_op3d_solve(pb_op3d* op, const double* b, double* x, const pb_solve_opts* opts,
+                             pb_solve_info* info) {
+  if (!op || !b || !x || !opts || !info) return pb_fail(op ? op->ctx : nullptr, PB_ERR_ARG, "pb_op3d_solve: null");
+  if (!op->assembled) return pb_fail(op->ctx, PB_ERR_STATE, "pb_op3d_solve: operator not assembled");
+  if (!(opts->tol >= 0) || opts->maxiter < 1) return pb_fail(op->ctx, PB_ERR_ARG, "pb_op3d_solve: bad tol/maxiter");
+  if (opts->x0_given) return pb_fail(op->ctx, PB_ERR_ARG, "pb_op3d_solve: x0 must be zero");
+  int rc = alloc_work3(op);
+  if (rc) return rc;
+  hipStream_t s = op->ctx->stream;
+  const long n = op->n;
+  const dim3 eg = elem_grid3(n);
+  pb_solve_info out{};
+  if (opts->mode == PB_SOLVE_F64 || opts->mode == PB_SOLVE_F64_FAST) {
+    KS3 h{};
+    rc = krylov3<double>(op, b, opts->tol, opts->maxiter, 1, opts->check_every, s, &h);
+    if (rc) return rc;
+    PB_HIP(op->ctx, hipMemcpyAsync(x, op->x[0], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    out.info = info3(h);
+    out.iters = h.iter;
+    out.outer = 1;
+    out.half_step = 0;
+    out.resid = h.rnorm;
+    out.bnrm2 = h.bnrm;
+  } else if (opts->mode == PB_SOLVE_MIXED) {
+    const double inner = opts->inner_tol > 0 ? opts->inner_tol : 1e-4;
+    const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
+    hipLaunchKernelGGL(k3_fill, eg, dim3(256), 0, s, n, x, 0.0);
+    double bn = 0.0;
+    rc = resid3(op, x, b, s, &bn);   // r = b, b32 = (float) b
+    if (rc) return rc;
+    const double target = opts->tol * bn;
+    double rn = bn;
+    int iters = 0, outer = 0, last = 0;
+    bool fp64 = false;
+    while (rn > target && outer < max_outer && iters < opts->maxiter) {
+      KS3 h{};
+      const double tol_rel = std::max(inner, 0.5 * target / rn);
+      if (!fp64) {
+        rc = krylov3<float>(op, (const float*)op->b32, tol_rel, opts->maxiter - iters, 0, opts->check_every, s, &h);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k3_cast<float>, eg, dim3(256), 0, s, n, (const float*)op->x[1], x, 1);
+      } else {
+        rc = krylov3<double>(op, (const double*)op->r64, tol_rel, opts->maxiter - iters, 0, opts->check_every, s,
+                             &h);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k3_cast<double>, eg, dim3(256), 0, s, n, (const double*)op->x[0], x, 1);
+      }
+      PB_LAUNCH_CHECK(op->ctx);
+      iters += h.iter;
+      last = info3(h);
+      ++outer;
+      const double rn_old = rn;
+      rc = resid3(op, x, b, s, &rn);
+      if (rc) return rc;
+      if (!(rn < 0.5 * rn_old)) fp64 = true;   // FP32 refinement stalled
+    }
+    out.info = rn <= target ? 0 : (last < 0 ? last : std::max(iters, 1));
+    out.iters = iters;
+    out.outer = outer;
+    out.resid = rn;
+    out.bnrm2 = bn;
+  } else {
+    return pb_fail(op->ctx, PB_ERR_ARG, "pb_op3d_solve: mode %d not available in 3D", opts->mode);
+  }
+  double tr = 0.0;
+  rc = resid3(op, x, b, s, &tr);
+  if (rc) return rc;
+  out.true_rel_resid = out.bnrm2 > 0 ? tr / out.bnrm2 : tr;
+  *info = out;
+  return out.info;
+}
+
+// Average device time (microseconds) of one launch of a 3D solver kernel on
+// pattern-filled vectors: 0 matvec, 1 launch 1 of the iteration, 2 launch 2.
+extern "C" int pb_op3d_time_kernel(pb_op3d* op, int kernel, int precision, int reps, double* avg_us) {
+  if (!op || !avg_us || reps < 1 || kernel < 0 || kernel > 2)
+    return pb_fail(op ? op->ctx : nullptr, PB_ERR_ARG, "pb_op3d_time_kernel: bad argument");
+  if (!op->assembled) return pb_fail(op->ctx, PB_ERR_STATE, "pb_op3d_time_kernel: operator not assembled");
+  int rc = alloc_work3(op);
+  if (rc) return rc;
+  hipStream_t s = op->ctx->stream;
+  const long n = op->n;
+  const dim3 eg = elem_grid3(n), mg = march_grid(op);
+  const int P = precision == PB_F64 ? 0 : 1;
+  // vectors r, p, v, t, rt, x of both sets hold a constant pattern; the state
+  // carries small fixed scalars so repeated launches stay finite
+  if (P == 0) {
+    for (void* vec : {op->r[0][0], op->p[0][0], op->v[0][0], op->t[0], op->rt[0], op->x[0], op->r[0][1]})
+      hipLaunchKernelGGL(k3_fill, eg, dim3(256), 0, s, n, (double*)vec, 1e-3);
+  } else {
+    hipLaunchKernelGGL(k3_fill, eg, dim3(256), 0, s, n, op->r64, 1e-3);
+    for (void* vec : {op->r[1][0], op->p[1][0], op->v[1][0], op->t[1], op->rt[1], op->x[1], op->r[1][1]})
+      hipLaunchKernelGGL(k3_cast_down, eg, dim3(256), 0, s, n, (const double*)op->r64, (float*)vec);
+  }
+  KS3 k{};
+  k.rho = 1.0;
+  k.alpha = 0.5;
+  k.omega = 0.25;
+  k.beta = 0.125;
+  k.maxiter = 1 << 30;
+  k.status = S3_RUN;
+  *op->hst = k;
+  PB_HIP(op->ctx, hipMemcpyAsync(op->st, op->hst, sizeof(KS3), hipMemcpyHostToDevice, s));
+  auto launch = [&]() {
+    if (kernel == 0) {
+      if (P == 0)
+        hipLaunchKernelGGL(k3_matvec<double>, mg, kBlk3, 0, s, op3<double>(op), (const double*)op->p[0][0],
+                           (double*)op->v[0][1]);
+      else
+        hipLaunchKernelGGL(k3_matvec<float>, mg, kBlk3, 0, s, op3<float>(op), (const float*)op->p[1][0],
+                           (float*)op->v[1][1]);
+    } else if (kernel == 1) {
+      if (P == 0)
+        hipLaunchKernelGGL(k3_iter1<double>, mg, kBlk3, 0, s, op3<double>(op), (const KS3*)op->st,
+                           (const double*)op->r[0][0], (const double*)op->p[0][0], (const double*)op->v[0][0],
+                           (const double*)op->t[0], (const double*)op->rt[0], (double*)op->r[0][1],
+                           (double*)op->p[0][1], (double*)op->v[0][1], (double*)op->x[0], op->part);
+      else
+        hipLaunchKernelGGL(k3_iter1<float>, mg, kBlk3, 0, s, op3<float>(op), (const KS3*)op->st,
+                           (const float*)op->r[1][0], (const float*)op->p[1][0], (const float*)op->v[1][0],
+                           (const float*)op->t[1], (const float*)op->rt[1], (float*)op->r[1][1],
+                           (float*)op->p[1][1], (float*)op->v[1][1], (float*)op->x[1], op->part);
+    } else {
+      if (P == 0)
+        hipLaunchKernelGGL(k3_iter2<double>, mg, kBlk3, 0, s, op3<double>(op), (const KS3*)op->st,
+                           (const double*)op->r[0][0], (const double*)op->v[0][0], (const double*)op->rt[0],
+                           (double*)op->t[0], op->part);
+      else
+        hipLaunchKernelGGL(k3_iter2<float>, mg, kBlk3, 0, s, op3<float>(op), (const KS3*)op->st,
+                           (const float*)op->r[1][0], (const float*)op->v[1][0], (const float*)op->rt[1],
+                           (float*)op->t[1], op->part);
+    }
+  };
+  launch();
+  PB_LAUNCH_CHECK(op->ctx);
+  hipEvent_t e0, e1;
+  PB_HIP(op->ctx, hipEventCreate(&e0));
+  PB_HIP(op->ctx, hipEventCreate(&e1));
+  PB_HIP(op->ctx, hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) launch();
+  PB_HIP(op->ctx, hipEventRecord(e1, s));
+  PB_HIP(op->ctx, hipEventSynchronize(e1));
+  float ms = 0.f;
+  PB_HIP(op->ctx, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  PB_LAUNCH_CHECK(op->ctx);
+  *avg_us = 1000.0 * ms / reps;
+  return PB_OK;
+}

@@ -168,6 +168,72 @@ def batch_member_bubble(nx, ny, member, base_seed=20261204, dt=0.05):
     return rising_bubble_state(nx, ny, delth=delth, xc=xc, r0=r0, dt=dt)
 
 
+class State3D:
+    """Inputs of the isolated 3D nodal solve: operator-cell fields rho, rhoY
+    of shape (nz+1, ny+1, nx+1) ([z][y][x]; cell (k, j, i) is centred between
+    nodes k-1..k, j-1..j, i-1..i, the outer layers being the ghost cells),
+    grid spacings, periodicity (x, y, z) and the reference's constants."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, periodic, rho_c, rhoY_c, ud, th):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.dx, self.dy, self.dz = dx, dy, dz
+        self.periodic = periodic
+        self.rho_c, self.rhoY_c = rho_c, rhoY_c
+        self.ud, self.th = ud, th
+
+
+def stratified_state_3d(nx=257, ny=129, nz=129, theta0=300.0, theta1=330.0, compressible=True):
+    """The stratified atmosphere of ``stratified_state`` on a 3D node grid:
+    x in [-1, 1] and z in [-0.5, 0.5] periodic (duplicated end nodes), y in
+    [0, 1] (gravity) with walls; compressible regime by default (the
+    pseudo-incompressible 27-point operator is singular on every in-plane
+    checkerboard mode)."""
+    ud = UserData(inx=nx, iny=ny, inz=nz, zmin=-0.5, zmax=0.5, is_compressible=1 if compressible else 0,
+                  compressibility=1.0 if compressible else 0.0)
+    ud.bdry_type[2] = BdryType.PERIODIC
+    th = Thermo(ud)
+    g = ud.gravity_strength[1]
+    a = (theta1 - theta0) / theta0
+    dx = (ud.xmax - ud.xmin) / (nx - 1)
+    dy = (ud.ymax - ud.ymin) / (ny - 1)
+    dz = (ud.zmax - ud.zmin) / (nz - 1)
+    yc = ud.ymin + (np.arange(ny + 1) - 0.5) * dy            # cell centres incl. ghosts
+    exner = 1.0 - th.Gamma * g * np.log1p(a * yc) / a
+    P = exner ** th.gm1inv
+    rho = P / (1.0 + a * yc)
+    shape = (nz + 1, ny + 1, nx + 1)
+    rho_c = np.broadcast_to(rho[None, :, None], shape).copy()
+    rhoY_c = np.broadcast_to(P[None, :, None], shape).copy()
+    return State3D(nx, ny, nz, dx, dy, dz, (True, False, True), rho_c, rhoY_c, ud, th)
+
+
+def fourier_rhs_3d(nx, ny, nz, seed, nmodes=64, kmax=12):
+    """Seeded smooth zero-mean node field (nz, ny, nx): sum of modes
+    a cos(2 pi kx s + phi) cos(pi ky t) cos(2 pi kz u + psi) with integer
+    (kx, ky, kz) != 0, |k| <= kmax, s, t, u in [0, 1] over the nodes; the
+    trapezoid-weighted mean (weights 1/2 on the duplicated periodic end
+    nodes) is removed."""
+    rng = np.random.default_rng(seed)
+    ks = [(kx, ky, kz) for kx in range(-kmax, kmax + 1) for ky in range(0, kmax + 1) for kz in range(0, kmax + 1)
+          if (kx, ky, kz) != (0, 0, 0) and kx * kx + ky * ky + kz * kz <= kmax * kmax]
+    pick = rng.choice(len(ks), size=nmodes, replace=False)
+    amp = rng.standard_normal(nmodes)
+    phi = rng.uniform(0.0, 2.0 * np.pi, (nmodes, 2))
+    s = np.linspace(0.0, 1.0, nx)
+    t = np.linspace(0.0, 1.0, ny)
+    u = np.linspace(0.0, 1.0, nz)
+    f = np.zeros((nz, ny, nx))
+    for m, k in enumerate(pick):
+        kx, ky, kz = ks[k]
+        f += amp[m] * np.cos(2 * np.pi * kz * u + phi[m, 1])[:, None, None] * \
+            np.cos(np.pi * ky * t)[None, :, None] * np.cos(2 * np.pi * kx * s + phi[m, 0])[None, None, :]
+    w = np.ones((nz, ny, nx))
+    w[:, :, 0] = w[:, :, -1] = 0.5
+    w[0] *= 0.5
+    w[-1] *= 0.5
+    return f - (w * f).sum() / w.sum()
+
+
 __all__ = ["stratified_state", "fourier_rhs", "left_null_vectors", "project_range",
            "stratified_rhs", "rising_bubble_state", "random_momentum_state",
-           "batch_member_bubble", "BdryType"]
+           "batch_member_bubble", "BdryType", "State3D", "stratified_state_3d", "fourier_rhs_3d"]
