@@ -105,6 +105,69 @@ def cpu_baseline(state, rhs, full_iters, sample_iters, dt=0.05):
     }
 
 
+# Algorithmic words per node of the 3D kernels (cell weights ~ one word per
+# node; compressible, so the centre term is read):
+#   0 matvec: read p, h, hc, dinv; write y                                 -> 5
+#   1 launch 1: read r, p, v, t, rt, x, h, hc, dinv; write r, p, v, x       -> 13
+#   2 launch 2: read r, v, rt, h, hc, dinv; write t                         -> 7
+KERNEL3_WORDS = {0: 5, 1: 13, 2: 7}
+KERNEL3_NAMES = {0: "matvec27", 1: "bicg3_launch1_update_pupdate_matvec", 2: "bicg3_launch2_s_matvec_dots"}
+
+
+def bench_3d(args, nx=257, ny=129, nz=129):
+    """Isolated 3D solve: stratified compressible atmosphere, seeded 64-mode
+    zero-mean RHS (|k| <= 12), mixed FP32 Krylov in FP64 refinement to true
+    relative residual 1e-8; per-kernel device times and algorithmic GB/s."""
+    import torch
+    from oracle import laplacian as olp
+    from pybella_amd import workloads as W
+    from pybella_amd.op3d import Operator3D, grid3_of, phys3_of
+    log(f"building 3D workload {nx}x{ny}x{nz} ...")
+    st = W.stratified_state_3d(nx, ny, nz)
+    rho = torch.from_numpy(st.rho_c).to("cuda")
+    rhoY = torch.from_numpy(st.rhoY_c).to("cuda")
+    f = torch.from_numpy(W.fourier_rhs_3d(nx, ny, nz, seed=20261203)).to("cuda")
+    op = Operator3D(grid3_of(st))
+    phys = phys3_of(st)
+    di = torch.empty(nz, ny, nx, dtype=torch.float64, device="cuda")
+    x = torch.empty_like(di)
+
+    def step(mode):
+        op.assemble(phys, 0.05, rho, rhoY, None, di)
+        b = f * di
+        _, info = op.solve(b, x, tol=1e-8, mode=mode)
+        if info["info"] != 0 or info["true_rel_resid"] > 1e-8:
+            raise SystemExit(f"3D solve did not reach the tolerance: {info}")
+        return info
+
+    rates, infos = {}, {}
+    for mode in ("mixed", "f64"):
+        step(mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            infos[mode] = step(mode)
+        torch.cuda.synchronize()
+        rates[mode] = args.steps / (time.perf_counter() - t0)
+        log(f"3D {mode}: {rates[mode]:.2f} solves/s {infos[mode]}")
+    n = nx * ny * nz
+    kernels = {}
+    for prec, wb in (("f32", 4), ("f64", 8)):
+        for k, words in KERNEL3_WORDS.items():
+            us = op.time_kernel(k, prec, args.kernel_reps)
+            alg = words * wb * n
+            kernels[f"{KERNEL3_NAMES[k]}/{prec}"] = {"avg_us": round(us, 3), "alg_bytes": alg,
+                                                     "alg_GBps": round(alg / (us * 1e-6) / 1e9, 1),
+                                                     "frac": round(alg / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+    print(json.dumps({"workload": f"isolated 3D nodal 27-point solve, {nx}x{ny}x{nz} nodes, stratified theta "
+                                  "300->330 K, compressible, seeded 64-mode zero-mean RHS, true rel. resid <= 1e-8",
+                      "solves_per_s": {k: round(v, 3) for k, v in rates.items()},
+                      "iterations": {k: v["iters"] for k, v in infos.items()},
+                      "refinements": {k: v["outer"] for k, v in infos.items()},
+                      "true_rel_resid": {k: v["true_rel_resid"] for k, v in infos.items()},
+                      "kernels": kernels, "hbm_peak_GBps": HBM_PEAK_GBS}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +183,8 @@ def main():
                          "residual) on the workload, for profiler runs")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
     ap.add_argument("--tile", type=int, default=4, help="rows per thread for --kernel-only")
+    ap.add_argument("--workload", default="2d", choices=["2d", "3d"],
+                    help="3d: the isolated 257x129x129 27-point solve (its own JSON line, not the metric)")
     ap.add_argument("--copy-probe-mib", type=int, default=None,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -140,6 +205,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": n * 8, "launches": 10}))
+        return
+
+    if args.workload == "3d":
+        bench_3d(args)
         return
 
     log("building workload 1025x513 ...")
