@@ -168,6 +168,72 @@ def bench_3d(args, nx=257, ny=129, nz=129):
                       "kernels": kernels, "hbm_peak_GBps": HBM_PEAK_GBS}), flush=True)
 
 
+def bench_batch(args, sizes=((129, 65), (257, 129), (513, 257), (1025, 513))):
+    """64 rising-bubble pressure solves (members_per_size at each grid size,
+    seeded bubble amplitude, centre and radius), each member's operator and
+    right-hand side from its own projection step; all solved by one batched
+    solve (one hipGraph per chunk of iterations over every member) vs the
+    same solves one after the other.  Tile heights come from the autotuner,
+    one per grid size."""
+    import torch
+    from pybella_amd import second_projection as sp
+    from pybella_amd import workloads as W
+    from pybella_amd.autotune import tune_tile
+    from pybella_amd.batch import BatchSolver
+    log("building batch workload ...")
+    members, ops, bs = [], [], []
+    for nx, ny in sizes:
+        for m in range(args.members_per_size):
+            Sol, Sol0, mpv, elem, node, ud, th = W.batch_member_bubble(nx, ny, m)
+            op = sp.Operator2D(sp.grid_of(node, ud))
+            pr = sp._prepare(Sol, mpv, elem, node, ud, th, 0.05, Sol0, op)
+            members.append((nx, ny, m))
+            ops.append(op)
+            bs.append(pr.b)
+    torch.cuda.synchronize()
+    tiles = {}
+    for nx, ny in sizes:   # autotune the tile height on the first member of each size
+        k = members.index((nx, ny, 0))
+        best, table = tune_tile(ops[k], lambda: ops[k].solve(bs[k], tol=1e-8, mode="mixed"),
+                                key=(nx, ny, "mixed"))
+        tiles[f"{nx}x{ny}"] = {"rows_per_thread": best, "seconds": {str(a): round(b, 5) for a, b in table.items()}}
+        for (mx, my, _), op in zip(members, ops):
+            if (mx, my) == (nx, ny):
+                op.set_tile(best)
+    batch = BatchSolver(ops)
+    out = {"workload": f"{len(ops)} rising-bubble pressure solves, {args.members_per_size} each at "
+                       + ", ".join(f"{nx}x{ny}" for nx, ny in sizes) + " nodes, true rel. resid <= 1e-8",
+           "unknowns": int(sum(op.nx * op.ny for op in ops)), "tiles": tiles}
+    for mode in ("mixed", "f64fast"):
+        batch.solve(bs, tol=1e-8, mode=mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, infos = batch.solve(bs, tol=1e-8, mode=mode)
+        torch.cuda.synchronize()
+        t_batch = (time.perf_counter() - t0) / args.steps
+        bad = [i for i in infos if i["info"] != 0 or i["true_rel_resid"] > 1e-8]
+        if bad:
+            raise SystemExit(f"batch member did not converge: {bad[0]}")
+        for op, b in zip(ops, bs):
+            op.solve(b, tol=1e-8, mode=mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for op, b in zip(ops, bs):
+                op.solve(b, tol=1e-8, mode=mode)
+        torch.cuda.synchronize()
+        t_seq = (time.perf_counter() - t0) / args.steps
+        out[mode] = {"batched_solves_per_s": round(len(ops) / t_batch, 2),
+                     "sequential_solves_per_s": round(len(ops) / t_seq, 2),
+                     "batched_ms": round(1e3 * t_batch, 2), "sequential_ms": round(1e3 * t_seq, 2),
+                     "iterations_per_size": {f"{nx}x{ny}": int(np.mean([i["iters"] for (mx, my, _), i in
+                                                                       zip(members, infos) if (mx, my) == (nx, ny)]))
+                                             for nx, ny in sizes}}
+        log(f"batch {mode}: {out[mode]}")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,8 +249,10 @@ def main():
                          "residual) on the workload, for profiler runs")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
     ap.add_argument("--tile", type=int, default=4, help="rows per thread for --kernel-only")
-    ap.add_argument("--workload", default="2d", choices=["2d", "3d"],
-                    help="3d: the isolated 257x129x129 27-point solve (its own JSON line, not the metric)")
+    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "batch"],
+                    help="3d: the isolated 257x129x129 27-point solve; batch: 64 uneven rising-bubble "
+                         "solves in one graph per chunk (each its own JSON line, not the metric)")
+    ap.add_argument("--members-per-size", type=int, default=16)
     ap.add_argument("--copy-probe-mib", type=int, default=None,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -209,6 +277,9 @@ def main():
 
     if args.workload == "3d":
         bench_3d(args)
+        return
+    if args.workload == "batch":
+        bench_batch(args)
         return
 
     log("building workload 1025x513 ...")

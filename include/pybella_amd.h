@@ -59,7 +59,9 @@ extern "C" {
 typedef struct pb_ctx pb_ctx;
 typedef struct pb_op2d pb_op2d;
 typedef struct pb_op3d pb_op3d;
+typedef struct pb_batch pb_batch;
 typedef struct pb_op3d pb_op3d;
+typedef struct pb_batch pb_batch;
 typedef struct pb_nn pb_nn;
 
 typedef struct pb_grid2d {
@@ -221,6 +223,21 @@ int pb_nn_features(pb_nn* nn, const pb_op2d* op, float* f1, float* f2);
 /* attach a network to an operator: PB_SOLVE_NN then uses it as the initial
  * guess and the preconditioner of flexible GMRES (NULL detaches) */
 int pb_op2d_attach_nn(pb_op2d* op, pb_nn* nn);
+
+/* ---- Batched uneven 2D solves (ensemble members of any grid sizes) ----
+ * No reference counterpart: the reference solves members one after the other
+ * (src/__main__.py:223-254, each through second_projection.py:230).  A batch
+ * holds distinct assembled operators (one per member, each with its own tile
+ * height); a solve captures one chunk of iterations of every member as a
+ * single hipGraph (one branch per member) and replays it until all members
+ * are final.  Per member the iterates equal a single solve's. */
+int pb_batch_create(pb_ctx* ctx, pb_op2d* const* ops, int count, pb_batch** out);
+int pb_batch_destroy(pb_batch* batch);
+/* b[m], x[m]: device FP64 Krylov vectors of member m.  Modes
+ * PB_SOLVE_F64_FAST and PB_SOLVE_MIXED.  Synchronous.  Returns 0 when every
+ * member converged, > 0 the number that did not, < 0 an error. */
+int pb_batch_solve(pb_batch* batch, const double* const* b, double* const* x, const pb_solve_opts* opts,
+                   pb_solve_info* infos);
 
 /* ---- 3D 27-point operator (stencil_27pt / lap3D, laplacian.py:872-1026) ----
  * Layout: nodes (nz, ny, nx) x fastest; operator cells (nz+1, ny+1, nx+1)

@@ -61,6 +61,9 @@ D = C.c_double
 # name -> (restype, argtypes); the list every test checks against the header
 SIGNATURES = {
     "pb_version": (I, []),
+    "pb_batch_create": (I, [VP, C.POINTER(VP), I, C.POINTER(VP)]),
+    "pb_batch_destroy": (I, [VP]),
+    "pb_batch_solve": (I, [VP, C.POINTER(VP), C.POINTER(VP), C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "pb_op3d_create": (I, [VP, C.POINTER(Grid3D), C.POINTER(VP)]),
     "pb_op3d_destroy": (I, [VP]),
     "pb_op3d_assemble": (I, [VP, C.POINTER(Phys), D, VP, VP, VP, VP]),

@@ -23,6 +23,7 @@
 // block 0 stores the new state in the other half of a double buffer.  x is
 // kept in FP64 also when the Krylov vectors are FP32.
 #include <cstdlib>
+#include <type_traits>
 
 #include "pb_op2d.hpp"
 
@@ -1055,4 +1056,297 @@ extern "C" int pb_op2d_set_tile(pb_op2d* op, int rows_per_thread) {
     return pb_fail(op->ctx, PB_ERR_ARG, "pb_op2d_set_tile: rows per thread must be 2, 4 or 8");
   op->ry = rows_per_thread;
   return PB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched uneven solves (ensemble members, possibly of different grid sizes).
+// Every member keeps its own operator, workspace, tile height and kernels, so
+// its iterates are those of a single solve.  A chunk of iterations of ALL
+// members is captured as one hipGraph: a fork from the caller's stream into
+// one branch per member stream, joined back at the end.  Each replay advances
+// every member by the chunk; members that have finished run no-op launches
+// (their state is final).  The host reads all members' states once per chunk.
+// No reference counterpart: the reference solves ensemble members one after
+// the other (src/__main__.py:223-254).
+// ---------------------------------------------------------------------------
+struct pb_batch {
+  pb_ctx* ctx = nullptr;
+  std::vector<pb_op2d*> ops;
+  std::vector<hipStream_t> streams;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  std::map<long, hipGraphExec_t> graphs;   // key: precision, chunk
+  std::vector<int> ry_captured;            // tile heights the graphs were captured with
+  KState* hst = nullptr;                   // pinned, one per member
+};
+
+namespace {
+
+template <typename T>
+int batch_graph(pb_batch* bt, int chunk, hipStream_t s, hipGraphExec_t* out) {
+  std::vector<int> ry;
+  for (pb_op2d* op : bt->ops) ry.push_back(op->ry);
+  if (ry != bt->ry_captured) {   // a member's tile changed: the captured launches are stale
+    for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
+    bt->graphs.clear();
+    bt->ry_captured = ry;
+  }
+  const long key = (long)chunk * 2 + (std::is_same<T, float>::value ? 1 : 0);
+  auto it = bt->graphs.find(key);
+  if (it == bt->graphs.end()) {
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ex = nullptr;
+    PB_HIP(bt->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    PB_HIP(bt->ctx, hipEventRecord(bt->fork, s));
+    for (size_t m = 0; m < bt->ops.size(); ++m) {
+      PB_HIP(bt->ctx, hipStreamWaitEvent(bt->streams[m], bt->fork, 0));
+      launch_chunk<T>(bt->ops[m], work_of<T>(bt->ops[m]), chunk, false, bt->streams[m]);
+      PB_HIP(bt->ctx, hipEventRecord(bt->joins[m], bt->streams[m]));
+      PB_HIP(bt->ctx, hipStreamWaitEvent(s, bt->joins[m], 0));
+    }
+    PB_HIP(bt->ctx, hipStreamEndCapture(s, &gr));
+    PB_HIP(bt->ctx, hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(gr);
+    it = bt->graphs.emplace(key, ex).first;
+  }
+  *out = it->second;
+  return PB_OK;
+}
+
+// One batched Krylov run (fast two-launch iteration, x0 = 0): member m solves
+// A_m x = b[m] to tolerance tol[m] (legacy or relative), if active[m].
+template <typename T>
+int krylov_batch(pb_batch* bt, const std::vector<char>& active, const std::vector<const T*>& b,
+                 const std::vector<double>& tol, int maxiter, int legacy, int check_every, hipStream_t s,
+                 std::vector<KState>& out) {
+  const size_t M = bt->ops.size();
+  for (size_t m = 0; m < M; ++m) {
+    pb_op2d* op = bt->ops[m];
+    Work<T>& w = work_of<T>(op);
+    KState init{};
+    init.tol = tol[m];
+    init.maxiter = maxiter;
+    init.legacy = legacy;
+    init.phase = PH_INIT;
+    init.status = active[m] ? ST_RUN : ST_CONV;
+    bt->hst[m] = init;
+    if (active[m]) {
+      PB_HIP(bt->ctx, hipMemcpyAsync(w.S, bt->hst + m, sizeof(KState), hipMemcpyHostToDevice, s));
+      const dim3 grd = stencil_grid(op, op->ry);
+      const OpDev<T> d = opdev<T>(op);
+      InitIn<T> in{nullptr, false};
+      InitOut<T> o{b[m], w.r[0], w.rt, w.p[0], w.v[0], w.t, w.x, nullptr, false, 0.0, 0.0};
+      PB_DISPATCH(k_init5, T, op, grd, s, d, in, o, w.PB);
+      launch_a<T>(op, w, 1, s);
+    } else {   // parked: both state slots final, so every launch of the chunk is a no-op
+      PB_HIP(bt->ctx, hipMemcpyAsync(w.S, bt->hst + m, sizeof(KState), hipMemcpyHostToDevice, s));
+      PB_HIP(bt->ctx, hipMemcpyAsync(w.S + 1, bt->hst + m, sizeof(KState), hipMemcpyHostToDevice, s));
+    }
+  }
+  PB_LAUNCH_CHECK(bt->ctx);
+  const int chunk = std::max(2, (check_every > 0 ? check_every : 32) & ~1);
+  hipGraphExec_t ex = nullptr;
+  const bool graphs = graphs_enabled() && s != nullptr;
+  if (graphs) {
+    int rc = batch_graph<T>(bt, chunk, s, &ex);
+    if (rc) return rc;
+  }
+  long issued = 1;
+  for (;;) {
+    for (size_t m = 0; m < M; ++m)
+      PB_HIP(bt->ctx, hipMemcpyAsync(bt->hst + m, work_of<T>(bt->ops[m]).S + 1, sizeof(KState),
+                                     hipMemcpyDeviceToHost, s));
+    PB_HIP(bt->ctx, hipStreamSynchronize(s));
+    bool done = true;
+    for (size_t m = 0; m < M; ++m) done = done && st_final(bt->hst[m].status);
+    if (done) break;
+    if (issued > (long)maxiter + 2L * chunk + 2)
+      return pb_fail(bt->ctx, PB_ERR_STATE, "batch bicgstab: %ld iterations issued without final states", issued);
+    if (graphs) {
+      PB_HIP(bt->ctx, hipGraphLaunch(ex, s));
+    } else {
+      for (size_t m = 0; m < M; ++m) launch_chunk<T>(bt->ops[m], work_of<T>(bt->ops[m]), chunk, false, s);
+    }
+    PB_LAUNCH_CHECK(bt->ctx);
+    issued += chunk;
+  }
+  out.assign(bt->hst, bt->hst + M);
+  return PB_OK;
+}
+
+}  // namespace
+
+extern "C" int pb_batch_create(pb_ctx* ctx, pb_op2d* const* ops, int count, pb_batch** out) {
+  if (!ctx || !ops || !out || count < 1 || count > 4096)
+    return pb_fail(ctx, PB_ERR_ARG, "pb_batch_create: bad argument");
+  for (int m = 0; m < count; ++m) {
+    if (!ops[m] || ops[m]->ctx != ctx) return pb_fail(ctx, PB_ERR_ARG, "pb_batch_create: member %d invalid", m);
+    for (int q = 0; q < m; ++q)
+      if (ops[q] == ops[m]) return pb_fail(ctx, PB_ERR_ARG, "pb_batch_create: operator %d repeated", m);
+  }
+  pb_batch* bt = new pb_batch();
+  bt->ctx = ctx;
+  bt->ops.assign(ops, ops + count);
+  auto fail = [&](const char* what) {
+    pb_batch_destroy(bt);
+    return pb_fail(ctx, PB_ERR_HIP, "pb_batch_create: %s failed", what);
+  };
+  for (int m = 0; m < count; ++m) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return fail("hipStreamCreate");
+    bt->streams.push_back(st);
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return fail("hipEventCreate");
+    bt->joins.push_back(ev);
+    if (alloc_work(ops[m])) return fail("workspace allocation");
+  }
+  if (hipEventCreateWithFlags(&bt->fork, hipEventDisableTiming) != hipSuccess) return fail("hipEventCreate");
+  void* hp = nullptr;
+  if (hipHostMalloc(&hp, sizeof(KState) * count, hipHostMallocDefault) != hipSuccess) return fail("hipHostMalloc");
+  bt->hst = (KState*)hp;
+  *out = bt;
+  return PB_OK;
+}
+
+extern "C" int pb_batch_destroy(pb_batch* bt) {
+  if (!bt) return PB_OK;
+  if (bt->ctx) (void)hipStreamSynchronize(bt->ctx->stream);
+  for (hipStream_t st : bt->streams) (void)hipStreamSynchronize(st);
+  for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (hipStream_t st : bt->streams) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : bt->joins) (void)hipEventDestroy(ev);
+  if (bt->fork) (void)hipEventDestroy(bt->fork);
+  if (bt->hst) (void)hipHostFree(bt->hst);
+  delete bt;
+  return PB_OK;
+}
+
+// Solve every member: (D_m^-1 A_m) x[m] = b[m], x0 = 0.  Modes:
+//   PB_SOLVE_F64_FAST  FP64 two-launch BiCGStab, legacy tolerance (as a single
+//                      PB_SOLVE_F64_FAST solve of each member);
+//   PB_SOLVE_MIXED     FP32 Krylov (FP64 iterate) inside FP64 iterative
+//                      refinement to ||b - A x|| <= tol ||b||; the inner runs
+//                      of all members still refining share one graph per
+//                      chunk; a member whose refinement stalls finishes alone
+//                      in FP64.
+// Synchronous; infos[m] as pb_bicgstab_solve's.  Returns 0 when every member
+// converged, else the number of members that did not.
+extern "C" int pb_batch_solve(pb_batch* bt, const double* const* b, double* const* x, const pb_solve_opts* opts,
+                              pb_solve_info* infos) {
+  if (!bt) return PB_ERR_ARG;
+  pb_ctx* ctx = bt->ctx;
+  if (!b || !x || !opts || !infos) return pb_fail(ctx, PB_ERR_ARG, "pb_batch_solve: null argument");
+  if (!(opts->tol >= 0) || opts->maxiter < 1) return pb_fail(ctx, PB_ERR_ARG, "pb_batch_solve: bad tol/maxiter");
+  if (opts->x0_given) return pb_fail(ctx, PB_ERR_ARG, "pb_batch_solve: x0 must be zero");
+  const size_t M = bt->ops.size();
+  for (size_t m = 0; m < M; ++m) {
+    if (!b[m] || !x[m]) return pb_fail(ctx, PB_ERR_ARG, "pb_batch_solve: member %d has a null vector", (int)m);
+    if (!bt->ops[m]->assembled) return pb_fail(ctx, PB_ERR_STATE, "pb_batch_solve: member %d not assembled", (int)m);
+  }
+  hipStream_t s = solve_stream(ctx);
+  std::vector<KState> st;
+  std::vector<char> active(M, 1);
+  int failed = 0;
+  if (opts->mode == PB_SOLVE_F64_FAST) {
+    std::vector<double> tol(M, opts->tol);
+    int rc = krylov_batch<double>(bt, active, std::vector<const double*>(b, b + M), tol, opts->maxiter, 1,
+                                  opts->check_every, s, st);
+    if (rc) return rc;
+    for (size_t m = 0; m < M; ++m) {
+      pb_op2d* op = bt->ops[m];
+      PB_HIP(ctx, hipMemcpyAsync(x[m], op->w64.x, sizeof(double) * op->n, hipMemcpyDeviceToDevice, s));
+      pb_solve_info out{};
+      out.info = info_of(st[m]);
+      out.iters = st[m].iter;
+      out.outer = 1;
+      out.half_step = st[m].half;
+      out.resid = st[m].resid;
+      out.bnrm2 = st[m].bnrm2;
+      double tr = 0.0;
+      rc = true_residual(op, b[m], x[m], s, &tr);
+      if (rc) return rc;
+      out.true_rel_resid = out.bnrm2 > 0 ? tr / out.bnrm2 : tr;
+      infos[m] = out;
+      failed += out.info != 0;
+    }
+    return failed;
+  }
+  if (opts->mode != PB_SOLVE_MIXED) return pb_fail(ctx, PB_ERR_ARG, "pb_batch_solve: mode %d not batched", opts->mode);
+  const double inner = opts->inner_tol > 0 ? opts->inner_tol : 1e-5;
+  const int max_outer = opts->max_outer > 0 ? opts->max_outer : 10;
+  std::vector<double> bn(M), rn(M), target(M);
+  std::vector<int> iters(M, 0), outer(M, 0), last(M, 0);
+  std::vector<char> stalled(M, 0);
+  int rc = 0;
+  for (size_t m = 0; m < M; ++m) {
+    pb_op2d* op = bt->ops[m];
+    hipLaunchKernelGGL(k_fill_const, dim3(elem_blocks(op->n)), dim3(256), 0, s, op->n, x[m], 0.0);
+    rc = true_residual(op, b[m], x[m], s, &bn[m]);   // op->rtrue = b
+    if (rc) return rc;
+    rn[m] = bn[m];
+    target[m] = opts->tol * bn[m];
+  }
+  for (int round = 0; round < max_outer; ++round) {
+    bool any = false;
+    std::vector<const float*> bf(M);
+    for (size_t m = 0; m < M; ++m) {
+      pb_op2d* op = bt->ops[m];
+      active[m] = rn[m] > target[m] && !stalled[m];
+      any = any || active[m];
+      bf[m] = op->w32.b;
+      if (active[m])
+        hipLaunchKernelGGL((k_cast<double, float>), dim3(elem_blocks(op->n)), dim3(256), 0, s, op->n,
+                           (const double*)op->rtrue, op->w32.b);
+    }
+    if (!any) break;
+    rc = krylov_batch<float>(bt, active, bf, std::vector<double>(M, inner), opts->maxiter, 0, opts->check_every, s,
+                             st);
+    if (rc) return rc;
+    for (size_t m = 0; m < M; ++m) {
+      if (!active[m]) continue;
+      pb_op2d* op = bt->ops[m];
+      const int nb = elem_blocks(op->n);
+      iters[m] += st[m].iter;
+      last[m] = info_of(st[m]);
+      ++outer[m];
+      const double old = rn[m];
+      hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, s, op->n, 1.0, (const double*)op->w32.x, x[m]);
+      rc = true_residual(op, b[m], x[m], s, &rn[m]);
+      if (rc) return rc;
+      if (!(rn[m] < old)) {   // no progress: undo the correction
+        hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, s, op->n, -1.0, (const double*)op->w32.x, x[m]);
+        rc = true_residual(op, b[m], x[m], s, &rn[m]);
+        if (rc) return rc;
+      }
+      if (!(rn[m] < 0.5 * old)) stalled[m] = 1;
+    }
+  }
+  for (size_t m = 0; m < M; ++m) {
+    pb_op2d* op = bt->ops[m];
+    if (stalled[m] && rn[m] > target[m]) {   // finish alone in FP64 from the current iterate
+      KState h64{};
+      PB_HIP(ctx, hipMemcpyAsync(op->xcorr, op->rtrue, sizeof(double) * op->n, hipMemcpyDeviceToDevice, s));
+      rc = krylov_run<double>(op, op->w64, op->xcorr, nullptr, target[m] / std::max(rn[m], 1e-300), opts->maxiter,
+                              0, opts->check_every, s, &h64);
+      if (rc) return rc;
+      iters[m] += h64.iter;
+      last[m] = info_of(h64);
+      hipLaunchKernelGGL(k_axpy, dim3(elem_blocks(op->n)), dim3(256), 0, s, op->n, 1.0, (const double*)op->w64.x,
+                         x[m]);
+      rc = true_residual(op, b[m], x[m], s, &rn[m]);
+      if (rc) return rc;
+      ++outer[m];
+    }
+    pb_solve_info out{};
+    out.info = rn[m] <= target[m] ? 0 : (last[m] < 0 ? last[m] : std::max(iters[m], 1));
+    out.iters = iters[m];
+    out.outer = outer[m];
+    out.resid = rn[m];
+    out.bnrm2 = bn[m];
+    out.true_rel_resid = bn[m] > 0 ? rn[m] / bn[m] : rn[m];
+    out.fallback = stalled[m];
+    infos[m] = out;
+    failed += out.info != 0;
+  }
+  return failed;
 }

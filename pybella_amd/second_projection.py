@@ -212,40 +212,47 @@ def _bc_cells(ctx, grid, phys, ds: _DevState, rhoY0):
               "pb_bc_cells")
 
 
-def euler_backward_non_advective_impl_part(Sol, mpv, elem, node, ud, th, t, dt, alpha_diff, Sol0=None,
-                                           writer=None, label=None, debug=False):
-    """GPU second projection with the reference's signature and side effects."""
-    global total_iter, total_calls
+class _Projection:
+    """Device state of one projection step between assembly and the update."""
+
+
+def _prepare(Sol, mpv, elem, node, ud, th, dt, Sol0, op):
+    """second_projection.py:113-214 on the device: ghost cells, the correction
+    with the old p2, divergence / dt, the regime blend, operator assembly and
+    the scaled right-hand side.  Returns a _Projection holding every device
+    array the solve and the update need."""
     torch = _torch()
     if ud.is_compressible == 0 and ud.is_ArakawaKonor:
         raise NotImplementedError("Arakawa-Konor branch (second_projection.py:145-147) is not runnable "
                                   "in the reference (shape mismatch of wcenter and dp2_nodes)")
     if hasattr(ud, "LAMB_BDRY"):
         raise NotImplementedError("LAMB_BDRY operator variant is not part of this path")
-    grid = grid_of(node, ud)
-    phys = phys_of(ud, th)
-    op = operator_for(node, ud)
-    ctx = op.ctx
+    pr = _Projection()
+    pr.grid = grid = grid_of(node, ud)
+    pr.phys = phys = phys_of(ud, th)
+    pr.op = op
+    pr.ctx = ctx = op.ctx
+    pr.dt = dt
     ctx.sync_stream()
     nx, ny = grid.nx, grid.ny
-    dS = to_dev(dSdy_cells(mpv, elem, node))
-    rhoY0 = torch.from_numpy(np.ascontiguousarray(mpv.HydroState.rhoY0, dtype=np.float64)).to("cuda")
-    sol = _DevState(Sol)
-    sol0 = _DevState(Sol0) if Sol0 is not None else None
+    pr.dS = to_dev(dSdy_cells(mpv, elem, node))
+    pr.rhoY0 = torch.from_numpy(np.ascontiguousarray(mpv.HydroState.rhoY0, dtype=np.float64)).to("cuda")
+    pr.sol = sol = _DevState(Sol)
+    pr.sol0 = sol0 = _DevState(Sol0) if Sol0 is not None else None
     S = sol0 if sol0 is not None else sol
-    p2n = to_dev(mpv.p2_nodes)
+    pr.p2n = p2n = to_dev(mpv.p2_nodes)
     cells = lambda: torch.empty(ny + 3, nx + 3, dtype=torch.float64, device="cuda")
-    u, v, w = cells(), cells(), cells()
+    pr.u, pr.v, pr.w = cells(), cells(), cells()
     F = sol.f
     # :113-118 ghost cells + coefficients' state
-    _bc_cells(ctx, grid, phys, S, rhoY0)
+    _bc_cells(ctx, grid, phys, S, pr.rhoY0)
     # :130-132 correction with the old p2, then ghost cells
     ctx.check(ctx.lib.pb_ghostnodes(ctx.h, C.byref(grid), _abi.ptr(p2n)), "pb_ghostnodes")
     ctx.check(ctx.lib.pb_correction_nodes(ctx.h, C.byref(grid), C.byref(phys), float(dt), 0.0, _abi.ptr(p2n),
                                           _abi.ptr(F["rho"]), _abi.ptr(F["rhou"]), _abi.ptr(F["rhov"]),
-                                          _abi.ptr(F["rhoY"]), _abi.ptr(F["rhoX"]), _abi.ptr(dS),
-                                          _abi.ptr(u), _abi.ptr(v), _abi.ptr(w)), "pb_correction_nodes")
-    _bc_cells(ctx, grid, phys, sol, rhoY0)
+                                          _abi.ptr(F["rhoY"]), _abi.ptr(F["rhoX"]), _abi.ptr(pr.dS),
+                                          _abi.ptr(pr.u), _abi.ptr(pr.v), _abi.ptr(pr.w)), "pb_correction_nodes")
+    _bc_cells(ctx, grid, phys, sol, pr.rhoY0)
     # :134-139 divergence / dt
     rhs = torch.empty(ny + 2, nx + 2, dtype=torch.float64, device="cuda")
     ctx.check(ctx.lib.pb_divergence_nodes(ctx.h, C.byref(grid), float(dt), _abi.ptr(F["rho"]),
@@ -254,46 +261,90 @@ def euler_backward_non_advective_impl_part(Sol, mpv, elem, node, ud, th, t, dt, 
     c = float(ud.compressibility)
     if ud.is_compressible == 0 and c not in (0.0, 1.0):
         rhs = c * rhs + (1.0 - c) * rhs          # :150, exact elementwise blend
+    pr.rhs = rhs
     # :115/:118, :141-153, :165-175 operator assembly (wcenter regime-scaled)
-    wplus0, wplus1, nu_c = cells(), cells(), cells()
-    wcenter = torch.empty(ny + 2, nx + 2, dtype=torch.float64, device="cuda")
-    diag_inv = torch.empty_like(wcenter)
-    op.assemble(phys, dt, S.f["rho"], S.f["rhoY"], F["rho"], F["rhoY"], dS, wplus0, wplus1, nu_c,
-                wcenter, diag_inv)
-    # :170, :214 scaled right-hand side; :230 solve
-    b = op.scale_rhs(rhs)
-    mode = solver_options.get("mode", "f64")
-    x, info = op.solve(b, tol=float(ud.tol), maxiter=int(ud.max_iterations), mode=mode,
-                       inner_tol=solver_options.get("inner_tol", 1e-3),
-                       max_outer=solver_options.get("max_outer", 12),
-                       check_every=solver_options.get("check_every", 32))
+    pr.wplus0, pr.wplus1, pr.nu_c = cells(), cells(), cells()
+    pr.wcenter = torch.empty(ny + 2, nx + 2, dtype=torch.float64, device="cuda")
+    pr.diag_inv = torch.empty_like(pr.wcenter)
+    op.assemble(phys, dt, S.f["rho"], S.f["rhoY"], F["rho"], F["rhoY"], pr.dS, pr.wplus0, pr.wplus1, pr.nu_c,
+                pr.wcenter, pr.diag_inv)
+    # :170, :214 scaled right-hand side
+    pr.b = op.scale_rhs(rhs)
+    return pr
+
+
+def _finish(pr, x, info, Sol, Sol0, mpv):
+    """second_projection.py:235-277: counters, scatter of the solution, the
+    correction with dp2, the Exner update, ghost cells; then the side effects
+    are written back to the host objects."""
+    global total_iter, total_calls
+    torch = _torch()
+    ctx, grid, phys, F = pr.ctx, pr.grid, pr.phys, pr.sol.f
+    nx, ny = grid.nx, grid.ny
     total_iter += info["iters"]
     total_calls += 1
     logging.info(info["iters"])
     logging.info("Total calls to BiCGStab routine = %i, total iterations = %i" % (total_calls, total_iter))
-    # :241-277 scatter, correction with dp2, p2 update, ghost cells
     p2_full = torch.empty(ny + 4, nx + 4, dtype=torch.float64, device="cuda")
     ctx.check(ctx.lib.pb_scatter_update_p2(ctx.h, C.byref(grid), _abi.ptr(x), _abi.ptr(p2_full),
-                                           _abi.ptr(p2n)), "pb_scatter_update_p2")
-    ctx.check(ctx.lib.pb_correction_nodes(ctx.h, C.byref(grid), C.byref(phys), float(dt), 1.0,
+                                           _abi.ptr(pr.p2n)), "pb_scatter_update_p2")
+    ctx.check(ctx.lib.pb_correction_nodes(ctx.h, C.byref(grid), C.byref(phys), float(pr.dt), 1.0,
                                           _abi.ptr(p2_full), _abi.ptr(F["rho"]), _abi.ptr(F["rhou"]),
                                           _abi.ptr(F["rhov"]), _abi.ptr(F["rhoY"]), _abi.ptr(F["rhoX"]),
-                                          _abi.ptr(dS), _abi.ptr(u), _abi.ptr(v), _abi.ptr(w)),
+                                          _abi.ptr(pr.dS), _abi.ptr(pr.u), _abi.ptr(pr.v), _abi.ptr(pr.w)),
               "pb_correction_nodes")
-    _bc_cells(ctx, grid, phys, sol, rhoY0)
+    _bc_cells(ctx, grid, phys, pr.sol, pr.rhoY0)
     torch.cuda.synchronize()
-    # write back the side effects
-    sol.download(Sol)
-    if sol0 is not None:
-        sol0.download(Sol0)
-    from_dev(p2n, mpv.p2_nodes)
-    mpv.rhs[...] = from_dev(rhs)
-    mpv.wplus[0][...] = from_dev(wplus0)
-    mpv.wplus[1][...] = from_dev(wplus1)
-    mpv.wcenter = from_dev(wcenter)
-    mpv.nu_c = from_dev(nu_c)
-    mpv.u, mpv.v, mpv.w = from_dev(u), from_dev(v), from_dev(w)
+    pr.sol.download(Sol)
+    if pr.sol0 is not None:
+        pr.sol0.download(Sol0)
+    from_dev(pr.p2n, mpv.p2_nodes)
+    mpv.rhs[...] = from_dev(pr.rhs)
+    mpv.wplus[0][...] = from_dev(pr.wplus0)
+    mpv.wplus[1][...] = from_dev(pr.wplus1)
+    mpv.wcenter = from_dev(pr.wcenter)
+    mpv.nu_c = from_dev(pr.nu_c)
+    mpv.u, mpv.v, mpv.w = from_dev(pr.u), from_dev(pr.v), from_dev(pr.w)
+
+
+def euler_backward_non_advective_impl_part(Sol, mpv, elem, node, ud, th, t, dt, alpha_diff, Sol0=None,
+                                           writer=None, label=None, debug=False):
+    """GPU second projection with the reference's signature and side effects
+    (second_projection.py:97-277).  Returns the solver info dict."""
+    op = operator_for(node, ud)
+    pr = _prepare(Sol, mpv, elem, node, ud, th, dt, Sol0, op)
+    mode = solver_options.get("mode", "f64")
+    x, info = op.solve(pr.b, tol=float(ud.tol), maxiter=int(ud.max_iterations), mode=mode,
+                       inner_tol=solver_options.get("inner_tol", 1e-3),
+                       max_outer=solver_options.get("max_outer", 12),
+                       check_every=solver_options.get("check_every", 32))
+    _finish(pr, x, info, Sol, Sol0, mpv)
     return info
+
+
+def euler_backward_non_advective_impl_part_batch(members, dt, mode="mixed", ops=None, batch=None):
+    """The projection step of many ensemble members (grids of any sizes) with
+    all their pressure solves in one batched solve (pb_batch_solve: one
+    hipGraph per chunk of iterations covering every member).  ``members`` is a
+    list of (Sol, mpv, elem, node, ud, th, Sol0) tuples, each updated in place
+    exactly as euler_backward_non_advective_impl_part updates one member;
+    ``ops`` (one Operator2D per member) and ``batch`` (a BatchSolver over
+    them) may be passed in to be reused across time steps.  Returns the list
+    of per-member info dicts."""
+    from .batch import BatchSolver
+    if ops is None:
+        ops = [Operator2D(grid_of(m[3], m[4])) for m in members]
+    preps = [_prepare(Sol, mpv, elem, node, ud, th, dt, Sol0, op)
+             for (Sol, mpv, elem, node, ud, th, Sol0), op in zip(members, ops)]
+    batch = batch or BatchSolver(ops)
+    ud0 = members[0][4]
+    xs, infos = batch.solve([pr.b for pr in preps], tol=float(ud0.tol), maxiter=int(ud0.max_iterations), mode=mode,
+                            inner_tol=solver_options.get("inner_tol", 1e-3),
+                            max_outer=solver_options.get("max_outer", 12),
+                            check_every=solver_options.get("check_every", 32))
+    for pr, x, info, (Sol, mpv, elem, node, ud, th, Sol0) in zip(preps, xs, infos, members):
+        _finish(pr, x, info, Sol, Sol0, mpv)
+    return infos
 
 
 def solve_isolated(Sol, mpv, elem, node, ud, th, dt, rhs_isc, mode="f64", tol=None, maxiter=None,
