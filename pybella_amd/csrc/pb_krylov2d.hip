@@ -734,7 +734,9 @@ bool graphs_enabled() {
 template <typename T>
 int run_chunk(pb_op2d* op, Work<T>& w, int c, bool exact, hipStream_t s) {
   if (graphs_enabled() && s != nullptr) {
-    const int key = c * 16 + (exact ? 1 : 0) + 2 * op->ry;
+    // captured launches are specialised on the tile height and on the
+    // operator's (cross, center) flags, which a re-assembly may change
+    const int key = (((c * 2 + (exact ? 1 : 0)) * 16 + op->ry) * 2 + (op->cross ? 1 : 0)) * 2 + (op->center ? 1 : 0);
     auto it = w.graphs.find(key);
     if (it == w.graphs.end()) {
       hipGraph_t gr = nullptr;
@@ -1076,7 +1078,7 @@ struct pb_batch {
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> joins;
   std::map<long, hipGraphExec_t> graphs;   // key: precision, chunk
-  std::vector<int> ry_captured;            // tile heights the graphs were captured with
+  std::vector<int> ry_captured;            // member launch signatures the graphs were captured with
   KState* hst = nullptr;                   // pinned, one per member
 };
 
@@ -1084,9 +1086,9 @@ namespace {
 
 template <typename T>
 int batch_graph(pb_batch* bt, int chunk, hipStream_t s, hipGraphExec_t* out) {
-  std::vector<int> ry;
-  for (pb_op2d* op : bt->ops) ry.push_back(op->ry);
-  if (ry != bt->ry_captured) {   // a member's tile changed: the captured launches are stale
+  std::vector<int> ry;   // launch signature of every member: tile height and (cross, center)
+  for (pb_op2d* op : bt->ops) ry.push_back(op->ry * 4 + (op->cross ? 2 : 0) + (op->center ? 1 : 0));
+  if (ry != bt->ry_captured) {   // a member's tile or operator kind changed: the launches are stale
     for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
     bt->graphs.clear();
     bt->ry_captured = ry;

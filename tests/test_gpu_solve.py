@@ -183,3 +183,34 @@ def test_impl_part_dropin(case):
     a = strip_null(B[2].p2_nodes[2:-2, 2:-2].T)
     r = strip_null(A[2].p2_nodes[2:-2, 2:-2].T)
     assert np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-300) < 1e-9
+
+
+def test_reassembly_with_other_operator_kind():
+    """One handle: a pseudo-incompressible solve (no centre term) captures its
+    iteration graphs; re-assembling the handle with a compressible state must
+    not replay them.  The second solve equals a fresh handle's solve."""
+    import torch
+    sp = _sp()
+    print("case reassembly psinc -> compressible", flush=True)
+    A = W.random_momentum_state(33, 17, seed=1)
+    B = W.random_momentum_state(33, 17, seed=2, compressible=True)
+
+    def assemble(op, case):
+        Sol, mpv, elem, node, ud, th = case
+        rho, rhoY = sp.to_dev(Sol.rho), sp.to_dev(Sol.rhoY)
+        op.assemble(sp.phys_of(ud, th), 0.05, rho, rhoY, rho, rhoY, sp.to_dev(sp.dSdy_cells(mpv, elem, node)))
+        return op.scale_rhs(sp.to_dev(W.stratified_rhs(node, seed=3)))
+
+    shared = sp.Operator2D(sp.grid_of(A[3], A[4]))
+    for mode in ("f64", "f64fast", "mixed"):
+        b = assemble(shared, A)
+        shared.solve(b, tol=1e-10, mode=mode)
+        b = assemble(shared, B)
+        x1, i1 = shared.solve(b, tol=1e-10, mode=mode)
+        fresh = sp.Operator2D(sp.grid_of(B[3], B[4]))
+        b2 = assemble(fresh, B)
+        x2, i2 = fresh.solve(b2, tol=1e-10, mode=mode)
+        torch.cuda.synchronize()
+        assert i1["info"] == 0 and i2["info"] == 0, (mode, i1, i2)
+        assert i1["iters"] == i2["iters"] and torch.equal(x1, x2), mode
+        fresh.close()
