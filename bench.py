@@ -131,6 +131,14 @@ def bench_3d(args, nx=257, ny=129, nz=129):
     phys = phys3_of(st)
     di = torch.empty(nz, ny, nx, dtype=torch.float64, device="cuda")
     x = torch.empty_like(di)
+    if args.kernel_only is not None:   # one kernel only, for profiler counter runs
+        op.assemble(phys, 0.05, rho, rhoY, None, di)
+        k = args.kernel_only
+        us = op.time_kernel(k, args.precision, args.kernel_reps)
+        alg = KERNEL3_WORDS[k] * (4 if args.precision == "f32" else 8) * nx * ny * nz
+        print(json.dumps({"kernel": KERNEL3_NAMES[k], "precision": args.precision, "avg_us": us,
+                          "alg_bytes": alg, "alg_GBps": alg / (us * 1e-6) / 1e9}), flush=True)
+        return
 
     def step(mode):
         op.assemble(phys, 0.05, rho, rhoY, None, di)
