@@ -12,8 +12,8 @@
 // cross term of lap3D (:1017-1022) is zero for every state here
 // (coriolis_strength[0] = 0) and is not built.
 //
-// Kernel shape: a 2.5D march.  A 256-thread block owns a 64 x 4 node tile
-// and a chunk of 8 z planes; two node planes (with a one-node halo) and one
+// Kernel shape: a 2.5D march.  A 512-thread block owns a 64 x 8 node tile
+// and a chunk of 16 z planes; two node planes (with a one-node halo) and one
 // plane of cell fluxes live in LDS.  Each cell plane is computed once and
 // used by the node planes below and above it: for node plane k,
 //   y = D(k-1/2) + D(k+1/2) + Z(k+1/2) - Z(k-1/2)
@@ -34,7 +34,8 @@
 
 namespace {
 
-constexpr int X3 = 64, Y3 = 4, Z3 = 8;
+constexpr int X3 = 64, Y3 = 8, Z3 = 16;
+constexpr int NT3 = X3 * Y3;   // threads of a marching block
 constexpr int kNV = 5;   // max partials per block
 
 enum : int { S3_RUN = 0, S3_CONV = 1, S3_MAXIT = 3, S3_BRK_RHO = -10, S3_BRK_OMEGA = -11,
@@ -78,17 +79,35 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (
   const int nx = o.nx, ny = o.ny;
   const int i = x0 + tx, j = y0 + ty;
   const bool own = i < nx && j < ny;
-  auto load_plane = [&](int k, int slot) {
+  // a node plane with its halo is NQ values, at most QPT per thread
+  constexpr int NQ = (Y3 + 2) * (X3 + 2), QPT = (NQ + NT3 - 1) / NT3;
+  auto fetch = [&](int k, T (&r)[QPT]) {   // global loads of plane k into registers
     const long kk = wrap3(k, o.nz, o.pz);
-    for (int q = tid; q < (Y3 + 2) * (X3 + 2); q += X3 * Y3) {
-      const int jj = q / (X3 + 2), ii = q - jj * (X3 + 2);
-      const int gj = wrap3(y0 - 1 + jj, ny, o.py), gi = wrap3(x0 - 1 + ii, nx, o.px);
-      P[slot][jj][ii] = in((kk * ny + gj) * nx + gi);
+#pragma unroll
+    for (int e = 0; e < QPT; ++e) {
+      const int q = tid + e * NT3;
+      if (q < NQ) {
+        const int jj = q / (X3 + 2), ii = q - jj * (X3 + 2);
+        const int gj = wrap3(y0 - 1 + jj, ny, o.py), gi = wrap3(x0 - 1 + ii, nx, o.px);
+        r[e] = in((kk * ny + gj) * nx + gi);
+      }
     }
+  };
+  auto store = [&](int slot, const T (&r)[QPT]) {
+#pragma unroll
+    for (int e = 0; e < QPT; ++e) {
+      const int q = tid + e * NT3;
+      if (q < NQ) P[slot][q / (X3 + 2)][q % (X3 + 2)] = r[e];
+    }
+  };
+  auto load_plane = [&](int k, int slot) {
+    T r[QPT];
+    fetch(k, r);
+    store(slot, r);
   };
   auto cells = [&](int k, int s0, int s1) {   // cell plane between node planes k and k+1
     const long ck = k + 1;
-    for (int q = tid; q < (Y3 + 1) * (X3 + 1); q += X3 * Y3) {
+    for (int q = tid; q < (Y3 + 1) * (X3 + 1); q += NT3) {
       const int cj = q / (X3 + 1), ci = q - cj * (X3 + 1);
       const int cy = y0 + cj, cx = x0 + ci;
       T h = 0;
@@ -110,6 +129,8 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (
   };
   load_plane(kb - 1, 0);
   load_plane(kb, 1);
+  T pre[QPT];
+  fetch(kb + 1, pre);       // plane kb+1 in flight during the first cell plane
   __syncthreads();
   cells(kb - 1, 0, 1);
   __syncthreads();
@@ -118,7 +139,8 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (
   for (int k = kb; k < ke; ++k) {
     const int sk = (k - kb + 1) & 1, sn = sk ^ 1;
     __syncthreads();          // F and the slot of plane k-1 are free
-    load_plane(k + 1, sn);
+    store(sn, pre);           // plane k+1
+    if (k + 1 < ke) fetch(k + 2, pre);   // next plane in flight during this step's compute
     __syncthreads();
     cells(k, sk, sn);
     __syncthreads();
@@ -139,7 +161,7 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (
 
 template <int NV>
 __device__ __forceinline__ void write_partials3(double (&acc)[NV], double* part) {
-  __shared__ double sh[NV * 4];
+  __shared__ double sh[NV * 16];   // one slot per wave of a block of up to 1024 threads
   block_sum<NV>(acc, sh);
   if (tid3() == 0)
 #pragma unroll
@@ -173,13 +195,13 @@ __device__ __forceinline__ void reduce_records(const double* part, int nb, doubl
 // kernels
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k3_matvec(Op3<T> o, const T* p, T* y) {
+__global__ __launch_bounds__(NT3) void k3_matvec(Op3<T> o, const T* p, T* y) {
   double acc[1] = {0.0};
   march3<T, 0>(o, [&](long g) { return p[g]; }, [&](long g, T v, T, double(&)[1]) { y[g] = v; }, acc);
 }
 
 // r = b - A x (FP64), r32 = (float) r, partials of r.r
-__global__ __launch_bounds__(256) void k3_resid(Op3<double> o, const double* x, const double* b, double* r,
+__global__ __launch_bounds__(NT3) void k3_resid(Op3<double> o, const double* x, const double* b, double* r,
                                                 float* r32, double* part) {
   double acc[1] = {0.0};
   march3<double, 1>(o, [&](long g) { return x[g]; },
@@ -233,7 +255,7 @@ __global__ void k3_s0(KS3* st, const double* part, int nb, double tol, int maxit
 // launch 1 of iteration k: r_{k-1} = r - alpha v - omega t, x += alpha p + omega (r - alpha v),
 // p_k = r_{k-1} + beta (p - omega v), v_k = A p_k; partials of rt.v_k.  Reads set a, writes set b.
 template <typename T>
-__global__ __launch_bounds__(256) void k3_iter1(Op3<T> o, const KS3* st, const T* ra, const T* pa, const T* va,
+__global__ __launch_bounds__(NT3) void k3_iter1(Op3<T> o, const KS3* st, const T* ra, const T* pa, const T* va,
                                                 const T* t, const T* rt, T* rb, T* pb, T* vb, T* x, double* part) {
   if (st->status != S3_RUN) return;
   const T al = (T)st->alpha, om = (T)st->omega, be = (T)st->beta;
@@ -275,7 +297,7 @@ __global__ void k3_s1(KS3* st, const double* part, int nb) {
 
 // launch 2: s = r - alpha v (on the fly), t = A s; partials s.s, t.s, t.t, rt.s, rt.t
 template <typename T>
-__global__ __launch_bounds__(256) void k3_iter2(Op3<T> o, const KS3* st, const T* r, const T* v, const T* rt, T* t,
+__global__ __launch_bounds__(NT3) void k3_iter2(Op3<T> o, const KS3* st, const T* r, const T* v, const T* rt, T* t,
                                                 double* part) {
   if (st->status != S3_RUN) return;
   const T al = (T)st->alpha;
