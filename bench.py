@@ -209,6 +209,24 @@ def bench_batch(args, sizes=((129, 65), (257, 129), (513, 257), (1025, 513))):
             if (mx, my) == (nx, ny):
                 op.set_tile(best)
     batch = BatchSolver(ops)
+    # one tile height for the whole batch lets it run one launch over every
+    # member's tiles per phase; the autotuner times the batch itself
+    per_size = {v["rows_per_thread"] for v in tiles.values()}
+    btable = {}
+    for ry in (2, 4, 8):
+        for op in ops:
+            op.set_tile(ry)
+        batch.solve(bs, tol=1e-8, mode="f64fast")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch.solve(bs, tol=1e-8, mode="f64fast")
+        torch.cuda.synchronize()
+        btable[ry] = time.perf_counter() - t0
+    bry = min(btable, key=btable.get)
+    for op in ops:
+        op.set_tile(bry)
+    tiles["batch"] = {"rows_per_thread": bry, "seconds": {str(a): round(b, 5) for a, b in btable.items()},
+                      "per_size_choices": sorted(per_size)}
     out = {"workload": f"{len(ops)} rising-bubble pressure solves, {args.members_per_size} each at "
                        + ", ".join(f"{nx}x{ny}" for nx, ny in sizes) + " nodes, true rel. resid <= 1e-8",
            "unknowns": int(sum(op.nx * op.ny for op in ops)), "tiles": tiles}
@@ -261,6 +279,8 @@ def main():
                     help="3d: the isolated 257x129x129 27-point solve; batch: 64 uneven rising-bubble "
                          "solves in one graph per chunk (each its own JSON line, not the metric)")
     ap.add_argument("--members-per-size", type=int, default=16)
+    ap.add_argument("--batch-sizes", default="129x65,257x129,513x257,1025x513",
+                    help="grid sizes of the batch workload, NXxNY comma-separated")
     ap.add_argument("--copy-probe-mib", type=int, default=None,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -287,7 +307,7 @@ def main():
         bench_3d(args)
         return
     if args.workload == "batch":
-        bench_batch(args)
+        bench_batch(args, tuple(tuple(int(v) for v in z.split("x")) for z in args.batch_sizes.split(",")))
         return
 
     log("building workload 1025x513 ...")

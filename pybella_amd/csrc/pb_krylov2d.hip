@@ -74,28 +74,30 @@ struct InitOut {
   }
 };
 
+// tile (bx, by) of an operator whose launch has nb blocks, this one numbered bid
 template <int NREC, typename T, bool CROSS, bool CENTER, int RY>
-__device__ __forceinline__ void init_body(OpDev<T>& op, InitIn<T>& in, InitOut<T>& out, double* P) {
+__device__ __forceinline__ void init_body(OpDev<T>& op, InitIn<T>& in, InitOut<T>& out, double* P, int bx, int by,
+                                          int bid, int nb) {
   __shared__ TileLds<T, RY> L;
   __shared__ double sred[8 * (NTHR / kWave)];
   out.rr = 0.0;
   out.bb = 0.0;
-  stencil_tile2<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, [] { return true; });
+  stencil_tile2<T, CROSS, CENTER, RY>(op, bx, by, in, out, L, [] { return true; });
   double v[NREC];
 #pragma unroll
   for (int k = 0; k < NREC; ++k) v[k] = 0.0;
   v[0] = out.rr;
   v[1] = out.bb;
-  write_partials<NREC>(v, sred, P, nblocks(), block_id());
+  write_partials<NREC>(v, sred, P, nb, bid);
 }
 
 template <typename T, bool CROSS, bool CENTER, int RY>
 __global__ __launch_bounds__(NTHR) void k_init5(OpDev<T> op, InitIn<T> in, InitOut<T> out, double* P) {
-  init_body<5, T, CROSS, CENTER, RY>(op, in, out, P);
+  init_body<5, T, CROSS, CENTER, RY>(op, in, out, P, blockIdx.x, blockIdx.y, block_id(), nblocks());
 }
 template <typename T, bool CROSS, bool CENTER, int RY>
 __global__ __launch_bounds__(NTHR) void k_init2(OpDev<T> op, InitIn<T> in, InitOut<T> out, double* P) {
-  init_body<2, T, CROSS, CENTER, RY>(op, in, out, P);
+  init_body<2, T, CROSS, CENTER, RY>(op, in, out, P, blockIdx.x, blockIdx.y, block_id(), nblocks());
 }
 
 // ---------------------------------------------------------------------------
@@ -252,11 +254,11 @@ struct AOut {
 };
 
 template <typename T, bool CROSS, bool CENTER, int RY>
-__global__ __launch_bounds__(NTHR) void k_iter_a(OpDev<T> op, IterPtrs<T> q, const KState* Sin, KState* Sout,
-                                                 const double* PB, double* PA) {
+__device__ __forceinline__ void iter_a_body(const OpDev<T>& op, const IterPtrs<T>& q, const KState* Sin, KState* Sout,
+                                            const double* PB, double* PA, int bx, int by, int bid, int nb) {
   __shared__ TileLds<T, RY> L;
   __shared__ double sred[8 * (NTHR / kWave)];
-  const int tid = thread_id(), bid = block_id(), nb = nblocks();
+  const int tid = thread_id();
   AIn<T> in{q.r_prev, q.p_prev, q.v_prev, q.t, T(0), T(0), T(0)};
   AOut<T> out{q, T(0), T(0), 0.0, 0.0, 0, 0.0};
   KState st;
@@ -277,7 +279,7 @@ __global__ __launch_bounds__(NTHR) void k_iter_a(OpDev<T> op, IterPtrs<T> q, con
         constexpr int TY = TYT * RY;
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-          const int j = blockIdx.y * TY + threadIdx.y + TYT * r, i = blockIdx.x * TX + threadIdx.x;
+          const int j = by * TY + threadIdx.y + TYT * r, i = bx * TX + threadIdx.x;
           if (j < op.ny && i < op.nx) {
             const long g = (long)j * op.nx + i;
             const double xg = q.x[g] + st.alpha * (double)q.p_prev[g];
@@ -303,11 +305,17 @@ __global__ __launch_bounds__(NTHR) void k_iter_a(OpDev<T> op, IterPtrs<T> q, con
     out.xupd = st.x_upd;
     return true;
   };
-  stencil_tile2<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, pro);
+  stencil_tile2<T, CROSS, CENTER, RY>(op, bx, by, in, out, L, pro);
   if (st.status != ST_RUN || st_final(st.status)) return;
   double v[1] = {out.rv};
   write_partials<1>(v, sred, PA, nb, bid);
   if (bid == 0 && tid == 0) *Sout = st;
+}
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_iter_a(OpDev<T> op, IterPtrs<T> q, const KState* Sin, KState* Sout,
+                                                 const double* PB, double* PA) {
+  iter_a_body<T, CROSS, CENTER, RY>(op, q, Sin, Sout, PB, PA, blockIdx.x, blockIdx.y, block_id(), nblocks());
 }
 
 // ---------------------------------------------------------------------------
@@ -342,11 +350,12 @@ struct BOut {
 };
 
 template <typename T, bool CROSS, bool CENTER, int RY>
-__global__ __launch_bounds__(NTHR) void k_iter_b(OpDev<T> op, const T* r, const T* v, const T* rt, T* t,
-                                                 const KState* Sin, KState* Sout, const double* PA, double* PB) {
+__device__ __forceinline__ void iter_b_body(const OpDev<T>& op, const T* r, const T* v, const T* rt, T* t,
+                                            const KState* Sin, KState* Sout, const double* PA, double* PB, int bx,
+                                            int by, int bid, int nb) {
   __shared__ TileLds<T, RY> L;
   __shared__ double sred[8 * (NTHR / kWave)];
-  const int tid = thread_id(), bid = block_id(), nb = nblocks();
+  const int tid = thread_id();
   BIn<T> in{r, v, T(0)};
   BOut<T> out{t, rt, 0.0, 0.0, 0.0, 0.0, 0.0};
   KState st;
@@ -371,11 +380,18 @@ __global__ __launch_bounds__(NTHR) void k_iter_b(OpDev<T> op, const T* r, const 
     run = true;
     return true;
   };
-  stencil_tile2<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, pro);
+  stencil_tile2<T, CROSS, CENTER, RY>(op, bx, by, in, out, L, pro);
   if (!run) return;
   double vals[5] = {out.ss, out.ts, out.tt, out.rts, out.rtt};
   write_partials<5>(vals, sred, PB, nb, bid);
   if (bid == 0 && tid == 0) *Sout = st;
+}
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_iter_b(OpDev<T> op, const T* r, const T* v, const T* rt, T* t,
+                                                 const KState* Sin, KState* Sout, const double* PA, double* PB) {
+  iter_b_body<T, CROSS, CENTER, RY>(op, r, v, rt, t, Sin, Sout, PA, PB, blockIdx.x, blockIdx.y, block_id(),
+                                    nblocks());
 }
 
 // ---------------------------------------------------------------------------
@@ -1074,15 +1090,228 @@ extern "C" int pb_op2d_set_tile(pb_op2d* op, int rows_per_thread) {
 struct pb_batch {
   pb_ctx* ctx = nullptr;
   std::vector<pb_op2d*> ops;
+  void* mtab[2] = {nullptr, nullptr};      // device member tables (FP64, FP32) of the flattened launches
+  KState* dstates = nullptr;               // device gather target, one per member
+  std::vector<char> htab[2];               // host images of the member tables
+  std::map<long, hipGraphExec_t> flat_graphs;
+  int flat_sig = -1;                       // (ry, cross, center) the flat graphs were captured with
   std::vector<hipStream_t> streams;
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> joins;
   std::map<long, hipGraphExec_t> graphs;   // key: precision, chunk
   std::vector<int> ry_captured;            // member launch signatures the graphs were captured with
   KState* hst = nullptr;                   // pinned, one per member
+  // a user-facing batch is split into groups (each a batch of its own) whose
+  // working sets fit the Infinity Cache together; groups are solved in turn
+  std::vector<pb_batch*> groups;
+  std::vector<int> first;                  // index of each group's first member
 };
 
 namespace {
+
+// ---- flattened launches: one grid over the tiles of every member ----------
+// Block b belongs to the member whose tile range [begin, begin + nbx * nby)
+// contains it; within the member it is tile (local % nbx, local / nbx) with
+// block number ``local``, exactly the numbering of the member's own launch, so
+// partial sums and iterates are those of a single solve.  All members share
+// the template parameters (tile height, cross, center) of the launch.
+template <typename T>
+struct BMember {
+  OpDev<T> op;
+  const T* b;
+  T* r[2];
+  T* p[2];
+  T* v[2];
+  T* t;
+  T* rt;
+  double* x;
+  KState* S;
+  double* PA;
+  double* PB;
+  int begin, nbx, nby, pad;
+};
+
+template <typename T>
+__device__ __forceinline__ int find_member(const BMember<T>* mem, int nmem, int blk) {
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mem[mid].begin <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_init5_batch(const BMember<T>* mem, int nmem) {
+  const BMember<T> M = mem[find_member(mem, nmem, blockIdx.x)];
+  if (st_final(M.S[0].status)) return;   // parked member (uniform over the block)
+  const int local = blockIdx.x - M.begin;
+  OpDev<T> op = M.op;
+  InitIn<T> in{nullptr, 0};
+  InitOut<T> o{M.b, M.r[0], M.rt, M.p[0], M.v[0], M.t, M.x, nullptr, 0, 0.0, 0.0};
+  init_body<5, T, CROSS, CENTER, RY>(op, in, o, M.PB, local % M.nbx, local / M.nbx, local, M.nbx * M.nby);
+}
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_iter_a_batch(const BMember<T>* mem, int nmem, int k) {
+  const BMember<T> M = mem[find_member(mem, nmem, blockIdx.x)];
+  const int local = blockIdx.x - M.begin, po = (k - 1) & 1, pn = k & 1;
+  const IterPtrs<T> q{M.r[po], M.p[po], M.v[po], M.t, M.rt, M.r[pn], M.p[pn], M.v[pn], M.x};
+  iter_a_body<T, CROSS, CENTER, RY>(M.op, q, M.S, M.S + 1, M.PB, M.PA, local % M.nbx, local / M.nbx, local,
+                                    M.nbx * M.nby);
+}
+
+template <typename T, bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_iter_b_batch(const BMember<T>* mem, int nmem, int k) {
+  const BMember<T> M = mem[find_member(mem, nmem, blockIdx.x)];
+  const int local = blockIdx.x - M.begin, pn = k & 1;
+  iter_b_body<T, CROSS, CENTER, RY>(M.op, M.r[pn], M.v[pn], M.rt, M.t, M.S + 1, M.S, M.PA, M.PB, local % M.nbx,
+                                    local / M.nbx, local, M.nbx * M.nby);
+}
+
+template <typename T>
+__global__ void k_gather_states(const BMember<T>* mem, int nmem, int slot, KState* out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < nmem) out[m] = mem[m].S[slot];
+}
+
+// launch helper over the batch's uniform (cross, center, ry)
+#define PB_FLAT(KERNEL, T, cross, center, ry, grd, s, ...)                                        \
+  do {                                                                                           \
+    const dim3 _blk(TX, TYT);                                                                    \
+    if (ry == 2) {                                                                               \
+      if (cross && center) hipLaunchKernelGGL((KERNEL<T, true, true, 2>), grd, _blk, 0, s, __VA_ARGS__);   \
+      else if (cross) hipLaunchKernelGGL((KERNEL<T, true, false, 2>), grd, _blk, 0, s, __VA_ARGS__);       \
+      else if (center) hipLaunchKernelGGL((KERNEL<T, false, true, 2>), grd, _blk, 0, s, __VA_ARGS__);      \
+      else hipLaunchKernelGGL((KERNEL<T, false, false, 2>), grd, _blk, 0, s, __VA_ARGS__);                 \
+    } else if (ry == 8) {                                                                        \
+      if (cross && center) hipLaunchKernelGGL((KERNEL<T, true, true, 8>), grd, _blk, 0, s, __VA_ARGS__);   \
+      else if (cross) hipLaunchKernelGGL((KERNEL<T, true, false, 8>), grd, _blk, 0, s, __VA_ARGS__);       \
+      else if (center) hipLaunchKernelGGL((KERNEL<T, false, true, 8>), grd, _blk, 0, s, __VA_ARGS__);      \
+      else hipLaunchKernelGGL((KERNEL<T, false, false, 8>), grd, _blk, 0, s, __VA_ARGS__);                 \
+    } else {                                                                                     \
+      if (cross && center) hipLaunchKernelGGL((KERNEL<T, true, true, 4>), grd, _blk, 0, s, __VA_ARGS__);   \
+      else if (cross) hipLaunchKernelGGL((KERNEL<T, true, false, 4>), grd, _blk, 0, s, __VA_ARGS__);       \
+      else if (center) hipLaunchKernelGGL((KERNEL<T, false, true, 4>), grd, _blk, 0, s, __VA_ARGS__);      \
+      else hipLaunchKernelGGL((KERNEL<T, false, false, 4>), grd, _blk, 0, s, __VA_ARGS__);                 \
+    }                                                                                            \
+  } while (0)
+
+// the batch can use the flattened launches when every member has the same
+// tile height and operator kind; returns the (ry, cross, center) signature or -1
+int flat_signature(const pb_batch* bt) {
+  const pb_op2d* o0 = bt->ops[0];
+  for (const pb_op2d* op : bt->ops)
+    if (op->ry != o0->ry || op->cross != o0->cross || op->center != o0->center) return -1;
+  return o0->ry * 4 + (o0->cross ? 2 : 0) + (o0->center ? 1 : 0);
+}
+
+template <typename T>
+int krylov_batch_flat(pb_batch* bt, const std::vector<char>& active, const std::vector<const T*>& b,
+                      const std::vector<double>& tol, int maxiter, int legacy, int check_every, hipStream_t s,
+                      std::vector<KState>& out) {
+  const int P = std::is_same<T, float>::value ? 1 : 0;
+  const int M = (int)bt->ops.size();
+  const pb_op2d* o0 = bt->ops[0];
+  const int ry = o0->ry;
+  const bool cross = o0->cross, center = o0->center;
+  const int sig = flat_signature(bt);
+  if (sig != bt->flat_sig) {   // captured launches are specialised on the signature
+    for (auto& kv : bt->flat_graphs) (void)hipGraphExecDestroy(kv.second);
+    bt->flat_graphs.clear();
+    bt->flat_sig = sig;
+  }
+  // member table (device pointers are fixed for the life of the workspaces)
+  std::vector<BMember<T>> tab((size_t)M);
+  int total = 0;
+  for (int m = 0; m < M; ++m) {
+    pb_op2d* op = bt->ops[m];
+    Work<T>& w = work_of<T>(op);
+    const dim3 g = stencil_grid(op, ry);
+    BMember<T>& e = tab[(size_t)m];
+    e.op = opdev<T>(op);
+    e.b = b[m];
+    for (int q = 0; q < 2; ++q) {
+      e.r[q] = w.r[q];
+      e.p[q] = w.p[q];
+      e.v[q] = w.v[q];
+    }
+    e.t = w.t;
+    e.rt = w.rt;
+    e.x = w.x;
+    e.S = w.S;
+    e.PA = w.PA;
+    e.PB = w.PB;
+    e.begin = total;
+    e.nbx = (int)g.x;
+    e.nby = (int)g.y;
+    e.pad = 0;
+    total += (int)(g.x * g.y);
+  }
+  std::vector<char>& ht = bt->htab[P];
+  ht.assign((const char*)tab.data(), (const char*)tab.data() + sizeof(BMember<T>) * tab.size());
+  PB_HIP(bt->ctx, hipMemcpyAsync(bt->mtab[P], ht.data(), ht.size(), hipMemcpyHostToDevice, s));
+  for (int m = 0; m < M; ++m) {
+    KState init{};
+    init.tol = tol[(size_t)m];
+    init.maxiter = maxiter;
+    init.legacy = legacy;
+    init.phase = PH_INIT;
+    init.status = active[(size_t)m] ? ST_RUN : ST_CONV;
+    bt->hst[m] = init;
+    Work<T>& w = work_of<T>(bt->ops[m]);
+    PB_HIP(bt->ctx, hipMemcpyAsync(w.S, bt->hst + m, sizeof(KState), hipMemcpyHostToDevice, s));
+    if (!active[(size_t)m])
+      PB_HIP(bt->ctx, hipMemcpyAsync(w.S + 1, bt->hst + m, sizeof(KState), hipMemcpyHostToDevice, s));
+  }
+  const BMember<T>* dt = (const BMember<T>*)bt->mtab[P];
+  const dim3 grd((unsigned)total);
+  PB_FLAT(k_init5_batch, T, cross, center, ry, grd, s, dt, M);
+  PB_FLAT(k_iter_a_batch, T, cross, center, ry, grd, s, dt, M, 1);
+  PB_LAUNCH_CHECK(bt->ctx);
+  const int chunk = std::max(2, (check_every > 0 ? check_every : 32) & ~1);
+  const bool graphs = graphs_enabled() && s != nullptr;
+  auto issue_chunk = [&]() -> int {
+    for (int i = 0; i < chunk; ++i) {
+      PB_FLAT(k_iter_b_batch, T, cross, center, ry, grd, s, dt, M, 1 + i);
+      PB_FLAT(k_iter_a_batch, T, cross, center, ry, grd, s, dt, M, 2 + i);
+    }
+    return PB_OK;
+  };
+  hipGraphExec_t ex = nullptr;
+  if (graphs) {
+    const long key = ((long)chunk * 2 + P) * 65536 + total;
+    auto it = bt->flat_graphs.find(key);
+    if (it == bt->flat_graphs.end()) {
+      hipGraph_t gr = nullptr;
+      PB_HIP(bt->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      issue_chunk();
+      PB_HIP(bt->ctx, hipStreamEndCapture(s, &gr));
+      PB_HIP(bt->ctx, hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(gr);
+      it = bt->flat_graphs.emplace(key, ex).first;
+    }
+    ex = it->second;
+  }
+  long issued = 1;
+  for (;;) {
+    hipLaunchKernelGGL(k_gather_states<T>, dim3((M + 63) / 64), dim3(64), 0, s, dt, M, 1, bt->dstates);
+    PB_HIP(bt->ctx, hipMemcpyAsync(bt->hst, bt->dstates, sizeof(KState) * M, hipMemcpyDeviceToHost, s));
+    PB_HIP(bt->ctx, hipStreamSynchronize(s));
+    bool done = true;
+    for (int m = 0; m < M; ++m) done = done && st_final(bt->hst[m].status);
+    if (done) break;
+    if (issued > (long)maxiter + 2L * chunk + 2)
+      return pb_fail(bt->ctx, PB_ERR_STATE, "batch bicgstab: %ld iterations issued without final states", issued);
+    if (graphs) PB_HIP(bt->ctx, hipGraphLaunch(ex, s));
+    else issue_chunk();
+    PB_LAUNCH_CHECK(bt->ctx);
+    issued += chunk;
+  }
+  out.assign(bt->hst, bt->hst + M);
+  return PB_OK;
+}
 
 template <typename T>
 int batch_graph(pb_batch* bt, int chunk, hipStream_t s, hipGraphExec_t* out) {
@@ -1121,6 +1350,8 @@ template <typename T>
 int krylov_batch(pb_batch* bt, const std::vector<char>& active, const std::vector<const T*>& b,
                  const std::vector<double>& tol, int maxiter, int legacy, int check_every, hipStream_t s,
                  std::vector<KState>& out) {
+  // one launch over every member's tiles when they share the launch signature
+  if (flat_signature(bt) >= 0) return krylov_batch_flat<T>(bt, active, b, tol, maxiter, legacy, check_every, s, out);
   const size_t M = bt->ops.size();
   for (size_t m = 0; m < M; ++m) {
     pb_op2d* op = bt->ops[m];
@@ -1178,6 +1409,20 @@ int krylov_batch(pb_batch* bt, const std::vector<char>& active, const std::vecto
 
 }  // namespace
 
+namespace {
+int batch_group_new(pb_ctx* ctx, pb_op2d* const* ops, int count, pb_batch** out);
+int batch_group_solve(pb_batch* bt, const double* const* b, double* const* x, const pb_solve_opts* opts,
+                      pb_solve_info* infos, hipStream_t s);
+void batch_group_free(pb_batch* bt);
+
+// Bytes a member's FP64 solve keeps live: 9 Krylov vectors in T, the FP64
+// iterate and ~4 operator words per node.  Groups are filled up to half of
+// MI355X's 256 MB Infinity Cache so the lockstep iterations of a group stay
+// cache-resident (members bigger than that form a group alone).
+constexpr double kBatchCacheBudget = 128.0 * 1024 * 1024;
+double member_bytes(const pb_op2d* op) { return (double)op->n * 8.0 * 14.0; }
+}  // namespace
+
 extern "C" int pb_batch_create(pb_ctx* ctx, pb_op2d* const* ops, int count, pb_batch** out) {
   if (!ctx || !ops || !out || count < 1 || count > 4096)
     return pb_fail(ctx, PB_ERR_ARG, "pb_batch_create: bad argument");
@@ -1186,11 +1431,38 @@ extern "C" int pb_batch_create(pb_ctx* ctx, pb_op2d* const* ops, int count, pb_b
     for (int q = 0; q < m; ++q)
       if (ops[q] == ops[m]) return pb_fail(ctx, PB_ERR_ARG, "pb_batch_create: operator %d repeated", m);
   }
+  pb_batch* top = new pb_batch();
+  top->ctx = ctx;
+  top->ops.assign(ops, ops + count);
+  int beg = 0;
+  double bytes = 0.0;
+  for (int m = 0; m <= count; ++m) {
+    const double mb = m < count ? member_bytes(ops[m]) : 0.0;
+    if (m == count || (m > beg && bytes + mb > kBatchCacheBudget)) {
+      pb_batch* g = nullptr;
+      const int rc = batch_group_new(ctx, ops + beg, m - beg, &g);
+      if (rc) {
+        pb_batch_destroy(top);
+        return rc;
+      }
+      top->groups.push_back(g);
+      top->first.push_back(beg);
+      beg = m;
+      bytes = 0.0;
+    }
+    bytes += mb;
+  }
+  *out = top;
+  return PB_OK;
+}
+
+namespace {
+int batch_group_new(pb_ctx* ctx, pb_op2d* const* ops, int count, pb_batch** out) {
   pb_batch* bt = new pb_batch();
   bt->ctx = ctx;
   bt->ops.assign(ops, ops + count);
   auto fail = [&](const char* what) {
-    pb_batch_destroy(bt);
+    batch_group_free(bt);
     return pb_fail(ctx, PB_ERR_HIP, "pb_batch_create: %s failed", what);
   };
   for (int m = 0; m < count; ++m) {
@@ -1206,19 +1478,34 @@ extern "C" int pb_batch_create(pb_ctx* ctx, pb_op2d* const* ops, int count, pb_b
   void* hp = nullptr;
   if (hipHostMalloc(&hp, sizeof(KState) * count, hipHostMallocDefault) != hipSuccess) return fail("hipHostMalloc");
   bt->hst = (KState*)hp;
+  if (hipMalloc(&bt->mtab[0], sizeof(BMember<double>) * count) != hipSuccess ||
+      hipMalloc(&bt->mtab[1], sizeof(BMember<float>) * count) != hipSuccess ||
+      hipMalloc((void**)&bt->dstates, sizeof(KState) * count) != hipSuccess)
+    return fail("hipMalloc");
   *out = bt;
   return PB_OK;
 }
 
-extern "C" int pb_batch_destroy(pb_batch* bt) {
-  if (!bt) return PB_OK;
+void batch_group_free(pb_batch* bt) {
+  if (!bt) return;
   if (bt->ctx) (void)hipStreamSynchronize(bt->ctx->stream);
   for (hipStream_t st : bt->streams) (void)hipStreamSynchronize(st);
   for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : bt->flat_graphs) (void)hipGraphExecDestroy(kv.second);
+  for (void* p : bt->mtab)
+    if (p) (void)hipFree(p);
+  if (bt->dstates) (void)hipFree(bt->dstates);
   for (hipStream_t st : bt->streams) (void)hipStreamDestroy(st);
   for (hipEvent_t ev : bt->joins) (void)hipEventDestroy(ev);
   if (bt->fork) (void)hipEventDestroy(bt->fork);
   if (bt->hst) (void)hipHostFree(bt->hst);
+  delete bt;
+}
+}  // namespace
+
+extern "C" int pb_batch_destroy(pb_batch* bt) {
+  if (!bt) return PB_OK;
+  for (pb_batch* g : bt->groups) batch_group_free(g);
   delete bt;
   return PB_OK;
 }
@@ -1245,7 +1532,26 @@ extern "C" int pb_batch_solve(pb_batch* bt, const double* const* b, double* cons
     if (!b[m] || !x[m]) return pb_fail(ctx, PB_ERR_ARG, "pb_batch_solve: member %d has a null vector", (int)m);
     if (!bt->ops[m]->assembled) return pb_fail(ctx, PB_ERR_STATE, "pb_batch_solve: member %d not assembled", (int)m);
   }
+  if (opts->mode != PB_SOLVE_F64_FAST && opts->mode != PB_SOLVE_MIXED)
+    return pb_fail(ctx, PB_ERR_ARG, "pb_batch_solve: mode %d not batched", opts->mode);
   hipStream_t s = solve_stream(ctx);
+  int failed = 0;
+  for (size_t g = 0; g < bt->groups.size(); ++g) {
+    const int f = bt->first[g];
+    const int rc = batch_group_solve(bt->groups[g], b + f, x + f, opts, infos + f, s);
+    if (rc < 0) return rc;
+    failed += rc;
+  }
+  return failed;
+}
+
+namespace {
+// one group: members solved together (flattened launches when they share the
+// launch signature, else one graph branch per member)
+int batch_group_solve(pb_batch* bt, const double* const* b, double* const* x, const pb_solve_opts* opts,
+                      pb_solve_info* infos, hipStream_t s) {
+  pb_ctx* ctx = bt->ctx;
+  const size_t M = bt->ops.size();
   std::vector<KState> st;
   std::vector<char> active(M, 1);
   int failed = 0;
@@ -1352,3 +1658,4 @@ extern "C" int pb_batch_solve(pb_batch* bt, const double* const* b, double* cons
   }
   return failed;
 }
+}  // namespace

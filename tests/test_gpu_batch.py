@@ -115,3 +115,28 @@ def test_oracle_residual_per_member():
         bb = b.cpu().numpy().ravel()
         r = bb - olp.lap2d(x.cpu().numpy().ravel(), op_o)
         assert np.linalg.norm(r) <= 2e-9 * np.linalg.norm(bb), (np.linalg.norm(r) / np.linalg.norm(bb), info)
+
+
+@pytest.mark.parametrize("ry", [2, 4])
+def test_flattened_batch_equals_single_solves(ry):
+    """Uniform tile height: the batch runs one launch over every member's tiles
+    per phase; each member's iterates are still bitwise those of its own solve."""
+    import torch
+    from pybella_amd.batch import BatchSolver
+    print(f"case flattened batch ry={ry}", flush=True)
+    members = _members()
+    ops, bs = _systems(members)
+    for op in ops:
+        op.set_tile(ry)
+    singles = [op.solve(b, tol=1e-10, mode="f64fast") for op, b in zip(ops, bs)]
+    singles = [(x.clone(), info) for x, info in singles]
+    batch = BatchSolver(ops)
+    for _ in range(2):   # a second solve replays the captured graph
+        xs, infos = batch.solve(bs, tol=1e-10, mode="f64fast")
+        torch.cuda.synchronize()
+        for (xs1, i1), xb, ib in zip(singles, xs, infos):
+            assert ib["info"] == 0 and ib["iters"] == i1["iters"], (ib, i1)
+            assert torch.equal(xs1, xb)
+    xs, infos = batch.solve(bs, tol=1e-8, mode="mixed")
+    for op, b, info in zip(ops, bs, infos):
+        assert info["info"] == 0 and info["true_rel_resid"] <= 1e-8, info
