@@ -213,7 +213,8 @@ def bench_batch(args, sizes=((129, 65), (257, 129), (513, 257), (1025, 513))):
     # member's tiles per phase; the autotuner times the batch itself
     per_size = {v["rows_per_thread"] for v in tiles.values()}
     btable = {}
-    for ry in (2, 4, 8):
+    from pybella_amd.autotune import CANDIDATES
+    for ry in CANDIDATES:
         for op in ops:
             op.set_tile(ry)
         batch.solve(bs, tol=1e-8, mode="f64fast")

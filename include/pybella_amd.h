@@ -164,8 +164,9 @@ int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x,
                       const pb_solve_opts* opts, pb_solve_info* info);
 
 /* Tile height of the solver's stencil launches: 64 x (4 * rows_per_thread)
- * nodes per 256-thread block; rows_per_thread in {2, 4, 8} (default 4).  Used
- * by the tile autotuner.  No reference counterpart. */
+ * nodes per 256-thread block; rows_per_thread in {2, 3, 4, 5, 6, 8} (default
+ * 4).  The height sets the block count, and so how the grid fills the CUs'
+ * block slots; used by the tile autotuner.  No reference counterpart. */
 int pb_op2d_set_tile(pb_op2d* op, int rows_per_thread);
 
 /* Average device time (microseconds) of one launch of a solver kernel, timed

@@ -2,8 +2,10 @@
 
 The stencil launches tile the grid into 64 x (4 * rows_per_thread) node tiles
 (one 256-thread block each).  Taller tiles re-read fewer halo rows and emit
-fewer block partials; shorter tiles put more blocks in flight per CU.  The
-best height depends on the grid size and precision, so it is measured: each
+fewer block partials; shorter tiles put more blocks in flight per CU; and the
+block count decides whether the grid fits the CUs' block slots in one wave or
+leaves a short second wave (a tail).  The best height depends on the grid
+size, precision and register use, so it is measured: each
 candidate runs the caller's timing function on the live operator and the
 fastest wins.  Results are cached per (nx, ny, mode) for the process.
 """
@@ -13,7 +15,7 @@ import time
 
 _cache: dict = {}
 
-CANDIDATES = (2, 4, 8)
+CANDIDATES = (2, 3, 4, 5, 6, 8)
 
 
 def tune_tile(op, run, key=None, reps=1):

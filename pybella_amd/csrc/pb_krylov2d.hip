@@ -655,7 +655,8 @@ int alloc_work(pb_op2d* op) {
   return PB_OK;
 }
 
-// kernel dispatch over (CROSS, CENTER) and the tile height RY = op->ry (2, 4, 8)
+// kernel dispatch over (CROSS, CENTER) and the tile height RY = op->ry
+// (2, 3, 4, 5, 6, 8 rows per thread; see pb_op2d_set_tile)
 #define PB_DISPATCH_RY(KERNEL, T, RYV, grd, s, ...)                                                    \
   do {                                                                                                \
     const dim3 _blk(TX, TYT);                                                                         \
@@ -665,12 +666,22 @@ int alloc_work(pb_op2d* op) {
     else hipLaunchKernelGGL((KERNEL<T, false, false, RYV>), grd, _blk, 0, s, __VA_ARGS__);            \
   } while (0)
 
-#define PB_DISPATCH(KERNEL, T, op, grd, s, ...)                                 \
-  do {                                                                         \
-    const bool _c = (op)->cross, _h = (op)->center;                           \
-    if ((op)->ry == 8) PB_DISPATCH_RY(KERNEL, T, 8, grd, s, __VA_ARGS__);       \
-    else if ((op)->ry == 2) PB_DISPATCH_RY(KERNEL, T, 2, grd, s, __VA_ARGS__);  \
-    else PB_DISPATCH_RY(KERNEL, T, 4, grd, s, __VA_ARGS__);                     \
+#define PB_RY_SWITCH(RYVAL, M, ...)       \
+  do {                                   \
+    switch (RYVAL) {                     \
+      case 2: M(2, __VA_ARGS__); break;  \
+      case 3: M(3, __VA_ARGS__); break;  \
+      case 5: M(5, __VA_ARGS__); break;  \
+      case 6: M(6, __VA_ARGS__); break;  \
+      case 8: M(8, __VA_ARGS__); break;  \
+      default: M(4, __VA_ARGS__); break; \
+    }                                    \
+  } while (0)
+#define PB_DISPATCH_AT(RYV, KERNEL, T, grd, s, ...) PB_DISPATCH_RY(KERNEL, T, RYV, grd, s, __VA_ARGS__)
+#define PB_DISPATCH(KERNEL, T, op, grd, s, ...)                                  \
+  do {                                                                          \
+    const bool _c = (op)->cross, _h = (op)->center;                            \
+    PB_RY_SWITCH((op)->ry, PB_DISPATCH_AT, KERNEL, T, grd, s, __VA_ARGS__);     \
   } while (0)
 
 template <typename T>
@@ -711,18 +722,12 @@ template <typename T>
 void launch_l3(pb_op2d* op, Work<T>& w, int k, hipStream_t s) {
   const dim3 grd = stencil_grid(op, op->ry), blk(TX, TYT);
   const int pn = k & 1;
-  if (op->ry == 8)
-    hipLaunchKernelGGL((k_ex_update<T, 8>), grd, blk, 0, s, op->g.nx, op->g.ny, w.r[0], (const T*)w.p[pn],
-                       (const T*)w.v[pn], (const T*)w.t, (const T*)w.rt, w.x, (const KState*)(w.S + 1), w.S + 2,
-                       (const double*)w.PB, w.PC);
-  else if (op->ry == 2)
-    hipLaunchKernelGGL((k_ex_update<T, 2>), grd, blk, 0, s, op->g.nx, op->g.ny, w.r[0], (const T*)w.p[pn],
-                       (const T*)w.v[pn], (const T*)w.t, (const T*)w.rt, w.x, (const KState*)(w.S + 1), w.S + 2,
-                       (const double*)w.PB, w.PC);
-  else
-    hipLaunchKernelGGL((k_ex_update<T, 4>), grd, blk, 0, s, op->g.nx, op->g.ny, w.r[0], (const T*)w.p[pn],
-                       (const T*)w.v[pn], (const T*)w.t, (const T*)w.rt, w.x, (const KState*)(w.S + 1), w.S + 2,
-                       (const double*)w.PB, w.PC);
+#define PB_L3_AT(RYV, unused)                                                                                     \
+  hipLaunchKernelGGL((k_ex_update<T, RYV>), grd, blk, 0, s, op->g.nx, op->g.ny, w.r[0], (const T*)w.p[pn],           \
+                     (const T*)w.v[pn], (const T*)w.t, (const T*)w.rt, w.x, (const KState*)(w.S + 1), w.S + 2,       \
+                     (const double*)w.PB, w.PC)
+  PB_RY_SWITCH(op->ry, PB_L3_AT, 0);
+#undef PB_L3_AT
 }
 
 // chunk of c iterations starting at an odd k (the buffer parity pattern
@@ -834,6 +839,16 @@ int true_residual(pb_op2d* op, const double* b, const double* x, hipStream_t s, 
   return PB_OK;
 }
 
+// Iteration cap of one FP32 inner run of the mixed mode: FP32 BiCGStab can
+// stagnate for thousands of iterations depending on the rounding order, and
+// restarting it from the FP64 true residual (the next refinement step) breaks
+// the stagnation.  2 sqrt(n) is ~2.5x the iterations an inner run to 1e-3
+// takes on the stratified 1025x513 solve.
+int inner_cap(const pb_op2d* op, int maxiter) {
+  const int cap = std::max(100, (int)(2.0 * std::sqrt((double)op->n)));
+  return std::min(maxiter, cap);
+}
+
 int info_of(const KState& h) {
   switch (h.status) {
     case ST_CONV: return 0;
@@ -911,7 +926,7 @@ extern "C" int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x, const 
     Work<float>& w = op->w32;
     for (outer = 0; outer < max_outer && rn > target; ++outer) {
       hipLaunchKernelGGL((k_cast<double, float>), dim3(nb), dim3(256), 0, s, n, (const double*)op->rtrue, w.b);
-      rc = krylov_run<float>(op, w, w.b, nullptr, inner, opts->maxiter, 0, opts->check_every, s, &h);
+      rc = krylov_run<float>(op, w, w.b, nullptr, inner, inner_cap(op, opts->maxiter), 0, opts->check_every, s, &h);
       if (rc) return rc;
       iters += h.iter;
       last_info = info_of(h);
@@ -1070,8 +1085,8 @@ extern "C" int pb_op2d_time_kernel(pb_op2d* op, int kernel, int precision, int r
 
 extern "C" int pb_op2d_set_tile(pb_op2d* op, int rows_per_thread) {
   if (!op) return PB_ERR_ARG;
-  if (rows_per_thread != 2 && rows_per_thread != 4 && rows_per_thread != 8)
-    return pb_fail(op->ctx, PB_ERR_ARG, "pb_op2d_set_tile: rows per thread must be 2, 4 or 8");
+  if (rows_per_thread < 2 || rows_per_thread > 8 || rows_per_thread == 7)
+    return pb_fail(op->ctx, PB_ERR_ARG, "pb_op2d_set_tile: rows per thread must be 2, 3, 4, 5, 6 or 8");
   op->ry = rows_per_thread;
   return PB_OK;
 }
@@ -1177,26 +1192,16 @@ __global__ void k_gather_states(const BMember<T>* mem, int nmem, int slot, KStat
 }
 
 // launch helper over the batch's uniform (cross, center, ry)
-#define PB_FLAT(KERNEL, T, cross, center, ry, grd, s, ...)                                        \
-  do {                                                                                           \
-    const dim3 _blk(TX, TYT);                                                                    \
-    if (ry == 2) {                                                                               \
-      if (cross && center) hipLaunchKernelGGL((KERNEL<T, true, true, 2>), grd, _blk, 0, s, __VA_ARGS__);   \
-      else if (cross) hipLaunchKernelGGL((KERNEL<T, true, false, 2>), grd, _blk, 0, s, __VA_ARGS__);       \
-      else if (center) hipLaunchKernelGGL((KERNEL<T, false, true, 2>), grd, _blk, 0, s, __VA_ARGS__);      \
-      else hipLaunchKernelGGL((KERNEL<T, false, false, 2>), grd, _blk, 0, s, __VA_ARGS__);                 \
-    } else if (ry == 8) {                                                                        \
-      if (cross && center) hipLaunchKernelGGL((KERNEL<T, true, true, 8>), grd, _blk, 0, s, __VA_ARGS__);   \
-      else if (cross) hipLaunchKernelGGL((KERNEL<T, true, false, 8>), grd, _blk, 0, s, __VA_ARGS__);       \
-      else if (center) hipLaunchKernelGGL((KERNEL<T, false, true, 8>), grd, _blk, 0, s, __VA_ARGS__);      \
-      else hipLaunchKernelGGL((KERNEL<T, false, false, 8>), grd, _blk, 0, s, __VA_ARGS__);                 \
-    } else {                                                                                     \
-      if (cross && center) hipLaunchKernelGGL((KERNEL<T, true, true, 4>), grd, _blk, 0, s, __VA_ARGS__);   \
-      else if (cross) hipLaunchKernelGGL((KERNEL<T, true, false, 4>), grd, _blk, 0, s, __VA_ARGS__);       \
-      else if (center) hipLaunchKernelGGL((KERNEL<T, false, true, 4>), grd, _blk, 0, s, __VA_ARGS__);      \
-      else hipLaunchKernelGGL((KERNEL<T, false, false, 4>), grd, _blk, 0, s, __VA_ARGS__);                 \
-    }                                                                                            \
+#define PB_FLAT_AT(RYV, KERNEL, T, cross, center, grd, s, ...)                                          \
+  do {                                                                                                 \
+    const dim3 _blk(TX, TYT);                                                                          \
+    if (cross && center) hipLaunchKernelGGL((KERNEL<T, true, true, RYV>), grd, _blk, 0, s, __VA_ARGS__); \
+    else if (cross) hipLaunchKernelGGL((KERNEL<T, true, false, RYV>), grd, _blk, 0, s, __VA_ARGS__);     \
+    else if (center) hipLaunchKernelGGL((KERNEL<T, false, true, RYV>), grd, _blk, 0, s, __VA_ARGS__);    \
+    else hipLaunchKernelGGL((KERNEL<T, false, false, RYV>), grd, _blk, 0, s, __VA_ARGS__);               \
   } while (0)
+#define PB_FLAT(KERNEL, T, cross, center, ry, grd, s, ...) \
+  PB_RY_SWITCH(ry, PB_FLAT_AT, KERNEL, T, cross, center, grd, s, __VA_ARGS__)
 
 // the batch can use the flattened launches when every member has the same
 // tile height and operator kind; returns the (ry, cross, center) signature or -1
@@ -1607,7 +1612,9 @@ int batch_group_solve(pb_batch* bt, const double* const* b, double* const* x, co
                            (const double*)op->rtrue, op->w32.b);
     }
     if (!any) break;
-    rc = krylov_batch<float>(bt, active, bf, std::vector<double>(M, inner), opts->maxiter, 0, opts->check_every, s,
+    int cap = 0;
+    for (size_t m = 0; m < M; ++m) cap = std::max(cap, inner_cap(bt->ops[m], opts->maxiter));
+    rc = krylov_batch<float>(bt, active, bf, std::vector<double>(M, inner), cap, 0, opts->check_every, s,
                              st);
     if (rc) return rc;
     for (size_t m = 0; m < M; ++m) {

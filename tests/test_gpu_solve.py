@@ -97,7 +97,7 @@ def test_converged_f64fast(nx, ny):
     assert tr <= 1.05e-8
 
 
-@pytest.mark.parametrize("ry", [2, 8])
+@pytest.mark.parametrize("ry", [2, 3, 5, 6, 8])
 def test_tile_heights_agree(ry):
     """Every tile height gives the same first iterates (deterministic reductions)."""
     sp = _sp()
