@@ -35,6 +35,17 @@ __device__ __forceinline__ double eps2() { return 4.930380657631324e-32; }   // 
 
 __device__ __forceinline__ int block_id() { return blockIdx.x + blockIdx.y * gridDim.x; }
 __device__ __forceinline__ int nblocks() { return gridDim.x * gridDim.y; }
+
+// XCD-aware tile order.  Hardware block h is dispatched to XCD h mod 8 (8
+// XCDs, each with its own L2).  Giving XCD x the contiguous tile range
+// [start(x), start(x) + count(x)) keeps vertically adjacent tiles, which share
+// halo rows, in one L2.  Bijective on [0, n); partial records stay indexed by
+// the tile number, so sums and iterates do not depend on the mapping.
+__device__ __forceinline__ int xcd_tile(int h, int n) {
+  constexpr int NX = 8;
+  const int x = h % NX, k = h / NX, base = n / NX, rem = n % NX;
+  return x < rem ? x * (base + 1) + k : rem * (base + 1) + (x - rem) * base + k;
+}
 __device__ __forceinline__ int thread_id() { return threadIdx.x + threadIdx.y * blockDim.x; }
 
 // ---------------------------------------------------------------------------
@@ -93,11 +104,13 @@ __device__ __forceinline__ void init_body(OpDev<T>& op, InitIn<T>& in, InitOut<T
 
 template <typename T, bool CROSS, bool CENTER, int RY>
 __global__ __launch_bounds__(NTHR) void k_init5(OpDev<T> op, InitIn<T> in, InitOut<T> out, double* P) {
-  init_body<5, T, CROSS, CENTER, RY>(op, in, out, P, blockIdx.x, blockIdx.y, block_id(), nblocks());
+  const int tile = xcd_tile(block_id(), nblocks());
+  init_body<5, T, CROSS, CENTER, RY>(op, in, out, P, tile % gridDim.x, tile / gridDim.x, tile, nblocks());
 }
 template <typename T, bool CROSS, bool CENTER, int RY>
 __global__ __launch_bounds__(NTHR) void k_init2(OpDev<T> op, InitIn<T> in, InitOut<T> out, double* P) {
-  init_body<2, T, CROSS, CENTER, RY>(op, in, out, P, blockIdx.x, blockIdx.y, block_id(), nblocks());
+  const int tile = xcd_tile(block_id(), nblocks());
+  init_body<2, T, CROSS, CENTER, RY>(op, in, out, P, tile % gridDim.x, tile / gridDim.x, tile, nblocks());
 }
 
 // ---------------------------------------------------------------------------
@@ -315,7 +328,8 @@ __device__ __forceinline__ void iter_a_body(const OpDev<T>& op, const IterPtrs<T
 template <typename T, bool CROSS, bool CENTER, int RY>
 __global__ __launch_bounds__(NTHR) void k_iter_a(OpDev<T> op, IterPtrs<T> q, const KState* Sin, KState* Sout,
                                                  const double* PB, double* PA) {
-  iter_a_body<T, CROSS, CENTER, RY>(op, q, Sin, Sout, PB, PA, blockIdx.x, blockIdx.y, block_id(), nblocks());
+  const int tile = xcd_tile(block_id(), nblocks());
+  iter_a_body<T, CROSS, CENTER, RY>(op, q, Sin, Sout, PB, PA, tile % gridDim.x, tile / gridDim.x, tile, nblocks());
 }
 
 // ---------------------------------------------------------------------------
@@ -390,7 +404,8 @@ __device__ __forceinline__ void iter_b_body(const OpDev<T>& op, const T* r, cons
 template <typename T, bool CROSS, bool CENTER, int RY>
 __global__ __launch_bounds__(NTHR) void k_iter_b(OpDev<T> op, const T* r, const T* v, const T* rt, T* t,
                                                  const KState* Sin, KState* Sout, const double* PA, double* PB) {
-  iter_b_body<T, CROSS, CENTER, RY>(op, r, v, rt, t, Sin, Sout, PA, PB, blockIdx.x, blockIdx.y, block_id(),
+  const int tile = xcd_tile(block_id(), nblocks());
+  iter_b_body<T, CROSS, CENTER, RY>(op, r, v, rt, t, Sin, Sout, PA, PB, tile % gridDim.x, tile / gridDim.x, tile,
                                     nblocks());
 }
 
@@ -433,7 +448,7 @@ __global__ __launch_bounds__(NTHR) void k_ex_p(OpDev<T> op, const T* r, const T*
                                                const double* PC, double* PA) {
   __shared__ TileLds<T, RY> L;
   __shared__ double sred[8 * (NTHR / kWave)];
-  const int tid = thread_id(), bid = block_id(), nb = nblocks();
+  const int tid = thread_id(), nb = nblocks(), bid = xcd_tile(block_id(), nb);
   L1In<T> in{r, p_prev, v_prev, T(0), T(0)};
   L1Out<T> out{p_new, v_new, rt, 0.0};
   KState st;
@@ -457,7 +472,7 @@ __global__ __launch_bounds__(NTHR) void k_ex_p(OpDev<T> op, const T* r, const T*
     run = true;
     return true;
   };
-  stencil_tile2<T, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, pro);
+  stencil_tile2<T, CROSS, CENTER, RY>(op, bid % gridDim.x, bid / gridDim.x, in, out, L, pro);
   if (!run) return;
   double v[1] = {out.rv};
   write_partials<1>(v, sred, PA, nb, bid);
@@ -469,7 +484,8 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
                                                     const T* rt, double* x, const KState* Sin, KState* Sout,
                                                     const double* PB, double* PC) {
   __shared__ double sred[8 * (NTHR / kWave)];
-  const int tid = thread_id(), bid = block_id(), nb = nblocks();
+  const int tid = thread_id(), nb = nblocks(), bid = xcd_tile(block_id(), nb);
+  const int bx = bid % gridDim.x, by = bid / gridDim.x;
   constexpr int TY = TYT * RY;
   T er[RY], ep[RY], ev[RY], et[RY], ert[RY];
   double ex[RY];
@@ -477,7 +493,7 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
   long gg[RY];
 #pragma unroll
   for (int q = 0; q < RY; ++q) {
-    const int j = blockIdx.y * TY + threadIdx.y + TYT * q, i = blockIdx.x * TX + threadIdx.x;
+    const int j = by * TY + threadIdx.y + TYT * q, i = bx * TX + threadIdx.x;
     ok[q] = j < ny && i < nx;
     gg[q] = (long)j * nx + i;
     if (ok[q]) {
@@ -566,9 +582,10 @@ __global__ __launch_bounds__(NTHR) void k_true_res(OpDev<double> op, const doubl
   __shared__ double sred[8 * (NTHR / kWave)];
   TrIn in{x};
   TrOut out{b, r, 0.0};
-  stencil_tile2<double, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, [] { return true; });
+  const int tile = xcd_tile(block_id(), nblocks());
+  stencil_tile2<double, CROSS, CENTER, RY>(op, tile % gridDim.x, tile / gridDim.x, in, out, L, [] { return true; });
   double v[1] = {out.rr};
-  write_partials<1>(v, sred, PT, nblocks(), block_id());
+  write_partials<1>(v, sred, PT, nblocks(), tile);
 }
 
 __global__ __launch_bounds__(NTHR) void k_sum1(const double* P, int nb, double* out) {
@@ -1125,8 +1142,8 @@ struct pb_batch {
 namespace {
 
 // ---- flattened launches: one grid over the tiles of every member ----------
-// Block b belongs to the member whose tile range [begin, begin + nbx * nby)
-// contains it; within the member it is tile (local % nbx, local / nbx) with
+// Block b (after the XCD-aware remap of the whole grid) belongs to the member
+// whose tile range [begin, begin + nbx * nby) contains it; within the member it is tile (local % nbx, local / nbx) with
 // block number ``local``, exactly the numbering of the member's own launch, so
 // partial sums and iterates are those of a single solve.  All members share
 // the template parameters (tile height, cross, center) of the launch.
@@ -1159,9 +1176,10 @@ __device__ __forceinline__ int find_member(const BMember<T>* mem, int nmem, int 
 
 template <typename T, bool CROSS, bool CENTER, int RY>
 __global__ __launch_bounds__(NTHR) void k_init5_batch(const BMember<T>* mem, int nmem) {
-  const BMember<T> M = mem[find_member(mem, nmem, blockIdx.x)];
+  const int g = xcd_tile(blockIdx.x, gridDim.x);
+  const BMember<T> M = mem[find_member(mem, nmem, g)];
   if (st_final(M.S[0].status)) return;   // parked member (uniform over the block)
-  const int local = blockIdx.x - M.begin;
+  const int local = g - M.begin;
   OpDev<T> op = M.op;
   InitIn<T> in{nullptr, 0};
   InitOut<T> o{M.b, M.r[0], M.rt, M.p[0], M.v[0], M.t, M.x, nullptr, 0, 0.0, 0.0};
@@ -1170,8 +1188,9 @@ __global__ __launch_bounds__(NTHR) void k_init5_batch(const BMember<T>* mem, int
 
 template <typename T, bool CROSS, bool CENTER, int RY>
 __global__ __launch_bounds__(NTHR) void k_iter_a_batch(const BMember<T>* mem, int nmem, int k) {
-  const BMember<T> M = mem[find_member(mem, nmem, blockIdx.x)];
-  const int local = blockIdx.x - M.begin, po = (k - 1) & 1, pn = k & 1;
+  const int g = xcd_tile(blockIdx.x, gridDim.x);
+  const BMember<T> M = mem[find_member(mem, nmem, g)];
+  const int local = g - M.begin, po = (k - 1) & 1, pn = k & 1;
   const IterPtrs<T> q{M.r[po], M.p[po], M.v[po], M.t, M.rt, M.r[pn], M.p[pn], M.v[pn], M.x};
   iter_a_body<T, CROSS, CENTER, RY>(M.op, q, M.S, M.S + 1, M.PB, M.PA, local % M.nbx, local / M.nbx, local,
                                     M.nbx * M.nby);
@@ -1179,8 +1198,9 @@ __global__ __launch_bounds__(NTHR) void k_iter_a_batch(const BMember<T>* mem, in
 
 template <typename T, bool CROSS, bool CENTER, int RY>
 __global__ __launch_bounds__(NTHR) void k_iter_b_batch(const BMember<T>* mem, int nmem, int k) {
-  const BMember<T> M = mem[find_member(mem, nmem, blockIdx.x)];
-  const int local = blockIdx.x - M.begin, pn = k & 1;
+  const int g = xcd_tile(blockIdx.x, gridDim.x);
+  const BMember<T> M = mem[find_member(mem, nmem, g)];
+  const int local = g - M.begin, pn = k & 1;
   iter_b_body<T, CROSS, CENTER, RY>(M.op, M.r[pn], M.v[pn], M.rt, M.t, M.S + 1, M.S, M.PA, M.PB, local % M.nbx,
                                     local / M.nbx, local, M.nbx * M.nby);
 }
