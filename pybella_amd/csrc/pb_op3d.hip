@@ -74,8 +74,14 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (
   __shared__ T P[2][Y3 + 2][X3 + 2];
   __shared__ T F[3][Y3 + 1][X3 + 1];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tid3();
-  const int x0 = blockIdx.x * X3, y0 = blockIdx.y * Y3;
-  const int kb = blockIdx.z * Z3, ke = min(kb + Z3, o.nz);
+  // XCD-aware block order: hardware block h runs on XCD h mod 8; each XCD
+  // takes a contiguous band of (x, y, z-chunk) tiles so shared halo planes
+  // and rows stay in its L2
+  const int nbk = nblk3(), h = bid3(), xx = h % 8, kk8 = h / 8, base = nbk / 8, rem = nbk % 8;
+  const int lin = xx < rem ? xx * (base + 1) + kk8 : rem * (base + 1) + (xx - rem) * base + kk8;
+  const int bxx = lin % gridDim.x, byz = lin / gridDim.x, byy = byz % gridDim.y, bzz = byz / gridDim.y;
+  const int x0 = bxx * X3, y0 = byy * Y3;
+  const int kb = bzz * Z3, ke = min(kb + Z3, o.nz);
   const int nx = o.nx, ny = o.ny;
   const int i = x0 + tx, j = y0 + ty;
   const bool own = i < nx && j < ny;
