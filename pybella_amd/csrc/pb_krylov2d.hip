@@ -72,7 +72,8 @@ struct InitOut {
   int given;
   double rr, bb;
   __device__ Pre prefetch(long g) const { return b[g]; }
-  __device__ void operator()(long g, T y, T, const Pre& bg) {
+  template <class R>
+  __device__ void operator()(long g, T y, T, const Pre& bg, const R&) {
     const T rg = bg - y;
     r[g] = rg;
     rt[g] = rg;
@@ -243,7 +244,7 @@ struct AIn {
 template <typename T>
 struct AOut {
   struct Pre {
-    T r, p, v, t, rt;
+    T rt;
     double x;
   };
   IterPtrs<T> q;
@@ -251,12 +252,13 @@ struct AOut {
   double alpha_d, omega_d;
   int xupd;
   double rv;
-  __device__ Pre prefetch(long g) const { return Pre{q.r_prev[g], q.p_prev[g], q.v_prev[g], q.t[g], q.rt[g], q.x[g]}; }
-  __device__ void operator()(long g, T y, T pn, const Pre& e) {
-    const T s = e.r - alpha * e.v;
-    const T rn = s - omega * e.t;
+  __device__ Pre prefetch(long g) const { return Pre{q.rt[g], q.x[g]}; }
+  // e: this node's prefetched operands; w: its raw r, p, v, t (loaded for the tile)
+  __device__ void operator()(long g, T y, T pn, const Pre& e, const typename AIn<T>::Raw& w) {
+    const T s = w.r - alpha * w.v;
+    const T rn = s - omega * w.t;
     if (xupd) {
-      const double xg = e.x + alpha_d * (double)e.p;
+      const double xg = e.x + alpha_d * (double)w.p;
       q.x[g] = xg + omega_d * (double)s;
     }
     q.r_new[g] = rn;
@@ -352,7 +354,8 @@ struct BOut {
   const T* rt;
   double ss, ts, tt, rts, rtt;
   __device__ Pre prefetch(long g) const { return rt[g]; }
-  __device__ void operator()(long g, T y, T s, const Pre& rtg) {
+  template <class R>
+  __device__ void operator()(long g, T y, T s, const Pre& rtg, const R&) {
     t[g] = y;
     const double sd = (double)s, yd = (double)y, rd = (double)rtg;
     ss += sd * sd;
@@ -435,7 +438,8 @@ struct L1Out {
   const T* rt;
   double rv;
   __device__ Pre prefetch(long g) const { return rt[g]; }
-  __device__ void operator()(long g, T y, T pn, const Pre& rtg) {
+  template <class R>
+  __device__ void operator()(long g, T y, T pn, const Pre& rtg, const R&) {
     p_new[g] = pn;
     v_new[g] = y;
     rv += (double)rtg * (double)y;
@@ -568,7 +572,8 @@ struct TrOut {
   double* r;
   double rr;
   __device__ Pre prefetch(long g) const { return b[g]; }
-  __device__ void operator()(long g, double y, double, const Pre& bg) {
+  template <class R>
+  __device__ void operator()(long g, double y, double, const Pre& bg, const R&) {
     const double rg = bg - y;
     r[g] = rg;
     rr += rg * rg;

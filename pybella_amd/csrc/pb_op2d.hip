@@ -33,7 +33,8 @@ struct OutStore {
   struct Pre {};
   T* y;
   __device__ Pre prefetch(long) const { return Pre{}; }
-  __device__ void operator()(long g, T v, T, const Pre&) { y[g] = v; }
+  template <class R>
+  __device__ void operator()(long g, T v, T, const Pre&, const R&) { y[g] = v; }
 };
 
 template <typename T, bool CROSS, bool CENTER, int RY>

@@ -137,12 +137,17 @@ __device__ __forceinline__ void stencil_tile(const OpDev<T>& op, int bx, int by,
 // then ``prologue()`` runs (it may reduce the previous launch's partials and
 // set the scalars that In::combine and Out use; returning false ends the
 // block), so the latencies of all of them overlap.
+//
+// Row assignment: in pass q < RY a thread loads the raw values of its own
+// output row q (halo row 1 + ty + TYT q), so the epilogue receives them as
+// ``out(g, y, pc, pre, raw)`` without loading them again; one more pass loads
+// the top (ty == 0) and bottom (ty == 1) halo rows.
 template <typename T, bool CROSS, bool CENTER, int RY, class In, class Out, class Pro>
 __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by, In& in, Out& out,
                                               TileLds<T, RY>& L, Pro&& prologue) {
   constexpr int TY = TYT * RY;
   constexpr int HR = TY + 2;
-  constexpr int NPASS = (HR + TYT - 1) / TYT;
+  constexpr int NPASS = RY + 1;
   constexpr int NCELLS = (TY + 1) * (TX + 1);
   constexpr int NCP = (NCELLS + NTHR - 1) / NTHR;
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -157,12 +162,14 @@ __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by
   const bool edge_ok = tx < 2 && ie < nx;
   typename In::Raw raw[NPASS], eraw[NPASS];
   bool rok[NPASS];
+  int hrow[NPASS];
 #pragma unroll
   for (int q = 0; q < NPASS; ++q) {
-    const int ly = ty + TYT * q;
-    int j = j0 - 1 + ly;
+    const int hl = q < RY ? 1 + ty + TYT * q : (ty == 0 ? 0 : (ty == 1 ? HR - 1 : -1));
+    hrow[q] = hl;
+    int j = j0 - 1 + hl;
     j = (j < 0) ? ny - 2 : (j == ny ? 1 : j);
-    rok[q] = ly < HR && j < ny;
+    rok[q] = hl >= 0 && j < ny;
     if (rok[q] && col_ok) raw[q] = in.load((long)j * nx + ic);
     if (rok[q] && edge_ok) eraw[q] = in.load((long)j * nx + ie);
   }
@@ -200,10 +207,10 @@ __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by
   if (!prologue()) return;
 #pragma unroll
   for (int q = 0; q < NPASS; ++q) {
-    const int ly = ty + TYT * q;
-    if (ly < HR) {
-      L.sp[ly * (TX + 2) + tx + 1] = (rok[q] && col_ok) ? in.combine(raw[q]) : T(0);
-      if (tx < 2) L.sp[ly * (TX + 2) + (tx == 0 ? 0 : TX + 1)] = (rok[q] && edge_ok) ? in.combine(eraw[q]) : T(0);
+    const int hl = hrow[q];
+    if (hl >= 0) {
+      L.sp[hl * (TX + 2) + tx + 1] = (rok[q] && col_ok) ? in.combine(raw[q]) : T(0);
+      if (tx < 2) L.sp[hl * (TX + 2) + (tx == 0 ? 0 : TX + 1)] = (rok[q] && edge_ok) ? in.combine(eraw[q]) : T(0);
     }
   }
   __syncthreads();
@@ -237,7 +244,7 @@ __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by
               ((L.sfy[c1] + L.sfy[c1 + 1]) - (L.sfy[c0] + L.sfy[c0 + 1]));
       const T pc = L.sp[(ly + 1) * (TX + 2) + lx + 1];
       if (CENTER) acc += hv[r] * pc;
-      out(g, dv[r] * acc, pc, pre[r]);
+      out(g, dv[r] * acc, pc, pre[r], raw[r]);
     }
   }
 }
