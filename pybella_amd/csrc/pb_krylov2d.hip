@@ -22,6 +22,7 @@
 // launch's block partials in block order (deterministic, no grid fence), and
 // block 0 stores the new state in the other half of a double buffer.  x is
 // kept in FP64 also when the Krylov vectors are FP32.
+#include <cstddef>
 #include <cstdlib>
 #include <type_traits>
 
@@ -276,11 +277,12 @@ __device__ __forceinline__ void iter_a_body(const OpDev<T>& op, const IterPtrs<T
   const int tid = thread_id();
   AIn<T> in{q.r_prev, q.p_prev, q.v_prev, q.t, T(0), T(0), T(0)};
   AOut<T> out{q, T(0), T(0), 0.0, 0.0, 0, 0.0};
-  KState st;
+  KState st = *Sin;
+  EarlyPartials<5> early;
+  early.load(PB, nb);
   auto pro = [&]() -> bool {
-    st = *Sin;
     double tot[5];
-    sum_partials<5>(PB, nb, tot, sred);
+    sum_partials<5>(early, PB, nb, tot, sred);
     if (st_final(st.status)) {
       if (bid == 0 && tid == 0) *Sout = st;
       return false;
@@ -375,25 +377,33 @@ __device__ __forceinline__ void iter_b_body(const OpDev<T>& op, const T* r, cons
   const int tid = thread_id();
   BIn<T> in{r, v, T(0)};
   BOut<T> out{t, rt, 0.0, 0.0, 0.0, 0.0, 0.0};
-  KState st;
+  // Only rho and the status are needed up front; the state record for the
+  // next launch is rebuilt from *Sin by the one thread that writes it, so no
+  // thread holds the whole record across the tile.
+  const double rho = Sin->rho;
+  const int status = Sin->status;
+  EarlyPartials<1> early;
+  early.load(PA, nb);
+  const bool writer = bid == 0 && tid == 0;
+  double alpha = 0.0;
   bool run = false;
   auto pro = [&]() -> bool {
-    st = *Sin;
     double tot[1];
-    sum_partials<1>(PA, nb, tot, sred);
-    if (st.status != ST_RUN) {
-      if (bid == 0 && tid == 0) *Sout = st;
+    sum_partials<1>(early, PA, nb, tot, sred);
+    if (status != ST_RUN) {
+      if (writer) copy_state(Sout, Sin);
       return false;
     }
-    const double a = st.rho / tot[0];
-    st.phase = PH_B;
-    if (tot[0] == 0.0 || !isfinite(a)) {
-      st.status = ST_BRK_OMEGA;
-      if (bid == 0 && tid == 0) *Sout = st;
+    alpha = rho / tot[0];
+    if (tot[0] == 0.0 || !isfinite(alpha)) {
+      if (writer) {
+        copy_state(Sout, Sin);
+        Sout->phase = PH_B;
+        Sout->status = ST_BRK_OMEGA;
+      }
       return false;
     }
-    st.alpha = a;
-    in.alpha = (T)a;
+    in.alpha = (T)alpha;
     run = true;
     return true;
   };
@@ -401,7 +411,11 @@ __device__ __forceinline__ void iter_b_body(const OpDev<T>& op, const T* r, cons
   if (!run) return;
   double vals[5] = {out.ss, out.ts, out.tt, out.rts, out.rtt};
   write_partials<5>(vals, sred, PB, nb, bid);
-  if (bid == 0 && tid == 0) *Sout = st;
+  if (writer) {
+    copy_state(Sout, Sin);
+    Sout->phase = PH_B;
+    Sout->alpha = alpha;
+  }
 }
 
 template <typename T, bool CROSS, bool CENTER, int RY>
@@ -455,12 +469,13 @@ __global__ __launch_bounds__(NTHR) void k_ex_p(OpDev<T> op, const T* r, const T*
   const int tid = thread_id(), nb = nblocks(), bid = xcd_tile(block_id(), nb);
   L1In<T> in{r, p_prev, v_prev, T(0), T(0)};
   L1Out<T> out{p_new, v_new, rt, 0.0};
-  KState st;
+  KState st = *Sin;
+  EarlyPartials<2> early;
+  early.load(PC, nb);
   bool run = false;
   auto pro = [&]() -> bool {
-    st = *Sin;
     double tot[2];
-    sum_partials<2>(PC, nb, tot, sred);
+    sum_partials<2>(early, PC, nb, tot, sred);
     if (st_final(st.status)) {
       if (bid == 0 && tid == 0) *Sout = st;
       return false;
@@ -491,6 +506,9 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
   const int tid = thread_id(), nb = nblocks(), bid = xcd_tile(block_id(), nb);
   const int bx = bid % gridDim.x, by = bid / gridDim.x;
   constexpr int TY = TYT * RY;
+  KState st = *Sin;
+  EarlyPartials<5> early;
+  early.load(PB, nb);
   T er[RY], ep[RY], ev[RY], et[RY], ert[RY];
   double ex[RY];
   bool ok[RY];
@@ -509,9 +527,8 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
       ex[q] = x[gg[q]];
     }
   }
-  KState st = *Sin;
   double tot[5];
-  sum_partials<5>(PB, nb, tot, sred);
+  sum_partials<5>(early, PB, nb, tot, sred);
   if (st.status != ST_RUN) {
     if (bid == 0 && tid == 0) *Sout = st;
     return;
@@ -1168,6 +1185,41 @@ struct BMember {
   int begin, nbx, nby, pad;
 };
 
+// Device view of a BMember with every pointer typed in the global address
+// space (same layout).  Pointers read from the member table are otherwise
+// generic, and generic (flat) loads also count against the scalar-load
+// counter, so each wait for a scalar load would wait for the whole tile.
+#define PB_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+struct BMemberG {
+  PB_GLOBAL const T *wxx, *wyy, *wyx, *wxy, *hc, *dinv;
+  int nx, ny;
+  PB_GLOBAL const T* b;
+  PB_GLOBAL T* r[2];
+  PB_GLOBAL T* p[2];
+  PB_GLOBAL T* v[2];
+  PB_GLOBAL T* t;
+  PB_GLOBAL T* rt;
+  PB_GLOBAL double* x;
+  PB_GLOBAL KState* S;
+  PB_GLOBAL double* PA;
+  PB_GLOBAL double* PB;
+  int begin, nbx, nby, pad;
+};
+static_assert(sizeof(BMemberG<double>) == sizeof(BMember<double>), "member view layout");
+static_assert(sizeof(BMemberG<float>) == sizeof(BMember<float>), "member view layout");
+static_assert(offsetof(BMemberG<double>, begin) == offsetof(BMember<double>, begin), "member view layout");
+
+template <typename P>
+__device__ __forceinline__ P* gen(PB_GLOBAL P* p) {
+  return (P*)p;
+}
+
+template <typename T>
+__device__ __forceinline__ OpDev<T> member_op(const BMemberG<T>& G) {
+  return OpDev<T>{gen(G.wxx), gen(G.wyy), gen(G.wyx), gen(G.wxy), gen(G.hc), gen(G.dinv), G.nx, G.ny};
+}
+
 template <typename T>
 __device__ __forceinline__ int find_member(const BMember<T>* mem, int nmem, int blk) {
   int lo = 0, hi = nmem - 1;
@@ -1180,34 +1232,36 @@ __device__ __forceinline__ int find_member(const BMember<T>* mem, int nmem, int 
 }
 
 template <typename T, bool CROSS, bool CENTER, int RY>
-__global__ __launch_bounds__(NTHR) void k_init5_batch(const BMember<T>* mem, int nmem) {
+__global__ __launch_bounds__(NTHR) void k_init5_batch(const BMember<T>* __restrict__ mem, int nmem) {
   const int g = xcd_tile(blockIdx.x, gridDim.x);
-  const BMember<T> M = mem[find_member(mem, nmem, g)];
+  const BMemberG<T>& M = reinterpret_cast<const BMemberG<T>&>(mem[find_member(mem, nmem, g)]);
   if (st_final(M.S[0].status)) return;   // parked member (uniform over the block)
   const int local = g - M.begin;
-  OpDev<T> op = M.op;
+  OpDev<T> op = member_op(M);
   InitIn<T> in{nullptr, 0};
-  InitOut<T> o{M.b, M.r[0], M.rt, M.p[0], M.v[0], M.t, M.x, nullptr, 0, 0.0, 0.0};
-  init_body<5, T, CROSS, CENTER, RY>(op, in, o, M.PB, local % M.nbx, local / M.nbx, local, M.nbx * M.nby);
+  InitOut<T> o{gen(M.b), gen(M.r[0]), gen(M.rt), gen(M.p[0]), gen(M.v[0]), gen(M.t), gen(M.x), nullptr, 0, 0.0, 0.0};
+  init_body<5, T, CROSS, CENTER, RY>(op, in, o, gen(M.PB), local % M.nbx, local / M.nbx, local, M.nbx * M.nby);
 }
 
 template <typename T, bool CROSS, bool CENTER, int RY>
-__global__ __launch_bounds__(NTHR) void k_iter_a_batch(const BMember<T>* mem, int nmem, int k) {
+__global__ __launch_bounds__(NTHR) void k_iter_a_batch(const BMember<T>* __restrict__ mem, int nmem, int k) {
   const int g = xcd_tile(blockIdx.x, gridDim.x);
-  const BMember<T> M = mem[find_member(mem, nmem, g)];
+  const BMemberG<T>& M = reinterpret_cast<const BMemberG<T>&>(mem[find_member(mem, nmem, g)]);
   const int local = g - M.begin, po = (k - 1) & 1, pn = k & 1;
-  const IterPtrs<T> q{M.r[po], M.p[po], M.v[po], M.t, M.rt, M.r[pn], M.p[pn], M.v[pn], M.x};
-  iter_a_body<T, CROSS, CENTER, RY>(M.op, q, M.S, M.S + 1, M.PB, M.PA, local % M.nbx, local / M.nbx, local,
-                                    M.nbx * M.nby);
+  const IterPtrs<T> q{gen(M.r[po]), gen(M.p[po]), gen(M.v[po]), gen(M.t), gen(M.rt),
+                      gen(M.r[pn]), gen(M.p[pn]), gen(M.v[pn]), gen(M.x)};
+  iter_a_body<T, CROSS, CENTER, RY>(member_op(M), q, gen(M.S), gen(M.S) + 1, gen(M.PB), gen(M.PA), local % M.nbx,
+                                    local / M.nbx, local, M.nbx * M.nby);
 }
 
 template <typename T, bool CROSS, bool CENTER, int RY>
-__global__ __launch_bounds__(NTHR) void k_iter_b_batch(const BMember<T>* mem, int nmem, int k) {
+__global__ __launch_bounds__(NTHR) void k_iter_b_batch(const BMember<T>* __restrict__ mem, int nmem, int k) {
   const int g = xcd_tile(blockIdx.x, gridDim.x);
-  const BMember<T> M = mem[find_member(mem, nmem, g)];
+  const BMemberG<T>& M = reinterpret_cast<const BMemberG<T>&>(mem[find_member(mem, nmem, g)]);
   const int local = g - M.begin, pn = k & 1;
-  iter_b_body<T, CROSS, CENTER, RY>(M.op, M.r[pn], M.v[pn], M.rt, M.t, M.S + 1, M.S, M.PA, M.PB, local % M.nbx,
-                                    local / M.nbx, local, M.nbx * M.nby);
+  iter_b_body<T, CROSS, CENTER, RY>(member_op(M), gen(M.r[pn]), gen(M.v[pn]), gen(M.rt), gen(M.t), gen(M.S) + 1,
+                                    gen(M.S), gen(M.PA), gen(M.PB), local % M.nbx, local / M.nbx, local,
+                                    M.nbx * M.nby);
 }
 
 template <typename T>

@@ -249,15 +249,45 @@ __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by
   }
 }
 
+// The first two partial records of this thread (b = tid, tid + NTHR), loaded
+// before the tile's own loads so that their latency overlaps them: vector
+// loads complete in issue order, so records requested after the tile would
+// wait for the whole tile to arrive first.
+template <int NV>
+struct EarlyPartials {
+  double v[2][NV];
+  // Unconditional loads of a clamped record (nb >= 1): a guarded load would
+  // share its branch with the guarded add of sum_partials, and the add would
+  // then wait for the load right here.
+  __device__ __forceinline__ void load(const double* P, int nb) {
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int b = min(tid + h * NTHR, nb - 1);
+#pragma unroll
+      for (int k = 0; k < NV; ++k) v[h][k] = P[(long)b * NV + k];
+    }
+  }
+};
+
 // Sum ``nb`` block partials of NV values (record layout [b * NV + k]) in a
 // fixed order; the result is valid in every thread.  sred: NV * (NTHR/64).
+// ``early`` holds records tid and tid + NTHR (EarlyPartials::load on the same
+// P and nb); thread tid adds its records b = tid, tid + NTHR, ... in order.
 template <int NV>
-__device__ __forceinline__ void sum_partials(const double* P, int nb, double (&tot)[NV], double* sred) {
+__device__ __forceinline__ void sum_partials(const EarlyPartials<NV>& early, const double* P, int nb,
+                                             double (&tot)[NV], double* sred) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int nthr = blockDim.x * blockDim.y;
 #pragma unroll
   for (int k = 0; k < NV; ++k) tot[k] = 0.0;
-  for (int b = tid; b < nb; b += nthr) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (tid + h * NTHR < nb) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) tot[k] += early.v[h][k];
+    }
+  }
+  for (int b = tid + 2 * NTHR; b < nb; b += NTHR) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) tot[k] += P[(long)b * NV + k];
   }
@@ -271,6 +301,13 @@ __device__ __forceinline__ void sum_partials(const double* P, int nb, double (&t
 #pragma unroll
   for (int k = 0; k < NV; ++k) tot[k] = bc[k];
   __syncthreads();
+}
+
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* P, int nb, double (&tot)[NV], double* sred) {
+  EarlyPartials<NV> early;
+  early.load(P, nb);
+  sum_partials<NV>(early, P, nb, tot, sred);
 }
 
 // Block sum of NV thread values, written as this block's record (plain stores).
@@ -294,6 +331,16 @@ struct KState {
   double atol, bnrm2, resid, tol, rnorm0;
   int iter, maxiter, status, half, legacy, phase, x_upd;
 };
+
+// Word-wise copy of a state record (one thread), without a whole-record
+// temporary; the caller then overwrites the fields it changes.
+__device__ __forceinline__ void copy_state(KState* dst, const KState* src) {
+  static_assert(sizeof(KState) % 8 == 0, "KState is copied in 8-byte words");
+  const long long* s = reinterpret_cast<const long long*>(src);
+  long long* d = reinterpret_cast<long long*>(dst);
+#pragma unroll
+  for (int w = 0; w < (int)(sizeof(KState) / 8); ++w) d[w] = s[w];
+}
 
 enum {
   ST_RUN = 0,
