@@ -39,9 +39,28 @@ inline int pb_fail(pb_ctx* ctx, int code, const char* fmt, ...) {
 
 constexpr int kWave = 64;
 
+// v of the lane selected by the DPP control CTRL (both 32-bit halves moved).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wave (every lane active), valid in lane 0.  The
+// order is the xor butterfly's: lane 0 adds the partial sum at xor distance
+// 32, 16, 8, 4, 2, 1 in turn.  The two cross-row steps go through
+// ds_bpermute; the four row-local ones are DPP moves.  After the distance-8
+// step lanes i and i^8 hold equal sums, so the row rotation by 4 delivers
+// the distance-4 partner's value whichever way it rotates.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  v += __shfl_xor(v, 32, kWave);
+  v += __shfl_xor(v, 16, kWave);
+  v += dpp_f64<0x128>(v);   // row_ror 8: lane i ^ 8
+  v += dpp_f64<0x124>(v);   // row_ror 4: lane i ^ 4 or i ^ 12 (equal sums)
+  v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]: lane i ^ 2
+  v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]: lane i ^ 1
   return v;
 }
 

@@ -260,11 +260,11 @@ struct AOut {
     const T rn = s - omega * w.t;
     if (xupd) {
       const double xg = e.x + alpha_d * (double)w.p;
-      q.x[g] = xg + omega_d * (double)s;
+      __builtin_nontemporal_store(xg + omega_d * (double)s, q.x + g);
     }
-    q.r_new[g] = rn;
-    q.p_new[g] = pn;
-    q.v_new[g] = y;
+    __builtin_nontemporal_store(rn, q.r_new + g);
+    __builtin_nontemporal_store(pn, q.p_new + g);
+    __builtin_nontemporal_store(y, q.v_new + g);
     rv += (double)e.rt * (double)y;
   }
 };
@@ -358,7 +358,7 @@ struct BOut {
   __device__ Pre prefetch(long g) const { return rt[g]; }
   template <class R>
   __device__ void operator()(long g, T y, T s, const Pre& rtg, const R&) {
-    t[g] = y;
+    __builtin_nontemporal_store(y, t + g);
     const double sd = (double)s, yd = (double)y, rd = (double)rtg;
     ss += sd * sd;
     ts += yd * sd;
@@ -570,6 +570,7 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
     }
   }
   double vals[2] = {rtr, rr};
+  __syncthreads();   // every wave has read the prologue's sums out of sred
   write_partials<2>(vals, sred, PC, nb, bid);
   if (bid == 0 && tid == 0) *Sout = st;
 }

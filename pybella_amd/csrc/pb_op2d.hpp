@@ -141,7 +141,9 @@ __device__ __forceinline__ void stencil_tile(const OpDev<T>& op, int bx, int by,
 // Row assignment: in pass q < RY a thread loads the raw values of its own
 // output row q (halo row 1 + ty + TYT q), so the epilogue receives them as
 // ``out(g, y, pc, pre, raw)`` without loading them again; one more pass loads
-// the top (ty == 0) and bottom (ty == 1) halo rows.
+// the top (ty == 0) and bottom (ty == 1) halo rows.  The 2 x HR edge-column
+// values are loaded by the first 2 HR threads, one each, so the other lanes
+// spend neither load instructions nor registers on them.
 template <typename T, bool CROSS, bool CENTER, int RY, class In, class Out, class Pro>
 __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by, In& in, Out& out,
                                               TileLds<T, RY>& L, Pro&& prologue) {
@@ -157,10 +159,7 @@ __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by
   int ic = i0 + tx;
   ic = (ic == nx) ? 1 : ic;
   const bool col_ok = ic < nx;
-  int ie = (tx == 0) ? i0 - 1 : i0 + TX;
-  ie = (ie < 0) ? nx - 2 : (ie == nx ? 1 : ie);
-  const bool edge_ok = tx < 2 && ie < nx;
-  typename In::Raw raw[NPASS], eraw[NPASS];
+  typename In::Raw raw[NPASS];
   bool rok[NPASS];
   int hrow[NPASS];
 #pragma unroll
@@ -171,7 +170,21 @@ __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by
     j = (j < 0) ? ny - 2 : (j == ny ? 1 : j);
     rok[q] = hl >= 0 && j < ny;
     if (rok[q] && col_ok) raw[q] = in.load((long)j * nx + ic);
-    if (rok[q] && edge_ok) eraw[q] = in.load((long)j * nx + ie);
+  }
+  // The two edge columns of all HR halo rows: thread tid < 2 HR loads the
+  // value of halo row tid / 2, left (even tid) or right (odd tid) edge.
+  static_assert(2 * HR <= NTHR, "edge loads need 2 HR threads");
+  const bool edge_thr = tid < 2 * HR;
+  const int ehl = tid >> 1, eside = tid & 1;
+  bool eok = false;
+  typename In::Raw eraw;
+  {
+    int ie = eside ? i0 + TX : i0 - 1;
+    ie = (ie < 0) ? nx - 2 : (ie == nx ? 1 : ie);
+    int j = j0 - 1 + ehl;
+    j = (j < 0) ? ny - 2 : (j == ny ? 1 : j);
+    eok = edge_thr && ie < nx && j < ny;
+    if (eok) eraw = in.load((long)j * nx + ie);
   }
   T wa[NCP], wb[NCP], wc[NCP], wd[NCP];
 #pragma unroll
@@ -210,9 +223,9 @@ __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by
     const int hl = hrow[q];
     if (hl >= 0) {
       L.sp[hl * (TX + 2) + tx + 1] = (rok[q] && col_ok) ? in.combine(raw[q]) : T(0);
-      if (tx < 2) L.sp[hl * (TX + 2) + (tx == 0 ? 0 : TX + 1)] = (rok[q] && edge_ok) ? in.combine(eraw[q]) : T(0);
     }
   }
+  if (edge_thr) L.sp[ehl * (TX + 2) + (eside ? TX + 1 : 0)] = eok ? in.combine(eraw) : T(0);
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < NCP; ++c) {
@@ -291,16 +304,24 @@ __device__ __forceinline__ void sum_partials(const EarlyPartials<NV>& early, con
 #pragma unroll
     for (int k = 0; k < NV; ++k) tot[k] += P[(long)b * NV + k];
   }
-  block_sum<NV>(tot, sred);
-  __shared__ double bc[8];
-  if (tid == 0) {
+  // wave sums into sred, then every thread adds them in wave order (the same
+  // order in every thread and block); sred is next written after the tile's
+  // own barriers.
+  constexpr int NW = NTHR / kWave;
+  const int lane = tid & (kWave - 1), wid = tid / kWave;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) bc[k] = tot[k];
+  for (int k = 0; k < NV; ++k) {
+    const double s = wave_sum(tot[k]);
+    if (lane == 0) sred[k * NW + wid] = s;
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < NV; ++k) tot[k] = bc[k];
-  __syncthreads();
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += sred[k * NW + w];
+    tot[k] = s;
+  }
 }
 
 template <int NV>

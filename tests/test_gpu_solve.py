@@ -114,6 +114,26 @@ def test_tile_heights_agree(ry):
     assert np.max(np.abs(x.ravel() - ref.x)) / np.max(np.abs(ref.x)) < 1e-10
 
 
+@pytest.mark.parametrize("mode", ["f64", "f64fast"])
+def test_many_blocks_first_iterates(mode):
+    """1025 x 513 at tile height 2 is 17 x 65 = 1105 blocks, more than the two
+    partial records per thread that each block loads ahead of its tile: the
+    reductions then also read the remaining records in their tail loop."""
+    sp = _sp()
+    print(f"case 1105 blocks {mode}", flush=True)
+    (Sol, mpv, elem, node, ud, th, dt), op_o, rhs, b = system(1025, 513)
+    ref = obs.bicgstab(lambda v: olp.lap2d(v, op_o), b, tol=1e-30, maxiter=9)
+    op = sp.operator_for(node, ud)
+    op.set_tile(2)
+    try:
+        x, info = sp.solve_isolated(Sol, mpv, elem, node, ud, th, dt, rhs, mode=mode, tol=1e-30, maxiter=9,
+                                    check_every=2)
+    finally:
+        op.set_tile(4)
+    assert info["iters"] == 9
+    assert np.max(np.abs(x.ravel() - ref.x)) / np.max(np.abs(ref.x)) < 1e-10
+
+
 def test_tight_solution_parity():
     sp = _sp()
     print("case tight parity 129x65", flush=True)
