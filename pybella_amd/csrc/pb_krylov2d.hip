@@ -506,64 +506,65 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
   const int tid = thread_id(), nb = nblocks(), bid = xcd_tile(block_id(), nb);
   const int bx = bid % gridDim.x, by = bid / gridDim.x;
   constexpr int TY = TYT * RY;
-  KState st = *Sin;
+  // Only the status, atol and alpha are read up front; the state record for
+  // the next launch is rebuilt from *Sin by the one thread that writes it.
+  const int status = Sin->status;
+  const double atol = Sin->atol, alpha_d = Sin->alpha;
+  const bool writer = bid == 0 && tid == 0;
   EarlyPartials<5> early;
   early.load(PB, nb);
   T er[RY], ep[RY], ev[RY], et[RY], ert[RY];
   double ex[RY];
   bool ok[RY];
   long gg[RY];
+  // unconditional loads (threads past the grid read node 0, unused): one
+  // straight run of loads with no branch between the rows
 #pragma unroll
   for (int q = 0; q < RY; ++q) {
     const int j = by * TY + threadIdx.y + TYT * q, i = bx * TX + threadIdx.x;
     ok[q] = j < ny && i < nx;
-    gg[q] = (long)j * nx + i;
-    if (ok[q]) {
-      er[q] = r[gg[q]];
-      ep[q] = p[gg[q]];
-      ev[q] = v[gg[q]];
-      et[q] = t[gg[q]];
-      ert[q] = rt[gg[q]];
-      ex[q] = x[gg[q]];
-    }
+    const long g = ok[q] ? (long)j * nx + i : 0;
+    gg[q] = g;
+    er[q] = r[g];
+    ep[q] = p[g];
+    ev[q] = v[g];
+    et[q] = t[g];
+    ert[q] = rt[g];
+    ex[q] = x[g];
   }
   double tot[5];
   sum_partials<5>(early, PB, nb, tot, sred);
-  if (st.status != ST_RUN) {
-    if (bid == 0 && tid == 0) *Sout = st;
+  if (status != ST_RUN) {
+    if (writer) copy_state(Sout, Sin);
     return;
   }
   const double snrm = sqrt(tot[0]);
-  bool half = false;
-  if (snrm <= st.atol) {
-    half = true;
-    st.resid = snrm;
-    st.half = 1;
-    st.status = ST_CONV;
-  } else {
-    const double om = tot[1] / tot[2];
-    if (!isfinite(om) || !isfinite(snrm)) {
-      st.status = ST_BRK_OMEGA;
-      if (bid == 0 && tid == 0) *Sout = st;
+  const bool half = snrm <= atol;
+  double omega_d = 0.0;
+  if (!half) {
+    omega_d = tot[1] / tot[2];
+    if (!isfinite(omega_d) || !isfinite(snrm)) {
+      if (writer) {
+        copy_state(Sout, Sin);
+        Sout->status = ST_BRK_OMEGA;
+      }
       return;
     }
-    st.omega = om;
   }
-  st.phase = PH_L3;
-  const T alpha = (T)st.alpha, omega = (T)st.omega;
+  const T alpha = (T)alpha_d, omega = (T)omega_d;
   double rtr = 0.0, rr = 0.0;
 #pragma unroll
   for (int q = 0; q < RY; ++q) {
     if (ok[q]) {
       const long g = gg[q];
-      const double xg = ex[q] + st.alpha * (double)ep[q];
+      const double xg = ex[q] + alpha_d * (double)ep[q];
       if (half) {
         x[g] = xg;
       } else {
         const T s = er[q] - alpha * ev[q];
-        x[g] = xg + st.omega * (double)s;
+        __builtin_nontemporal_store(xg + omega_d * (double)s, x + g);
         const T rn = s - omega * et[q];
-        r[g] = rn;
+        __builtin_nontemporal_store(rn, r + g);
         rtr += (double)ert[q] * (double)rn;
         rr += (double)rn * (double)rn;
       }
@@ -572,7 +573,17 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
   double vals[2] = {rtr, rr};
   __syncthreads();   // every wave has read the prologue's sums out of sred
   write_partials<2>(vals, sred, PC, nb, bid);
-  if (bid == 0 && tid == 0) *Sout = st;
+  if (writer) {
+    copy_state(Sout, Sin);
+    Sout->phase = PH_L3;
+    if (half) {
+      Sout->resid = snrm;
+      Sout->half = 1;
+      Sout->status = ST_CONV;
+    } else {
+      Sout->omega = omega_d;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -169,7 +169,7 @@ __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by
     int j = j0 - 1 + hl;
     j = (j < 0) ? ny - 2 : (j == ny ? 1 : j);
     rok[q] = hl >= 0 && j < ny;
-    if (rok[q] && col_ok) raw[q] = in.load((long)j * nx + ic);
+    raw[q] = in.load((rok[q] && col_ok) ? (long)j * nx + ic : 0);   // node 0 when unused
   }
   // The two edge columns of all HR halo rows: thread tid < 2 HR loads the
   // value of halo row tid / 2, left (even tid) or right (odd tid) edge.
@@ -184,7 +184,7 @@ __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by
     int j = j0 - 1 + ehl;
     j = (j < 0) ? ny - 2 : (j == ny ? 1 : j);
     eok = edge_thr && ie < nx && j < ny;
-    if (eok) eraw = in.load((long)j * nx + ie);
+    eraw = in.load(eok ? (long)j * nx + ie : 0);
   }
   T wa[NCP], wb[NCP], wc[NCP], wd[NCP];
 #pragma unroll
@@ -192,30 +192,23 @@ __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by
     const int k = tid + c * NTHR;
     const int cy = k / (TX + 1), cx = k - cy * (TX + 1);
     const int j = j0 + cy, i = i0 + cx;
-    wa[c] = wb[c] = wc[c] = wd[c] = T(0);
-    if (k < NCELLS && j <= ny && i <= nx) {
-      const long cc = (long)j * (nx + 1) + i;
-      wa[c] = op.wxx[cc];
-      wb[c] = op.wyy[cc];
-      if (CROSS) {
-        wc[c] = op.wyx[cc];
-        wd[c] = op.wxy[cc];
-      }
-    }
+    const bool cok = k < NCELLS && j <= ny && i <= nx;
+    const long cc = cok ? (long)j * (nx + 1) + i : 0;
+    wa[c] = cok ? op.wxx[cc] : T(0);
+    wb[c] = cok ? op.wyy[cc] : T(0);
+    wc[c] = (CROSS && cok) ? op.wyx[cc] : T(0);
+    wd[c] = (CROSS && cok) ? op.wxy[cc] : T(0);
   }
   typename Out::Pre pre[RY];
   T dv[RY], hv[RY];
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
     const int j = j0 + ty + TYT * r, i = i0 + tx;
-    dv[r] = T(0);
-    hv[r] = T(0);
-    if (j < ny && i < nx) {
-      const long g = (long)j * nx + i;
-      pre[r] = out.prefetch(g);
-      dv[r] = op.dinv[g];
-      if (CENTER) hv[r] = op.hc[g];
-    }
+    const bool nok = j < ny && i < nx;
+    const long g = nok ? (long)j * nx + i : 0;
+    pre[r] = out.prefetch(g);
+    dv[r] = nok ? op.dinv[g] : T(0);
+    hv[r] = (CENTER && nok) ? op.hc[g] : T(0);
   }
   if (!prologue()) return;
 #pragma unroll
