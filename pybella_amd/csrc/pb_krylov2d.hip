@@ -377,9 +377,9 @@ __device__ __forceinline__ void iter_b_body(const OpDev<T>& op, const T* r, cons
   const int tid = thread_id();
   BIn<T> in{r, v, T(0)};
   BOut<T> out{t, rt, 0.0, 0.0, 0.0, 0.0, 0.0};
-  // Only rho and the status are needed up front; the state record for the
-  // next launch is rebuilt from *Sin by the one thread that writes it, so no
-  // thread holds the whole record across the tile.
+  // The state record as scalar words (see StateWords); rho and the status are
+  // read from it, the writer thread stores it back with alpha and the phase.
+  const StateWords sw = load_state_words(Sin);
   const double rho = Sin->rho;
   const int status = Sin->status;
   EarlyPartials<1> early;
@@ -391,13 +391,13 @@ __device__ __forceinline__ void iter_b_body(const OpDev<T>& op, const T* r, cons
     double tot[1];
     sum_partials<1>(early, PA, nb, tot, sred);
     if (status != ST_RUN) {
-      if (writer) copy_state(Sout, Sin);
+      if (writer) store_state_words(Sout, sw);
       return false;
     }
     alpha = rho / tot[0];
     if (tot[0] == 0.0 || !isfinite(alpha)) {
       if (writer) {
-        copy_state(Sout, Sin);
+        store_state_words(Sout, sw);
         Sout->phase = PH_B;
         Sout->status = ST_BRK_OMEGA;
       }
@@ -412,7 +412,7 @@ __device__ __forceinline__ void iter_b_body(const OpDev<T>& op, const T* r, cons
   double vals[5] = {out.ss, out.ts, out.tt, out.rts, out.rtt};
   write_partials<5>(vals, sred, PB, nb, bid);
   if (writer) {
-    copy_state(Sout, Sin);
+    store_state_words(Sout, sw);
     Sout->phase = PH_B;
     Sout->alpha = alpha;
   }
@@ -506,8 +506,9 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
   const int tid = thread_id(), nb = nblocks(), bid = xcd_tile(block_id(), nb);
   const int bx = bid % gridDim.x, by = bid / gridDim.x;
   constexpr int TY = TYT * RY;
-  // Only the status, atol and alpha are read up front; the state record for
-  // the next launch is rebuilt from *Sin by the one thread that writes it.
+  // The state record as scalar words (see StateWords); the writer thread
+  // stores it back with the fields this launch changes.
+  const StateWords sw = load_state_words(Sin);
   const int status = Sin->status;
   const double atol = Sin->atol, alpha_d = Sin->alpha;
   const bool writer = bid == 0 && tid == 0;
@@ -535,7 +536,7 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
   double tot[5];
   sum_partials<5>(early, PB, nb, tot, sred);
   if (status != ST_RUN) {
-    if (writer) copy_state(Sout, Sin);
+    if (writer) store_state_words(Sout, sw);
     return;
   }
   const double snrm = sqrt(tot[0]);
@@ -545,7 +546,7 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
     omega_d = tot[1] / tot[2];
     if (!isfinite(omega_d) || !isfinite(snrm)) {
       if (writer) {
-        copy_state(Sout, Sin);
+        store_state_words(Sout, sw);
         Sout->status = ST_BRK_OMEGA;
       }
       return;
@@ -574,7 +575,7 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
   __syncthreads();   // every wave has read the prologue's sums out of sred
   write_partials<2>(vals, sred, PC, nb, bid);
   if (writer) {
-    copy_state(Sout, Sin);
+    store_state_words(Sout, sw);
     Sout->phase = PH_L3;
     if (half) {
       Sout->resid = snrm;

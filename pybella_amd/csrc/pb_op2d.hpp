@@ -346,14 +346,27 @@ struct KState {
   int iter, maxiter, status, half, legacy, phase, x_upd;
 };
 
-// Word-wise copy of a state record (one thread), without a whole-record
-// temporary; the caller then overwrites the fields it changes.
-__device__ __forceinline__ void copy_state(KState* dst, const KState* src) {
+// A state record as 8-byte words.  Loaded at a kernel's entry from a uniform
+// address that nothing in the kernel writes, the words are scalar loads held
+// in SGPRs; the thread that writes the next record stores them back and then
+// overwrites the fields it changes, with no load at the kernel's end.
+struct StateWords {
   static_assert(sizeof(KState) % 8 == 0, "KState is copied in 8-byte words");
+  long long w[sizeof(KState) / 8];
+};
+
+__device__ __forceinline__ StateWords load_state_words(const KState* src) {
+  StateWords r;
   const long long* s = reinterpret_cast<const long long*>(src);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(KState) / 8); ++i) r.w[i] = s[i];
+  return r;
+}
+
+__device__ __forceinline__ void store_state_words(KState* dst, const StateWords& r) {
   long long* d = reinterpret_cast<long long*>(dst);
 #pragma unroll
-  for (int w = 0; w < (int)(sizeof(KState) / 8); ++w) d[w] = s[w];
+  for (int i = 0; i < (int)(sizeof(KState) / 8); ++i) d[i] = r.w[i];
 }
 
 enum {
