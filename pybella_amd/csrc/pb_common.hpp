@@ -64,30 +64,59 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block-wide sum of NV doubles; result valid in thread 0.  ``sh`` must hold
-// NV * (blockDim.x / 64) doubles.  Deterministic order.
+// wave_sum of NV values at once, step by step over all values, so the
+// cross-row exchanges of the NV values are in flight together instead of
+// one round trip after another.  Same order of additions as wave_sum.
 template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wid = threadIdx.x / kWave;
-  const int nw = (blockDim.x * blockDim.y) / kWave;
+__device__ __forceinline__ void wave_sum_n(double (&v)[NV]) {
+  double t[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) t[k] = __shfl_xor(v[k], 32, kWave);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] += t[k];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) t[k] = __shfl_xor(v[k], 16, kWave);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] += t[k];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] += dpp_f64<0x128>(v[k]);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] += dpp_f64<0x124>(v[k]);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] += dpp_f64<0x4E>(v[k]);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] += dpp_f64<0xB1>(v[k]);
+}
+
+// Block-wide sum of NV doubles over a block of exactly NW waves; the result is
+// valid in thread 0.  ``sh`` holds NV * NW doubles.  Deterministic order: wave
+// sums (wave_sum_n), then thread 0 adds them in wave order.  With the wave
+// count a constant, thread 0 reads the NV x NW wave sums in one batch.
+template <int NV, int NW>
+__device__ __forceinline__ void block_sum_w(double (&v)[NV], double* sh) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int lane2 = tid & (kWave - 1);
-  const int wid2 = tid / kWave;
-  (void)lane;
-  (void)wid;
+  const int lane = tid & (kWave - 1), wid = tid / kWave;
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = v[k];
+  wave_sum_n<NV>(s);
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    double s = wave_sum(v[k]);
-    if (lane2 == 0) sh[k * nw + wid2] = s;
+    if (lane == 0) sh[k * NW + wid] = s[k];
   }
   __syncthreads();
   if (tid == 0) {
+    double w[NV][NW];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int i = 0; i < NW; ++i) w[k][i] = sh[k * NW + i];
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < nw; ++w) s += sh[k * nw + w];
-      v[k] = s;
+      double a = 0.0;
+#pragma unroll
+      for (int i = 0; i < NW; ++i) a += w[k][i];
+      v[k] = a;
     }
   }
 }

@@ -74,7 +74,7 @@ struct InitOut {
   double rr, bb;
   __device__ Pre prefetch(long g) const { return b[g]; }
   template <class R>
-  __device__ void operator()(long g, T y, T, const Pre& bg, const R&) {
+  __device__ void operator()(int, long g, T y, T, const Pre& bg, const R&) {
     const T rg = bg - y;
     r[g] = rg;
     rt[g] = rg;
@@ -255,7 +255,7 @@ struct AOut {
   double rv;
   __device__ Pre prefetch(long g) const { return Pre{q.rt[g], q.x[g]}; }
   // e: this node's prefetched operands; w: its raw r, p, v, t (loaded for the tile)
-  __device__ void operator()(long g, T y, T pn, const Pre& e, const typename AIn<T>::Raw& w) {
+  __device__ void operator()(int, long g, T y, T pn, const Pre& e, const typename AIn<T>::Raw& w) {
     const T s = w.r - alpha * w.v;
     const T rn = s - omega * w.t;
     if (xupd) {
@@ -349,22 +349,33 @@ struct BIn {
   __device__ Raw load(long g) const { return Raw{r[g], v[g]}; }
   __device__ T combine(const Raw& q) const { return q.r - alpha * q.v; }
 };
-template <typename T>
+// t = y is held per row and stored by flush() after the block's reduction:
+// stores issued before the reduction would make its cross-lane steps wait
+// for them (their registers are reused by the reduction).
+template <typename T, int RY>
 struct BOut {
   using Pre = T;
   T* t;
   const T* rt;
   double ss, ts, tt, rts, rtt;
+  T tv[RY];
+  unsigned tg[RY];   // node of row r, or ~0u for a row outside the grid
   __device__ Pre prefetch(long g) const { return rt[g]; }
   template <class R>
-  __device__ void operator()(long g, T y, T s, const Pre& rtg, const R&) {
-    __builtin_nontemporal_store(y, t + g);
+  __device__ void operator()(int r, long g, T y, T s, const Pre& rtg, const R&) {
+    tv[r] = y;
+    tg[r] = (unsigned)g;
     const double sd = (double)s, yd = (double)y, rd = (double)rtg;
     ss += sd * sd;
     ts += yd * sd;
     tt += yd * yd;
     rts += rd * sd;
     rtt += rd * yd;
+  }
+  __device__ void flush() const {
+#pragma unroll
+    for (int r = 0; r < RY; ++r)
+      if (tg[r] != ~0u) __builtin_nontemporal_store(tv[r], t + tg[r]);
   }
 };
 
@@ -376,7 +387,9 @@ __device__ __forceinline__ void iter_b_body(const OpDev<T>& op, const T* r, cons
   __shared__ double sred[8 * (NTHR / kWave)];
   const int tid = thread_id();
   BIn<T> in{r, v, T(0)};
-  BOut<T> out{t, rt, 0.0, 0.0, 0.0, 0.0, 0.0};
+  BOut<T, RY> out{t, rt, 0.0, 0.0, 0.0, 0.0, 0.0, {}, {}};
+#pragma unroll
+  for (int r = 0; r < RY; ++r) out.tg[r] = ~0u;
   // The state record as scalar words (see StateWords); rho and the status are
   // read from it, the writer thread stores it back with alpha and the phase.
   const StateWords sw = load_state_words(Sin);
@@ -411,6 +424,7 @@ __device__ __forceinline__ void iter_b_body(const OpDev<T>& op, const T* r, cons
   if (!run) return;
   double vals[5] = {out.ss, out.ts, out.tt, out.rts, out.rtt};
   write_partials<5>(vals, sred, PB, nb, bid);
+  out.flush();
   if (writer) {
     store_state_words(Sout, sw);
     Sout->phase = PH_B;
@@ -453,7 +467,7 @@ struct L1Out {
   double rv;
   __device__ Pre prefetch(long g) const { return rt[g]; }
   template <class R>
-  __device__ void operator()(long g, T y, T pn, const Pre& rtg, const R&) {
+  __device__ void operator()(int, long g, T y, T pn, const Pre& rtg, const R&) {
     p_new[g] = pn;
     v_new[g] = y;
     rv += (double)rtg * (double)y;
@@ -603,7 +617,7 @@ struct TrOut {
   double rr;
   __device__ Pre prefetch(long g) const { return b[g]; }
   template <class R>
-  __device__ void operator()(long g, double y, double, const Pre& bg, const R&) {
+  __device__ void operator()(int, long g, double y, double, const Pre& bg, const R&) {
     const double rg = bg - y;
     r[g] = rg;
     rr += rg * rg;

@@ -34,7 +34,7 @@ struct OutStore {
   T* y;
   __device__ Pre prefetch(long) const { return Pre{}; }
   template <class R>
-  __device__ void operator()(long g, T v, T, const Pre&, const R&) { y[g] = v; }
+  __device__ void operator()(int, long g, T v, T, const Pre&, const R&) { y[g] = v; }
 };
 
 template <typename T, bool CROSS, bool CENTER, int RY>

@@ -140,7 +140,8 @@ __device__ __forceinline__ void stencil_tile(const OpDev<T>& op, int bx, int by,
 //
 // Row assignment: in pass q < RY a thread loads the raw values of its own
 // output row q (halo row 1 + ty + TYT q), so the epilogue receives them as
-// ``out(g, y, pc, pre, raw)`` without loading them again; one more pass loads
+// ``out(r, g, y, pc, pre, raw)`` (r: the thread's row 0..RY-1) without
+// loading them again; one more pass loads
 // the top (ty == 0) and bottom (ty == 1) halo rows.  The 2 x HR edge-column
 // values are loaded by the first 2 HR threads, one each, so the other lanes
 // spend neither load instructions nor registers on them.
@@ -250,7 +251,7 @@ __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by
               ((L.sfy[c1] + L.sfy[c1 + 1]) - (L.sfy[c0] + L.sfy[c0 + 1]));
       const T pc = L.sp[(ly + 1) * (TX + 2) + lx + 1];
       if (CENTER) acc += hv[r] * pc;
-      out(g, dv[r] * acc, pc, pre[r], raw[r]);
+      out(r, g, dv[r] * acc, pc, pre[r], raw[r]);
     }
   }
 }
@@ -302,10 +303,10 @@ __device__ __forceinline__ void sum_partials(const EarlyPartials<NV>& early, con
   // own barriers.
   constexpr int NW = NTHR / kWave;
   const int lane = tid & (kWave - 1), wid = tid / kWave;
+  wave_sum_n<NV>(tot);
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    const double s = wave_sum(tot[k]);
-    if (lane == 0) sred[k * NW + wid] = s;
+    if (lane == 0) sred[k * NW + wid] = tot[k];
   }
   __syncthreads();
 #pragma unroll
@@ -327,7 +328,7 @@ __device__ __forceinline__ void sum_partials(const double* P, int nb, double (&t
 // Block sum of NV thread values, written as this block's record (plain stores).
 template <int NV>
 __device__ __forceinline__ void write_partials(double (&v)[NV], double* sred, double* P, int nb, int bid) {
-  block_sum<NV>(v, sred);
+  block_sum_w<NV, NTHR / kWave>(v, sred);
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   (void)nb;
   if (tid == 0) {

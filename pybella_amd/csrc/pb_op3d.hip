@@ -165,10 +165,11 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (
   }
 }
 
-template <int NV>
+// NW: waves per block (8 for the marching blocks of NT3 threads)
+template <int NV, int NW = NT3 / kWave>
 __device__ __forceinline__ void write_partials3(double (&acc)[NV], double* part) {
-  __shared__ double sh[NV * 16];   // one slot per wave of a block of up to 1024 threads
-  block_sum<NV>(acc, sh);
+  __shared__ double sh[NV * NW];
+  block_sum_w<NV, NW>(acc, sh);
   if (tid3() == 0)
 #pragma unroll
     for (int v = 0; v < NV; ++v) part[(long)bid3() * NV + v] = acc[v];
@@ -235,7 +236,7 @@ __global__ void k3_init(long n, const T* b, T* r, T* rt, T* p, T* v, T* t, T* x,
     x[g] = 0;
     acc[0] += (double)bb * (double)bb;
   }
-  write_partials3<1>(acc, part);
+  write_partials3<1, 4>(acc, part);   // blocks of 256 threads
 }
 
 __global__ void k3_s0(KS3* st, const double* part, int nb, double tol, int maxiter, int legacy) {
