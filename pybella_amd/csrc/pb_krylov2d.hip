@@ -948,6 +948,45 @@ hipStream_t solve_stream(pb_ctx* ctx) {
 
 }  // namespace
 
+namespace {
+
+// FP64 finish of the mixed mode from the current iterate x (op->rtrue holds
+// b - A x, rn its norm): BiCGStab on A d = r_true to ||r|| <= target, x += d.
+// A run that ends short of the target (a breakdown near convergence, where
+// rho = r~.r falls below the absolute eps^2 test, or a recursive residual
+// that drifted from the true one) is restarted from the new true residual,
+// at most 8 times; a correction that does not reduce ||b - A x|| is undone.
+int fp64_finish(pb_op2d* op, const double* b, double* x, double target, int maxiter, int check_every,
+                hipStream_t s, double* rn, int* iters, int* last_info) {
+  pb_ctx* ctx = op->ctx;
+  const long n = op->n;
+  const int nb = elem_blocks(n);
+  int left = maxiter;
+  for (int restart = 0; restart < 8 && *rn > target && left > 0; ++restart) {
+    KState h64{};
+    PB_HIP(ctx, hipMemcpyAsync(op->xcorr, op->rtrue, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    int rc = krylov_run<double>(op, op->w64, op->xcorr, nullptr, target / std::max(*rn, 1e-300), left, 0,
+                                check_every, s, &h64);
+    if (rc) return rc;
+    *iters += h64.iter;
+    left -= std::max(h64.iter, 1);
+    *last_info = info_of(h64);
+    const double old = *rn;
+    hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, s, n, 1.0, (const double*)op->w64.x, x);
+    rc = true_residual(op, b, x, s, rn);
+    if (rc) return rc;
+    if (!(*rn < old)) {
+      hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, s, n, -1.0, (const double*)op->w64.x, x);
+      rc = true_residual(op, b, x, s, rn);
+      if (rc) return rc;
+      break;
+    }
+  }
+  return PB_OK;
+}
+
+}  // namespace
+
 extern "C" int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x, const pb_solve_opts* opts,
                                  pb_solve_info* info) {
   if (!op) return PB_ERR_ARG;
@@ -1010,17 +1049,8 @@ extern "C" int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x, const 
         break;
       }
     }
-    if (fallback && rn > target) {
-      // finish in FP64 from the current iterate: solve A d = r_true to ||.|| <= target
-      KState h64{};
-      PB_HIP(ctx, hipMemcpyAsync(op->xcorr, op->rtrue, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-      rc = krylov_run<double>(op, op->w64, op->xcorr, nullptr, target / std::max(rn, 1e-300), opts->maxiter, 0,
-                              opts->check_every, s, &h64);
-      if (rc) return rc;
-      iters += h64.iter;
-      last_info = info_of(h64);
-      hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, s, n, 1.0, (const double*)op->w64.x, x);
-      rc = true_residual(op, b, x, s, &rn);
+    if (fallback && rn > target) {   // finish in FP64 from the current iterate
+      rc = fp64_finish(op, b, x, target, opts->maxiter, opts->check_every, s, &rn, &iters, &last_info);
       if (rc) return rc;
       ++outer;
     }
@@ -1745,16 +1775,7 @@ int batch_group_solve(pb_batch* bt, const double* const* b, double* const* x, co
   for (size_t m = 0; m < M; ++m) {
     pb_op2d* op = bt->ops[m];
     if (stalled[m] && rn[m] > target[m]) {   // finish alone in FP64 from the current iterate
-      KState h64{};
-      PB_HIP(ctx, hipMemcpyAsync(op->xcorr, op->rtrue, sizeof(double) * op->n, hipMemcpyDeviceToDevice, s));
-      rc = krylov_run<double>(op, op->w64, op->xcorr, nullptr, target[m] / std::max(rn[m], 1e-300), opts->maxiter,
-                              0, opts->check_every, s, &h64);
-      if (rc) return rc;
-      iters[m] += h64.iter;
-      last[m] = info_of(h64);
-      hipLaunchKernelGGL(k_axpy, dim3(elem_blocks(op->n)), dim3(256), 0, s, op->n, 1.0, (const double*)op->w64.x,
-                         x[m]);
-      rc = true_residual(op, b[m], x[m], s, &rn[m]);
+      rc = fp64_finish(op, b[m], x[m], target[m], opts->maxiter, opts->check_every, s, &rn[m], &iters[m], &last[m]);
       if (rc) return rc;
       ++outer[m];
     }
