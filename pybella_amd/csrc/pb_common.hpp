@@ -106,16 +106,14 @@ __device__ __forceinline__ void block_sum_w(double (&v)[NV], double* sh) {
   }
   __syncthreads();
   if (tid == 0) {
-    double w[NV][NW];
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-#pragma unroll
-      for (int i = 0; i < NW; ++i) w[k][i] = sh[k * NW + i];
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
+      double w[NW];   // one value's wave sums at a time: NW registers, not NV x NW
+#pragma unroll
+      for (int i = 0; i < NW; ++i) w[i] = sh[k * NW + i];
       double a = 0.0;
 #pragma unroll
-      for (int i = 0; i < NW; ++i) a += w[k][i];
+      for (int i = 0; i < NW; ++i) a += w[i];
       v[k] = a;
     }
   }
