@@ -53,98 +53,23 @@ struct TileLds {
   T sfy[(TY + 1) * (TX + 1)];
 };
 
-// Apply the operator on tile (bx, by).  in(g) evaluates the input at node g;
-// out(g, y, center_value) consumes the result y = diag_inv * (A in)(g).
-template <typename T, bool CROSS, bool CENTER, int RY, class In, class Out>
-__device__ __forceinline__ void stencil_tile(const OpDev<T>& op, int bx, int by, const In& in, Out& out,
-                                             TileLds<T, RY>& L) {
-  constexpr int TY = TYT * RY;
-  constexpr int HR = TY + 2;                     // halo rows
-  constexpr int NPASS = (HR + TYT - 1) / TYT;    // row passes per thread row
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i0 = bx * TX, j0 = by * TY;
-  const int nx = op.nx, ny = op.ny;
-  // interior columns: wavefront ty loads halo rows ty, ty+4, ... (64 contiguous
-  // nodes each, coalesced); lanes 0 and 1 also load the two edge columns.
-  {
-    int ic = i0 + tx;
-    ic = (ic == nx) ? 1 : ic;                    // periodic wrap past the last node
-    const bool col_ok = ic < nx;
-    int ie = (tx == 0) ? i0 - 1 : i0 + TX;     // edge column of lanes 0/1
-    ie = (ie < 0) ? nx - 2 : (ie == nx ? 1 : ie);
-    const bool edge_ok = tx < 2 && ie < nx;
-    T vals[NPASS], evals[NPASS];
-#pragma unroll
-    for (int q = 0; q < NPASS; ++q) {
-      const int ly = ty + TYT * q;
-      int j = j0 - 1 + ly;
-      j = (j < 0) ? ny - 2 : (j == ny ? 1 : j);
-      const bool row_ok = ly < HR && j < ny;
-      vals[q] = (row_ok && col_ok) ? in((long)j * nx + ic) : T(0);
-      evals[q] = (row_ok && edge_ok) ? in((long)j * nx + ie) : T(0);
-    }
-#pragma unroll
-    for (int q = 0; q < NPASS; ++q) {
-      const int ly = ty + TYT * q;
-      if (ly < HR) {
-        L.sp[ly * (TX + 2) + tx + 1] = vals[q];
-        if (tx < 2) L.sp[ly * (TX + 2) + (tx == 0 ? 0 : TX + 1)] = evals[q];
-      }
-    }
-  }
-  __syncthreads();
-  const int tid = tx + TX * ty;
-  for (int k = tid; k < (TY + 1) * (TX + 1); k += NTHR) {
-    const int cy = k / (TX + 1), cx = k - cy * (TX + 1);
-    const int j = j0 + cy, i = i0 + cx;
-    T fx = T(0), fy = T(0);
-    if (j <= ny && i <= nx) {
-      const long c = (long)j * (nx + 1) + i;
-      const T* sp = L.sp + cy * (TX + 2) + cx;
-      const T tl = sp[0], tr = sp[1], bl = sp[TX + 2], br = sp[TX + 3];
-      const T gx = (tr - tl) + (br - bl);
-      const T gy = (bl - tl) + (br - tr);
-      fx = op.wxx[c] * gx;
-      fy = op.wyy[c] * gy;
-      if (CROSS) {
-        fx += op.wyx[c] * gy;
-        fy += op.wxy[c] * gx;
-      }
-    }
-    L.sfx[k] = fx;
-    L.sfy[k] = fy;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < RY; ++r) {
-    const int ly = ty + TYT * r, lx = tx;
-    const int j = j0 + ly, i = i0 + lx;
-    if (j < ny && i < nx) {
-      const long g = (long)j * nx + i;
-      const int c0 = ly * (TX + 1) + lx, c1 = c0 + (TX + 1);
-      T acc = ((L.sfx[c0 + 1] + L.sfx[c1 + 1]) - (L.sfx[c0] + L.sfx[c1])) +
-              ((L.sfy[c1] + L.sfy[c1 + 1]) - (L.sfy[c0] + L.sfy[c0 + 1]));
-      const T pc = L.sp[(ly + 1) * (TX + 2) + lx + 1];
-      if (CENTER) acc += op.hc[g] * pc;
-      out(g, op.dinv[g] * acc, pc);
-    }
-  }
-}
-
-// Second form with every global load issued up front: raw halo values
-// (In::load), this thread's cell weights and its nodes' epilogue operands
-// (Out::prefetch, diag_inv, hcenter) are requested before the first barrier,
-// then ``prologue()`` runs (it may reduce the previous launch's partials and
-// set the scalars that In::combine and Out use; returning false ends the
-// block), so the latencies of all of them overlap.
+// Apply the operator on tile (bx, by): y = diag_inv * (A in)(g) for each node g
+// of the tile, consumed by ``out``.  Every global load is issued up front: the
+// raw halo values (In::load), this thread's cell weights and its nodes'
+// epilogue operands (Out::prefetch, diag_inv, hcenter) are requested before the
+// first barrier, then ``prologue()`` runs (it may reduce the previous launch's
+// partials and set the scalars that In::combine and Out use; returning false
+// ends the block), so the latencies of all of them overlap.  Loads are
+// unconditional (out-of-grid threads read node 0 and discard it), so no branch
+// separates them and the waits before the prologue cover only what it needs.
 //
 // Row assignment: in pass q < RY a thread loads the raw values of its own
 // output row q (halo row 1 + ty + TYT q), so the epilogue receives them as
-// ``out(r, g, y, pc, pre, raw)`` (r: the thread's row 0..RY-1) without
-// loading them again; one more pass loads
-// the top (ty == 0) and bottom (ty == 1) halo rows.  The 2 x HR edge-column
-// values are loaded by the first 2 HR threads, one each, so the other lanes
-// spend neither load instructions nor registers on them.
+// ``out(r, g, y, pc, pre, raw)`` (r: the thread's row 0..RY-1) without loading
+// them again; one more pass loads the top (ty == 0) and bottom (ty == 1) halo
+// rows.  The 2 x HR edge-column values are loaded by the first 2 HR threads,
+// one each, so the other lanes spend neither load instructions nor registers
+// on them.
 template <typename T, bool CROSS, bool CENTER, int RY, class In, class Out, class Pro>
 __device__ __forceinline__ void stencil_tile2(const OpDev<T>& op, int bx, int by, In& in, Out& out,
                                               TileLds<T, RY>& L, Pro&& prologue) {
