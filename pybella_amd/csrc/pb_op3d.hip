@@ -28,6 +28,8 @@
 // refinement with the FP64 true residual.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <map>
 #include <vector>
 
 #include "pb_common.hpp"
@@ -456,6 +458,7 @@ struct pb_op3d {
   KS3* hst = nullptr;
   bool have_work = false;
   std::vector<void*> allocs;
+  std::map<int, hipGraphExec_t> graphs;   // captured iteration chunks, by (precision, chunk, centre flag)
 };
 
 namespace {
@@ -559,11 +562,12 @@ int krylov3(pb_op3d* op, const T* b, double tol, int maxiter, int legacy, int ch
   hipLaunchKernelGGL(k3_s0, dim3(1), dim3(256), 0, s, op->st, (const double*)op->part, (int)eg.x, tol, maxiter,
                      legacy);
   PB_LAUNCH_CHECK(op->ctx);
-  const int chunk = std::max(1, check_every > 0 ? check_every : 32);
-  long it = 0;
-  for (;;) {
+  // an even chunk starts every replay at the same buffer parity, so one
+  // captured graph per (precision, chunk) serves every chunk of every solve
+  const int chunk = 2 * std::max(1, (check_every > 0 ? check_every : 32) / 2);
+  auto launch_chunk = [&](long it0) {
     for (int c = 0; c < chunk; ++c) {
-      ++it;
+      const long it = it0 + c + 1;
       const int a = (int)((it - 1) & 1), bset = (int)(it & 1);
       hipLaunchKernelGGL(k3_iter1<T>, mg, kBlk3, 0, s, o, (const KS3*)op->st, (const T*)op->r[P][a],
                          (const T*)op->p[P][a], (const T*)op->v[P][a], (const T*)t, (const T*)rt, (T*)op->r[P][bset],
@@ -573,6 +577,31 @@ int krylov3(pb_op3d* op, const T* b, double tol, int maxiter, int legacy, int ch
                          (const T*)op->v[P][bset], (const T*)rt, t, op->part);
       hipLaunchKernelGGL(k3_s2, dim3(1), dim3(256), 0, s, op->st, (const double*)op->part, nbm);
     }
+  };
+  const char* no_graph = std::getenv("PYBELLA_AMD_NO_GRAPH");
+  const bool graphs = !(no_graph && no_graph[0] == '1') && s != nullptr;
+  long it = 0;
+  for (;;) {
+    if (graphs) {
+      // the captured arguments hold the operator's centre flag, which a
+      // re-assembly in the other regime changes
+      const int key = (chunk * 2 + P) * 2 + (op->center ? 1 : 0);
+      auto g = op->graphs.find(key);
+      if (g == op->graphs.end()) {
+        hipGraph_t gr = nullptr;
+        hipGraphExec_t ex = nullptr;
+        PB_HIP(op->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        launch_chunk(0);
+        PB_HIP(op->ctx, hipStreamEndCapture(s, &gr));
+        PB_HIP(op->ctx, hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+        g = op->graphs.emplace(key, ex).first;
+      }
+      PB_HIP(op->ctx, hipGraphLaunch(g->second, s));
+    } else {
+      launch_chunk(it);
+    }
+    it += chunk;
     PB_LAUNCH_CHECK(op->ctx);
     PB_HIP(op->ctx, hipMemcpyAsync(op->hst, op->st, sizeof(KS3), hipMemcpyDeviceToHost, s));
     PB_HIP(op->ctx, hipStreamSynchronize(s));
@@ -652,6 +681,8 @@ extern "C" int pb_op3d_create(pb_ctx* ctx, const pb_grid3d* grid, pb_op3d** out)
 extern "C" int pb_op3d_destroy(pb_op3d* op) {
   if (!op) return PB_OK;
   if (op->ctx) (void)hipStreamSynchronize(op->ctx->stream);
+  if (op->ctx) (void)hipStreamSynchronize(op->ctx->own_stream);
+  for (auto& kv : op->graphs) (void)hipGraphExecDestroy(kv.second);
   for (void* p : op->allocs) (void)hipFree(p);
   if (op->hst) (void)hipHostFree(op->hst);
   delete op;
@@ -706,6 +737,10 @@ extern "C" int pb_op3d_solve(pb_op3d* op, const double* b, double* x, const pb_s
   int rc = alloc_work3(op);
   if (rc) return rc;
   hipStream_t s = op->ctx->stream;
+  if (s == nullptr) {   // the null stream cannot be captured: run on the context's own stream
+    (void)hipStreamSynchronize(nullptr);
+    s = op->ctx->own_stream;
+  }
   const long n = op->n;
   const dim3 eg = elem_grid3(n);
   pb_solve_info out{};

@@ -104,3 +104,29 @@ def test_large_mixed_solve():
     torch.cuda.synchronize()
     print("mixed 129x65x65:", info, f"{time.perf_counter() - t0:.3f} s", flush=True)
     assert info["info"] == 0 and info["true_rel_resid"] <= 1e-8
+
+
+def test_reassembly_other_regime_replays_fresh_graph():
+    """One handle: a compressible solve captures its iteration chunks as
+    graphs; re-assembled in the pseudo-incompressible regime (no centre term),
+    the solve's first iterates must be the oracle's, not a replay with the
+    compressible launch arguments."""
+    import torch
+    from pybella_amd.op3d import Operator3D, phys3_of
+    print("case op3d reassembly compressible -> psinc", flush=True)
+    st_c = _state(33, 17, 19, compressible=True)
+    op = Operator3D.from_state(st_c)
+    ref_c = olp.assemble3d(st_c, 0.05)
+    b_c = W.fourier_rhs_3d(33, 17, 19, seed=8, nmodes=32, kmax=6).ravel() * ref_c["dinv"].ravel()
+    _, info = op.solve(_dev(b_c), tol=1e-10, mode="f64")
+    assert info["info"] == 0
+    st_p = _state(33, 17, 19, compressible=False)
+    ref_p = olp.assemble3d(st_p, 0.05)
+    op.assemble(phys3_of(st_p), 0.05, _dev(st_p.rho_c), _dev(st_p.rhoY_c))
+    b_p = W.fourier_rhs_3d(33, 17, 19, seed=9, nmodes=32, kmax=6).ravel() * ref_p["dinv"].ravel()
+    for k in (1, 5):
+        x, info = op.solve(_dev(b_p), tol=1e-30, maxiter=k, mode="f64")
+        torch.cuda.synchronize()
+        r = obs.bicgstab(lambda v: olp.lap3d(v, ref_p), b_p, tol=1e-30, maxiter=k)
+        assert info["iters"] == k == r.niter
+        assert rel(x.cpu().numpy().ravel(), r.x) < 1e-9, k
