@@ -69,10 +69,22 @@ __device__ __forceinline__ int nblk3() { return gridDim.x * gridDim.y * gridDim.
 __device__ __forceinline__ double eps2() { return 4.930380657631324e-32; }
 
 // The march: ``in(g)`` gives the operand at node g (may combine vectors),
-// ``out(g, y, pc, acc)`` consumes y = (A operand)(g) and the operand's own
-// value pc for the owned nodes; acc collects per-thread partial sums.
-template <typename T, int NV, class In, class Out>
-__device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (&acc)[NV > 0 ? NV : 1]) {
+// ``pre(g)`` loads the values ``out`` needs at node g, and
+// ``out(g, y, pc, e, acc)`` consumes y = (A operand)(g), the operand's own
+// value pc and e = pre(g) for the owned nodes; acc collects per-thread partial
+// sums.  The next node plane (operand values) is always loaded one step
+// ahead.  PF sets where the other loads of a step are issued:
+//   2 (FP32 default): one step ahead too -- the next cell plane's weights and
+//     the next node's diag_inv, hcenter and pre(g);
+//   1 (FP64 default): at the start of their own step, in flight during its
+//     barrier (one step ahead would cost the registers of a third block per
+//     CU), pre(g) where it is used;
+//   0: where they are used (kernels whose accumulators leave no registers).
+template <typename T, int NV, int PF = (sizeof(T) == 4 ? 2 : 1), class In, class PreFn, class Out>
+__device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out out,
+                                       double (&acc)[NV > 0 ? NV : 1]) {
+  using PreT = decltype(pre(0L));
+  constexpr bool AHEAD = PF == 2;   // node operands one step ahead
   __shared__ T P[2][Y3 + 2][X3 + 2];
   __shared__ T F[3][Y3 + 1][X3 + 1];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tid3();
@@ -87,8 +99,10 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (
   const int nx = o.nx, ny = o.ny;
   const int i = x0 + tx, j = y0 + ty;
   const bool own = i < nx && j < ny;
-  // a node plane with its halo is NQ values, at most QPT per thread
+  // a node plane with its halo is NQ values, at most QPT per thread; a cell
+  // plane NC weights, at most CPT per thread
   constexpr int NQ = (Y3 + 2) * (X3 + 2), QPT = (NQ + NT3 - 1) / NT3;
+  constexpr int NC = (Y3 + 1) * (X3 + 1), CPT = (NC + NT3 - 1) / NT3;
   auto fetch = [&](int k, T (&r)[QPT]) {   // global loads of plane k into registers
     const long kk = wrap3(k, o.nz, o.pz);
 #pragma unroll
@@ -113,22 +127,50 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (
     fetch(k, r);
     store(slot, r);
   };
-  auto cells = [&](int k, int s0, int s1) {   // cell plane between node planes k and k+1
+  auto fetch_cells = [&](int k, T (&hw)[CPT]) {   // weights of the cell plane between node planes k, k+1
     const long ck = k + 1;
-    for (int q = tid; q < (Y3 + 1) * (X3 + 1); q += NT3) {
+#pragma unroll
+    for (int e = 0; e < CPT; ++e) {
+      const int q = tid + e * NT3;
       const int cj = q / (X3 + 1), ci = q - cj * (X3 + 1);
       const int cy = y0 + cj, cx = x0 + ci;
-      T h = 0;
-      if (cy <= ny && cx <= nx) h = o.h[(ck * (ny + 1) + cy) * (nx + 1) + cx];
-      const T a00 = P[s0][cj][ci], a01 = P[s0][cj][ci + 1], a10 = P[s0][cj + 1][ci], a11 = P[s0][cj + 1][ci + 1];
-      const T b00 = P[s1][cj][ci], b01 = P[s1][cj][ci + 1], b10 = P[s1][cj + 1][ci], b11 = P[s1][cj + 1][ci + 1];
-      const T gx = ((a01 - a00) + (a11 - a10)) + ((b01 - b00) + (b11 - b10));
-      const T gy = ((a10 - a00) + (a11 - a01)) + ((b10 - b00) + (b11 - b01));
-      const T gz = ((b00 - a00) + (b01 - a01)) + ((b10 - a10) + (b11 - a11));
-      F[0][cj][ci] = h * gx * o.sx;
-      F[1][cj][ci] = h * gy * o.sy;
-      F[2][cj][ci] = h * gz * o.sz;
+      const bool ok = q < NC && cy <= ny && cx <= nx;
+      hw[e] = ok ? o.h[(ck * (ny + 1) + cy) * (nx + 1) + cx] : T(0);
     }
+  };
+  // cell fluxes between the node planes in slots s0, s1 of cell plane k+1;
+  // weights from hw, or (PF == 0) loaded here, each just before its use
+  auto cells = [&](int k, const T (&hw)[CPT], int s0, int s1) {
+#pragma unroll
+    for (int e = 0; e < CPT; ++e) {
+      const int q = tid + e * NT3;
+      if (q < NC) {
+        const int cj = q / (X3 + 1), ci = q - cj * (X3 + 1);
+        T hh = hw[e];
+        if (PF == 0) {
+          const int cy = y0 + cj, cx = x0 + ci;
+          hh = (cy <= ny && cx <= nx) ? o.h[((long)(k + 1) * (ny + 1) + cy) * (nx + 1) + cx] : T(0);
+        }
+        const T a00 = P[s0][cj][ci], a01 = P[s0][cj][ci + 1], a10 = P[s0][cj + 1][ci], a11 = P[s0][cj + 1][ci + 1];
+        const T b00 = P[s1][cj][ci], b01 = P[s1][cj][ci + 1], b10 = P[s1][cj + 1][ci], b11 = P[s1][cj + 1][ci + 1];
+        const T gx = ((a01 - a00) + (a11 - a10)) + ((b01 - b00) + (b11 - b10));
+        const T gy = ((a10 - a00) + (a11 - a01)) + ((b10 - b00) + (b11 - b01));
+        const T gz = ((b00 - a00) + (b01 - a01)) + ((b10 - a10) + (b11 - a11));
+        F[0][cj][ci] = hh * gx * o.sx;
+        F[1][cj][ci] = hh * gy * o.sy;
+        F[2][cj][ci] = hh * gz * o.sz;
+      }
+    }
+  };
+  struct NodeOps {
+    T dinv, hc;
+    PreT e;
+  };
+  auto fetch_node = [&](int k, NodeOps& nd) {   // this thread's node of plane k (node 0 when not owned)
+    const long g = own ? ((long)k * ny + j) * nx + i : 0;
+    nd.dinv = o.dinv[g];
+    nd.hc = o.center ? o.hc[g] : T(0);
+    if (AHEAD) nd.e = pre(g);
   };
   auto dz = [&](T& D, T& Z) {
     D = ((F[0][ty][tx + 1] + F[0][ty + 1][tx + 1]) - (F[0][ty][tx] + F[0][ty + 1][tx])) +
@@ -137,20 +179,40 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (
   };
   load_plane(kb - 1, 0);
   load_plane(kb, 1);
-  T pre[QPT];
-  fetch(kb + 1, pre);       // plane kb+1 in flight during the first cell plane
+  T nxt[QPT];
+  fetch(kb + 1, nxt);       // plane kb+1 in flight during the first cell plane
+  T h0[CPT], hw[CPT];
+#pragma unroll
+  for (int e = 0; e < CPT; ++e) hw[e] = T(0);
+  if (PF != 0) fetch_cells(kb - 1, h0);
+  if (AHEAD) fetch_cells(kb, hw);   // cell plane of step kb
+  NodeOps nd;
+  if (AHEAD) fetch_node(kb, nd);   // node operands of step kb
   __syncthreads();
-  cells(kb - 1, 0, 1);
+  cells(kb - 1, h0, 0, 1);
   __syncthreads();
   T Dp, Zp;
   dz(Dp, Zp);
   for (int k = kb; k < ke; ++k) {
     const int sk = (k - kb + 1) & 1, sn = sk ^ 1;
     __syncthreads();          // F and the slot of plane k-1 are free
-    store(sn, pre);           // plane k+1
-    if (k + 1 < ke) fetch(k + 2, pre);   // next plane in flight during this step's compute
+    store(sn, nxt);           // plane k+1
+    T hw1[CPT];
+    NodeOps nd1;
+    const bool more = k + 1 < ke;
+    if (more) {               // step k+1's loads in flight during this step's compute
+      fetch(k + 2, nxt);
+      if (AHEAD) {
+        fetch_cells(k + 1, hw1);
+        fetch_node(k + 1, nd1);
+      }
+    }
+    if (PF == 1) {            // this step's own loads, in flight during the barrier
+      fetch_cells(k, hw);
+      fetch_node(k, nd);
+    }
     __syncthreads();
-    cells(k, sk, sn);
+    cells(k, hw, sk, sn);
     __syncthreads();
     T Dn, Zn;
     dz(Dn, Zn);
@@ -158,12 +220,24 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, Out out, double (
       const long g = ((long)k * ny + j) * nx + i;
       const T pc = P[sk][ty + 1][tx + 1];
       T y = (Dp + Dn) + (Zn - Zp);
-      if (o.center) y += o.hc[g] * pc;
-      y *= o.dinv[g];
-      out(g, y, pc, acc);
+      if (PF == 0) {
+        if (o.center) y += o.hc[g] * pc;
+        y *= o.dinv[g];
+        out(g, y, pc, pre(g), acc);
+      } else {
+        if (o.center) y += nd.hc * pc;
+        y *= nd.dinv;
+        if (AHEAD) out(g, y, pc, nd.e, acc);
+        else out(g, y, pc, pre(g), acc);
+      }
     }
     Dp = Dn;
     Zp = Zn;
+    if (AHEAD && more) {
+#pragma unroll
+      for (int e = 0; e < CPT; ++e) hw[e] = hw1[e];
+      nd = nd1;
+    }
   }
 }
 
@@ -206,16 +280,18 @@ __device__ __forceinline__ void reduce_records(const double* part, int nb, doubl
 template <typename T>
 __global__ __launch_bounds__(NT3) void k3_matvec(Op3<T> o, const T* p, T* y) {
   double acc[1] = {0.0};
-  march3<T, 0>(o, [&](long g) { return p[g]; }, [&](long g, T v, T, double(&)[1]) { y[g] = v; }, acc);
+  march3<T, 0>(
+      o, [&](long g) { return p[g]; }, [&](long) { return 0; },
+      [&](long g, T v, T, int, double(&)[1]) { y[g] = v; }, acc);
 }
 
 // r = b - A x (FP64), r32 = (float) r, partials of r.r
 __global__ __launch_bounds__(NT3) void k3_resid(Op3<double> o, const double* x, const double* b, double* r,
                                                 float* r32, double* part) {
   double acc[1] = {0.0};
-  march3<double, 1>(o, [&](long g) { return x[g]; },
-                    [&](long g, double y, double, double(&a)[1]) {
-                      const double rr = b[g] - y;
+  march3<double, 1>(o, [&](long g) { return x[g]; }, [&](long g) { return b[g]; },
+                    [&](long g, double y, double, double bg, double(&a)[1]) {
+                      const double rr = bg - y;
                       r[g] = rr;
                       if (r32) r32[g] = (float)rr;
                       a[0] += rr * rr;
@@ -269,6 +345,9 @@ __global__ __launch_bounds__(NT3) void k3_iter1(Op3<T> o, const KS3* st, const T
   if (st->status != S3_RUN) return;
   const T al = (T)st->alpha, om = (T)st->omega, be = (T)st->beta;
   double acc[1] = {0.0};
+  struct Ops {   // this node's v, r, p, t, r~, x, loaded a plane step ahead
+    T v, r, p, t, rt, x;
+  };
   march3<T, 1>(
       o,
       [&](long g) {
@@ -276,14 +355,14 @@ __global__ __launch_bounds__(NT3) void k3_iter1(Op3<T> o, const KS3* st, const T
         const T rn = (ra[g] - al * v) - om * t[g];
         return rn + be * (pa[g] - om * v);
       },
-      [&](long g, T y, T pn, double(&a)[1]) {
-        const T v = va[g];
-        const T s = ra[g] - al * v;
-        x[g] += al * pa[g] + om * s;
-        rb[g] = s - om * t[g];
+      [&](long g) { return Ops{va[g], ra[g], pa[g], t[g], rt[g], x[g]}; },
+      [&](long g, T y, T pn, const Ops& e, double(&a)[1]) {
+        const T s = e.r - al * e.v;
+        x[g] = e.x + (al * e.p + om * s);
+        rb[g] = s - om * e.t;
         pb[g] = pn;
         vb[g] = y;
-        a[0] += (double)rt[g] * (double)y;
+        a[0] += (double)e.rt * (double)y;
       },
       acc);
   write_partials3<1>(acc, part);
@@ -311,11 +390,12 @@ __global__ __launch_bounds__(NT3) void k3_iter2(Op3<T> o, const KS3* st, const T
   if (st->status != S3_RUN) return;
   const T al = (T)st->alpha;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  march3<T, 5>(
-      o, [&](long g) { return r[g] - al * v[g]; },
-      [&](long g, T y, T s, double(&a)[5]) {
+  // FP64: five FP64 accumulators; its step loads stay where they are used
+  march3<T, 5, sizeof(T) == 4 ? 2 : 0>(
+      o, [&](long g) { return r[g] - al * v[g]; }, [&](long g) { return rt[g]; },
+      [&](long g, T y, T s, T rtg, double(&a)[5]) {
         t[g] = y;
-        const double sd = s, td = y, rd = rt[g];
+        const double sd = s, td = y, rd = rtg;
         a[0] += sd * sd;
         a[1] += td * sd;
         a[2] += td * td;
