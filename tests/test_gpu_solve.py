@@ -114,6 +114,28 @@ def test_tile_heights_agree(ry):
     assert np.max(np.abs(x.ravel() - ref.x)) / np.max(np.abs(ref.x)) < 1e-10
 
 
+@pytest.mark.parametrize("ry", [2, 5])
+@pytest.mark.parametrize("mode", ["f64", "f64fast"])
+@pytest.mark.parametrize("nx,ny", [(3, 3), (9, 5), (65, 9), (66, 21)])
+def test_small_and_ragged_grids_first_iterates(nx, ny, mode, ry):
+    """Grids smaller than one tile, one column past a tile, one tile row: the
+    tile's edge columns, halo rows, clamped loads and partial epilogue rows
+    at their limits; the first iterates must be the oracle's."""
+    sp = _sp()
+    print(f"case small grid {nx}x{ny} {mode} ry={ry}", flush=True)
+    (Sol, mpv, elem, node, ud, th, dt), op_o, rhs, b = system(nx, ny)
+    ref = obs.bicgstab(lambda v: olp.lap2d(v, op_o), b, tol=1e-30, maxiter=3)
+    op = sp.operator_for(node, ud)
+    op.set_tile(ry)
+    try:
+        x, info = sp.solve_isolated(Sol, mpv, elem, node, ud, th, dt, rhs, mode=mode, tol=1e-30, maxiter=3,
+                                    check_every=2)
+    finally:
+        op.set_tile(4)
+    assert info["iters"] == ref.niter == 3
+    assert np.max(np.abs(x.ravel() - ref.x)) / np.max(np.abs(ref.x)) < 1e-10
+
+
 @pytest.mark.parametrize("mode", ["f64", "f64fast"])
 def test_many_blocks_first_iterates(mode):
     """1025 x 513 at tile height 2 is 17 x 65 = 1105 blocks, more than the two
