@@ -71,27 +71,35 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
   f32x4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int nout = NT * 16;
-  for (int tap = 0; tap < 9; ++tap) {
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const int ys = ya + dy, xs = xa + dx;
-    const bool ok = va && ys >= 0 && ys < h && xs >= 0 && xs < w;
-    const long src = ok ? (((ba * h + ys) * w + xs) * in_ctot + in_coff + 4 * kg) : 0;
-    const int wt = flip ? 8 - tap : tap;
-    const bf16* wrow = wg + ((long)wt * nout + (lane & 15)) * cin + 4 * kg;
-    for (int ci0 = 0; ci0 < cin; ci0 += 16) {
-      bf16x4 a = {0, 0, 0, 0};
-      if (ok) a = *(const bf16x4*)(in + src + ci0);
+  // the block's output tiles: t0 .. t0+NT-1 of gridDim.y * NT (a grid split
+  // over the output channels gives small levels more waves)
+  const int t0 = blockIdx.y * NT;
+  const int nout = gridDim.y * NT * 16;
+  // chunk-outer, tap-inner: the nine taps of a 16-channel chunk are one
+  // straight run of independent loads, so a chunk costs one load latency
+  for (int ci0 = 0; ci0 < cin; ci0 += 16) {
+    bf16x4 av[9], bv[9][NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const bf16x4 b = *(const bf16x4*)(wrow + (long)t * 16 * cin + ci0);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-      }
+    for (int tap = 0; tap < 9; ++tap) {   // all loads of the chunk first
+      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      const int ys = ya + dy, xs = xa + dx;
+      const bool ok = va && ys >= 0 && ys < h && xs >= 0 && xs < w;
+      const long src = ok ? (((ba * h + ys) * w + xs) * in_ctot + in_coff + 4 * kg) : 0;
+      const int wt = flip ? 8 - tap : tap;
+      const bf16* wrow = wg + ((long)wt * nout + (lane & 15)) * cin + 4 * kg;
+      av[tap] = *(const bf16x4*)(in + src + ci0);   // src 0 when outside: read, then zeroed
+      if (!ok) av[tap] = bf16x4{0, 0, 0, 0};
+#pragma unroll
+      for (int t = 0; t < NT; ++t) bv[tap][t] = *(const bf16x4*)(wrow + (long)(t0 + t) * 16 * cin + ci0);
     }
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[tap], bv[tap][t], acc[t], 0, 0, 0);
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    const int co = t * 16 + (lane & 15);
+    const int co = (t0 + t) * 16 + (lane & 15);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const long p = p0 + 4 * kg + e;
@@ -251,21 +259,36 @@ __global__ void k_down_bwd(const bf16* dc, int cct, int cco, bf16* df, int fct, 
 }
 
 // linear up-sampling (fine = 2 coarse - 1 per axis) and its adjoint (gather)
+// Up-sampling of ch channels (a multiple of 8): one thread per fine pixel and
+// 8 channels, 16-byte loads of the 4 coarse neighbours and a 16-byte store
+// (channel strides and offsets are multiples of 16 elements).
 __global__ void k_up(const bf16* c, int cct, int cco, bf16* f, int fct, int fco, int ch, int nimg, int hc, int wc,
                      int hf, int wf) {
-  const long n = (long)nimg * hf * wf * ch;
+  const int cv = ch / 8;
+  const long n = (long)nimg * hf * wf * cv;
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
-    const int cc = (int)(k % ch);
-    long t = k / ch;
-    const int x = (int)(t % wf);
-    t /= wf;
+    const long pix = k / cv;
+    const int c8 = (int)(k - pix * cv) * 8;
+    const int x = (int)(pix % wf);
+    const long t = pix / wf;
     const int y = (int)(t % hf);
     const long b = t / hf;
     const int y0 = y >> 1, x0 = x >> 1;
     const int y1 = (y & 1) ? y0 + 1 : y0, x1 = (x & 1) ? x0 + 1 : x0;
-    auto at = [&](int yy, int xx) { return bf2f(c[((b * hc + yy) * wc + xx) * cct + cco + cc]); };
-    const float v = 0.25f * ((at(y0, x0) + at(y0, x1)) + (at(y1, x0) + at(y1, x1)));
-    f[((b * hf + y) * wf + x) * fct + fco + cc] = f2bf(v);
+    auto at = [&](int yy, int xx) {
+      return *(const uint4*)(c + ((b * hc + yy) * wc + xx) * cct + cco + c8);
+    };
+    const uint4 q00 = at(y0, x0), q01 = at(y0, x1), q10 = at(y1, x0), q11 = at(y1, x1);
+    const bf16* a00 = (const bf16*)&q00;
+    const bf16* a01 = (const bf16*)&q01;
+    const bf16* a10 = (const bf16*)&q10;
+    const bf16* a11 = (const bf16*)&q11;
+    uint4 o;
+    bf16* ov = (bf16*)&o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      ov[e] = f2bf(0.25f * ((bf2f(a00[e]) + bf2f(a01[e])) + (bf2f(a10[e]) + bf2f(a11[e]))));
+    *(uint4*)(f + ((b * hf + y) * wf + x) * fct + fco + c8) = o;
   }
 }
 __global__ void k_up_bwd(const bf16* df, int fct, int fco, bf16* dc, int cct, int cco, int ch, int nimg, int hc,
@@ -637,13 +660,17 @@ void conv(pb_nn* nn, Acts* a, int l, int level, const bf16* in, int in_ct, int i
   const LayerSpec& L = kLayers[l];
   const int cin = pad16(L.cin_v), nt = pad16(L.cout_v) / 16;
   const long npix = (long)a->nimg * a->lv[level].h * a->lv[level].w;
-  const dim3 grd((unsigned)((npix + 63) / 64)), blk(256);
   const float* bias = nn->params + nn->boff[l];
   const int h = a->lv[level].h, w = a->lv[level].w;
+  // A wave walks 9 taps x cin/16 chunks of dependent loads; on small levels
+  // one output tile per block (grid y over the tiles) gives nt times the waves
+  // to hide that latency (the input loads repeat, from cache).
+  const bool split = npix < 65536 && nt > 1;
+  const dim3 grd((unsigned)((npix + 63) / 64), split ? nt : 1), blk(256);
 #define PB_CONV(NT)                                                                                              \
   hipLaunchKernelGGL(k_conv3x3<NT>, grd, blk, 0, s, in, in_ct, in_co, cin, nn->wf[l], 0, bias, L.act, out, out_ct, \
                      out_co, L.cout_v, pre, pad16(L.cout_v), out_f32, a->nimg, h, w)
-  switch (nt) {
+  switch (split ? 1 : nt) {
     case 1: PB_CONV(1); break;
     case 2: PB_CONV(2); break;
     default: PB_CONV(4); break;
@@ -715,7 +742,7 @@ void down_bwd(Acts* a, int lf, const bf16* dc, int cct, int cco, bf16* df, int f
 }
 void up(Acts* a, int lf, const bf16* c, int cct, int cco, bf16* f, int fct, int fco, int ch, hipStream_t s) {
   const Level F = a->lv[lf], Cc = a->lv[lf + 1];
-  const long n = (long)a->nimg * F.h * F.w * ch;
+  const long n = (long)a->nimg * F.h * F.w * (ch / 8);   // threads: pixel x 8 channels
   hipLaunchKernelGGL(k_up, dim3(gs(n)), dim3(256), 0, s, c, cct, cco, f, fct, fco, ch, a->nimg, Cc.h, Cc.w, F.h, F.w);
 }
 void up_bwd(Acts* a, int lf, const bf16* df, int fct, int fco, bf16* dc, int cct, int cco, int ch, hipStream_t s) {
