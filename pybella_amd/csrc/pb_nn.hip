@@ -75,27 +75,22 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
   // over the output channels gives small levels more waves)
   const int t0 = blockIdx.y * NT;
   const int nout = gridDim.y * NT * 16;
-  // chunk-outer, tap-inner: the nine taps of a 16-channel chunk are one
-  // straight run of independent loads, so a chunk costs one load latency
-  for (int ci0 = 0; ci0 < cin; ci0 += 16) {
-    bf16x4 av[9], bv[9][NT];
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int ys = ya + dy, xs = xa + dx;
+    const bool ok = va && ys >= 0 && ys < h && xs >= 0 && xs < w;
+    const long src = ok ? (((ba * h + ys) * w + xs) * in_ctot + in_coff + 4 * kg) : 0;
+    const int wt = flip ? 8 - tap : tap;
+    const bf16* wrow = wg + ((long)wt * nout + (lane & 15)) * cin + 4 * kg;
+    for (int ci0 = 0; ci0 < cin; ci0 += 16) {
+      bf16x4 a = {0, 0, 0, 0};
+      if (ok) a = *(const bf16x4*)(in + src + ci0);
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {   // all loads of the chunk first
-      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-      const int ys = ya + dy, xs = xa + dx;
-      const bool ok = va && ys >= 0 && ys < h && xs >= 0 && xs < w;
-      const long src = ok ? (((ba * h + ys) * w + xs) * in_ctot + in_coff + 4 * kg) : 0;
-      const int wt = flip ? 8 - tap : tap;
-      const bf16* wrow = wg + ((long)wt * nout + (lane & 15)) * cin + 4 * kg;
-      av[tap] = *(const bf16x4*)(in + src + ci0);   // src 0 when outside: read, then zeroed
-      if (!ok) av[tap] = bf16x4{0, 0, 0, 0};
-#pragma unroll
-      for (int t = 0; t < NT; ++t) bv[tap][t] = *(const bf16x4*)(wrow + (long)(t0 + t) * 16 * cin + ci0);
+      for (int t = 0; t < NT; ++t) {
+        const bf16x4 b = *(const bf16x4*)(wrow + (long)(t0 + t) * 16 * cin + ci0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      }
     }
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap)
-#pragma unroll
-      for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[tap], bv[tap][t], acc[t], 0, 0, 0);
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
