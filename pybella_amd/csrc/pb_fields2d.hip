@@ -29,12 +29,13 @@ Dims2 dims_of(const pb_grid2d& g) {
 }
 
 // ---------------------------------------------------------------------------
-// set_explicit_boundary_data.  One thread per cell column.  Ghost columns copy
-// the inner rows of their source column (wrap or mirror, boundary.py:135-216),
-// then every column fills its two ghost rows at each y wall from the
-// hydrostatic image formula (boundary.py:56-132).  The reference pads x first
-// and then fills the y ghost rows of every column with column-local formulas,
-// which is what each thread does for its own column.
+// set_explicit_boundary_data.  Ghost columns copy the inner rows of their
+// source column (wrap or mirror, boundary.py:135-216), then every column fills
+// its two ghost rows at each y wall from the hydrostatic image formula
+// (boundary.py:56-132).  The reference pads x first and then fills the y ghost
+// rows of every column with column-local formulas; here the ghost-column rows
+// are copied one per thread, and each column's thread copies the four rows its
+// own images read before it forms them.
 // ---------------------------------------------------------------------------
 struct BcParams {
   int periodic_x;
@@ -67,33 +68,54 @@ __device__ __forceinline__ void y_image(double* rho, double* rhou, double* rhov,
   rhoX[ii] = r * X;
 }
 
-__global__ void k_bc_cells(Dims2 d, BcParams bp, double* rho, double* rhou, double* rhov,
-                           double* rhow, double* rhoY, double* rhoX, const double* rhoY0) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= d.cx) return;
-  const int cx = d.cx, cy = d.cy;
-  int srcc = -1;
+// x-axis ghost column ``col`` (0, 1, cx-2 or cx-1), inner row j: copy of the
+// wrapped (periodic) or mirrored (wall) inner column
+__device__ __forceinline__ void x_ghost_row(double* rho, double* rhou, double* rhov, double* rhow,
+                                            double* rhoY, double* rhoX, int cx, int col, int j,
+                                            const BcParams& bp) {
+  int srcc;
   double vsign = 1.0;
   bool copy_w = true;
-  if (col < 2 || col >= cx - 2) {
-    if (bp.periodic_x) {
-      srcc = col < 2 ? cx - 4 + col : col - (cx - 4);  // wrap over cx-4 inner columns
-    } else {
-      srcc = col < 2 ? 3 - col : 2 * cx - 5 - col;     // symmetric mirror
-      vsign = -1.0;                                    // rhov negative-symmetric
-      copy_w = false;                                  // rhow not padded on walls
-    }
+  if (bp.periodic_x) {
+    srcc = col < 2 ? cx - 4 + col : col - (cx - 4);  // wrap over cx-4 inner columns
+  } else {
+    srcc = col < 2 ? 3 - col : 2 * cx - 5 - col;     // symmetric mirror
+    vsign = -1.0;                                    // rhov negative-symmetric
+    copy_w = false;                                  // rhow not padded on walls
   }
-  if (srcc >= 0) {
-    for (int j = 2; j < cy - 2; ++j) {
-      const long a = (long)j * cx + col, b = (long)j * cx + srcc;
-      rho[a] = rho[b];
-      rhou[a] = rhou[b];
-      rhov[a] = vsign * rhov[b];
-      if (copy_w) rhow[a] = rhow[b];
-      rhoY[a] = rhoY[b];
-      rhoX[a] = rhoX[b];
-    }
+  const long a = (long)j * cx + col, b = (long)j * cx + srcc;
+  rho[a] = rho[b];
+  rhou[a] = rhou[b];
+  rhov[a] = vsign * rhov[b];
+  if (copy_w) rhow[a] = rhow[b];
+  rhoY[a] = rhoY[b];
+  rhoX[a] = rhoX[b];
+}
+
+// Threads 0 .. cx-1: column ``col``'s gravity-axis images (rows 0, 1, cy-2,
+// cy-1), after the x-axis fill of the inner rows those images read (2, 3,
+// cy-4, cy-3) when col is an x-ghost column.  Threads cx .. cx + 4 (cy-4) - 1:
+// the x-axis fill of the other inner rows of the 4 ghost columns, one row
+// each.  No thread reads a value another thread writes: the fill reads inner
+// columns, the images read their own column.
+__global__ void k_bc_cells(Dims2 d, BcParams bp, double* rho, double* rhou, double* rhov,
+                           double* rhow, double* rhoY, double* rhoX, const double* rhoY0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int cx = d.cx, cy = d.cy;
+  auto image_source_row = [&](int j) { return j == 2 || j == 3 || j == cy - 4 || j == cy - 3; };
+  if (t >= cx) {
+    const int u = t - cx, g = u & 3, j = 2 + (u >> 2);
+    if (j >= cy - 2 || image_source_row(j)) return;
+    const int col = g < 2 ? g : cx - 4 + g;
+    x_ghost_row(rho, rhou, rhov, rhow, rhoY, rhoX, cx, col, j, bp);
+    return;
+  }
+  const int col = t;
+  if (col < 2 || col >= cx - 2) {
+    const int rows[4] = {2, 3, cy - 4, cy - 3};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (rows[k] >= 2 && rows[k] < cy - 2) x_ghost_row(rho, rhou, rhov, rhow, rhoY, rhoX, cx, col, rows[k], bp);
   }
   // bottom (direction +1): image 1 <- source 2, image 0 <- source 3
   y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, 1, 2, 2, 1.0, bp);
@@ -263,7 +285,8 @@ extern "C" int pb_bc_cells(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* ph
   const double gm1 = phys->gamm - 1.0;
   BcParams bp{grid->bc_x == PB_BC_PERIODIC, phys->is_compressible == 1, (gm1 / phys->gamm) * phys->g,
               0.5 * grid->dy, gm1, 1.0 / gm1};
-  hipLaunchKernelGGL(k_bc_cells, dim3(ceil_div(d.cx, 64)), dim3(64), 0, ctx->stream, d, bp,
+  const int nthr = d.cx + 4 * (d.cy - 4);
+  hipLaunchKernelGGL(k_bc_cells, dim3(ceil_div(nthr, 256)), dim3(256), 0, ctx->stream, d, bp,
                      fields[0], fields[1], fields[2], fields[3], fields[4], fields[5], rhoY0_bg);
   PB_LAUNCH_CHECK(ctx);
   return PB_OK;

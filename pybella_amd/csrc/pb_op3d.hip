@@ -251,27 +251,33 @@ __device__ __forceinline__ void write_partials3(double (&acc)[NV], double* part)
     for (int v = 0; v < NV; ++v) part[(long)bid3() * NV + v] = acc[v];
 }
 
-// one-block reduction of the partial records [b][NV] in block order
+// threads of the one-block scalar kernels: one record per thread up to 1024 blocks
+constexpr int NRT = 1024;
+
+// One-block reduction of the partial records [b][NV] (NRT threads): thread t
+// adds records t, t + NRT, ... in order, then the block sum takes the wave
+// sums and the per-wave values in wave order.  The first record is loaded
+// unconditionally (clamped to the last one) and zeroed after, so its load is
+// not held behind a branch; first_record issues it, sum_records finishes.
+// The result is valid in thread 0.
 template <int NV>
-__device__ __forceinline__ void reduce_records(const double* part, int nb, double (&out)[NV]) {
-  __shared__ double sh[NV][256];
-  double s[NV];
+__device__ __forceinline__ void first_record(const double* part, int nb, double (&out)[NV]) {
+  const int b0 = min((int)threadIdx.x, nb - 1);
 #pragma unroll
-  for (int v = 0; v < NV; ++v) s[v] = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 256)
+  for (int v = 0; v < NV; ++v) out[v] = part[(long)b0 * NV + v];
+}
+
+template <int NV>
+__device__ __forceinline__ void sum_records(const double* part, int nb, double (&out)[NV]) {
+  __shared__ double sh[NV * (NRT / kWave)];
+  const int t = threadIdx.x;
+  if (t >= nb)
 #pragma unroll
-    for (int v = 0; v < NV; ++v) s[v] += part[(long)b * NV + v];
+    for (int v = 0; v < NV; ++v) out[v] = 0.0;
+  for (int b = t + NRT; b < nb; b += NRT)
 #pragma unroll
-  for (int v = 0; v < NV; ++v) sh[v][threadIdx.x] = s[v];
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off)
-#pragma unroll
-      for (int v = 0; v < NV; ++v) sh[v][threadIdx.x] += sh[v][threadIdx.x + off];
-    __syncthreads();
-  }
-#pragma unroll
-  for (int v = 0; v < NV; ++v) out[v] = sh[v][0];
+    for (int v = 0; v < NV; ++v) out[v] += part[(long)b * NV + v];
+  block_sum_w<NV, NRT / kWave>(out, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -317,9 +323,11 @@ __global__ void k3_init(long n, const T* b, T* r, T* rt, T* p, T* v, T* t, T* x,
   write_partials3<1, 4>(acc, part);   // blocks of 256 threads
 }
 
-__global__ void k3_s0(KS3* st, const double* part, int nb, double tol, int maxiter, int legacy) {
+__global__ __launch_bounds__(NRT) void k3_s0(KS3* st, const double* part, int nb, double tol, int maxiter,
+                                             int legacy) {
   double s[1];
-  reduce_records<1>(part, nb, s);
+  first_record<1>(part, nb, s);
+  sum_records<1>(part, nb, s);
   if (threadIdx.x == 0) {
     KS3 k{};
     k.rho = s[0];
@@ -368,18 +376,19 @@ __global__ __launch_bounds__(NT3) void k3_iter1(Op3<T> o, const KS3* st, const T
   write_partials3<1>(acc, part);
 }
 
-__global__ void k3_s1(KS3* st, const double* part, int nb) {
-  if (st->status != S3_RUN) return;
+// The state words each scalar kernel needs are read at entry, with the
+// partial records, so their loads overlap; thread 0 stores the fields it
+// changes.
+__global__ __launch_bounds__(NRT) void k3_s1(KS3* st, const double* part, int nb) {
   double s[1];
-  reduce_records<1>(part, nb, s);
+  first_record<1>(part, nb, s);
+  const int status = st->status;
+  const double rho = st->rho;
+  if (status != S3_RUN) return;
+  sum_records<1>(part, nb, s);
   if (threadIdx.x == 0) {
-    KS3 k = *st;
-    if (s[0] == 0.0 || !isfinite(s[0])) {
-      k.status = S3_BRK_OMEGA;
-    } else {
-      k.alpha = k.rho / s[0];
-    }
-    *st = k;
+    if (s[0] == 0.0 || !isfinite(s[0])) st->status = S3_BRK_OMEGA;
+    else st->alpha = rho / s[0];
   }
 }
 
@@ -406,35 +415,38 @@ __global__ __launch_bounds__(NT3) void k3_iter2(Op3<T> o, const KS3* st, const T
   write_partials3<5>(acc, part);
 }
 
-__global__ void k3_s2(KS3* st, const double* part, int nb) {
-  if (st->status != S3_RUN) return;
+__global__ __launch_bounds__(NRT) void k3_s2(KS3* st, const double* part, int nb) {
   double s[5];
-  reduce_records<5>(part, nb, s);
+  first_record<5>(part, nb, s);
+  const int status = st->status, iter0 = st->iter, maxiter = st->maxiter;
+  const double atol = st->atol, rho = st->rho, alpha = st->alpha;
+  if (status != S3_RUN) return;
+  sum_records<5>(part, nb, s);
   if (threadIdx.x == 0) {
-    KS3 k = *st;
     const double ss = s[0], ts = s[1], tt = s[2], rts = s[3], rtt = s[4];
-    if (sqrt(ss) <= k.atol) {
-      k.status = S3_P_HALF;     // x += alpha p
-      k.rnorm = sqrt(ss);
+    if (sqrt(ss) <= atol) {
+      st->status = S3_P_HALF;     // x += alpha p
+      st->rnorm = sqrt(ss);
     } else if (tt == 0.0 || !isfinite(tt)) {
-      k.status = S3_BRK_OMEGA;
+      st->status = S3_BRK_OMEGA;
     } else {
       const double om = ts / tt;
       const double rho_new = rts - om * rtt;
       const double rr = fmax(ss - 2.0 * om * ts + om * om * tt, 0.0);
-      k.omega = om;
-      k.iter += 1;
-      k.rnorm = sqrt(rr);
-      if (k.rnorm <= k.atol) k.status = S3_P_CONV;
-      else if (fabs(om) < eps2()) k.status = S3_P_BRK_OMEGA;
-      else if (k.iter >= k.maxiter) k.status = S3_P_MAXIT;
-      else if (fabs(rho_new) < eps2()) k.status = S3_P_BRK_RHO;
+      const int iter = iter0 + 1;
+      const double rnorm = sqrt(rr);
+      st->omega = om;
+      st->iter = iter;
+      st->rnorm = rnorm;
+      if (rnorm <= atol) st->status = S3_P_CONV;
+      else if (fabs(om) < eps2()) st->status = S3_P_BRK_OMEGA;
+      else if (iter >= maxiter) st->status = S3_P_MAXIT;
+      else if (fabs(rho_new) < eps2()) st->status = S3_P_BRK_RHO;
       else {
-        k.beta = (rho_new / k.rho) * (k.alpha / om);
-        k.rho = rho_new;
+        st->beta = (rho_new / rho) * (alpha / om);
+        st->rho = rho_new;
       }
     }
-    *st = k;
   }
 }
 
@@ -639,7 +651,7 @@ int krylov3(pb_op3d* op, const T* b, double tol, int maxiter, int legacy, int ch
   T* x = (T*)op->x[P];
   hipLaunchKernelGGL(k3_init<T>, eg, dim3(256), 0, s, n, b, (T*)op->r[P][0], rt, (T*)op->p[P][0], (T*)op->v[P][0], t,
                      x, op->part);
-  hipLaunchKernelGGL(k3_s0, dim3(1), dim3(256), 0, s, op->st, (const double*)op->part, (int)eg.x, tol, maxiter,
+  hipLaunchKernelGGL(k3_s0, dim3(1), dim3(NRT), 0, s, op->st, (const double*)op->part, (int)eg.x, tol, maxiter,
                      legacy);
   PB_LAUNCH_CHECK(op->ctx);
   // an even chunk starts every replay at the same buffer parity, so one
@@ -652,10 +664,10 @@ int krylov3(pb_op3d* op, const T* b, double tol, int maxiter, int legacy, int ch
       hipLaunchKernelGGL(k3_iter1<T>, mg, kBlk3, 0, s, o, (const KS3*)op->st, (const T*)op->r[P][a],
                          (const T*)op->p[P][a], (const T*)op->v[P][a], (const T*)t, (const T*)rt, (T*)op->r[P][bset],
                          (T*)op->p[P][bset], (T*)op->v[P][bset], x, op->part);
-      hipLaunchKernelGGL(k3_s1, dim3(1), dim3(256), 0, s, op->st, (const double*)op->part, nbm);
+      hipLaunchKernelGGL(k3_s1, dim3(1), dim3(NRT), 0, s, op->st, (const double*)op->part, nbm);
       hipLaunchKernelGGL(k3_iter2<T>, mg, kBlk3, 0, s, o, (const KS3*)op->st, (const T*)op->r[P][bset],
                          (const T*)op->v[P][bset], (const T*)rt, t, op->part);
-      hipLaunchKernelGGL(k3_s2, dim3(1), dim3(256), 0, s, op->st, (const double*)op->part, nbm);
+      hipLaunchKernelGGL(k3_s2, dim3(1), dim3(NRT), 0, s, op->st, (const double*)op->part, nbm);
     }
   };
   const char* no_graph = std::getenv("PYBELLA_AMD_NO_GRAPH");

@@ -43,16 +43,27 @@ def _torch():
 
 
 def to_dev(a):
-    """Reference-layout [x, y] float64 numpy array -> device [y][x] tensor."""
+    """Reference-layout [x, y] float64 numpy array -> device [y][x] tensor.
+    The host array crosses as it is laid out (one contiguous copy); the
+    transpose runs on the device."""
     torch = _torch()
-    return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)).to("cuda")
+    h = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda")
+    return h.permute(*reversed(range(h.dim()))).contiguous()
 
 
 def from_dev(t, out=None):
-    """Device [y][x] tensor -> reference-layout numpy array (written into ``out``)."""
-    a = t.detach().cpu().numpy().T
+    """Device [y][x] tensor -> reference-layout numpy array (written into
+    ``out``).  The transpose runs on the device; a C-contiguous float64 ``out``
+    receives the copy directly."""
+    torch = _torch()
+    d = t.detach()
+    d = d.permute(*reversed(range(d.dim()))).contiguous()
+    if out is not None and out.dtype == np.float64 and out.flags.c_contiguous and out.shape == tuple(d.shape):
+        torch.from_numpy(out).copy_(d)
+        return out
+    a = d.cpu().numpy()
     if out is None:
-        return np.ascontiguousarray(a)
+        return a
     out[...] = a
     return out
 
@@ -193,11 +204,77 @@ def operator_for(node, ud) -> Operator2D:
 _CELL_FIELDS = ("rho", "rhou", "rhov", "rhow", "rhoY", "rhoX")
 
 
+class _Staging:
+    """Pinned host buffers for the drop-in's host <-> device copies, kept
+    across calls.  A batch of copies goes host array -> pinned buffer (a
+    multi-threaded host copy) -> device (one DMA per array), or the reverse,
+    and ends with a synchronization of the stream, so every buffer is free
+    again when the next batch starts.  Transposes run on the device."""
+
+    def __init__(self):
+        self.bufs = {}
+        self.used = {}
+
+    def _get(self, shape):
+        torch = _torch()
+        k = tuple(shape)
+        i = self.used.get(k, 0)
+        self.used[k] = i + 1
+        lst = self.bufs.setdefault(k, [])
+        if i == len(lst):
+            lst.append(torch.empty(k, dtype=torch.float64, pin_memory=True))
+        return lst[i]
+
+    def upload(self, arrays):
+        """Reference-layout host arrays -> device tensors in [y][x] layout."""
+        torch = _torch()
+        self.used = {}
+        out = []
+        for a in arrays:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            p = self._get(a.shape)
+            p.copy_(torch.from_numpy(a))
+            h = torch.empty(a.shape, dtype=torch.float64, device="cuda")
+            h.copy_(p, non_blocking=True)
+            out.append(h.permute(*reversed(range(h.dim()))).contiguous())
+        torch.cuda.current_stream().synchronize()
+        return out
+
+    def download(self, pairs):
+        """(device [y][x] tensor, host array or None) pairs -> the host arrays
+        in reference layout: written into the given array, else new."""
+        torch = _torch()
+        self.used = {}
+        staged = []
+        for t, out in pairs:
+            d = t.detach()
+            d = d.permute(*reversed(range(d.dim()))).contiguous()
+            p = self._get(d.shape)
+            p.copy_(d, non_blocking=True)
+            staged.append((d, p, out))
+        torch.cuda.current_stream().synchronize()
+        res = []
+        for _, p, out in staged:
+            if out is not None and out.dtype == np.float64 and out.flags.c_contiguous and out.shape == tuple(p.shape):
+                torch.from_numpy(out).copy_(p)
+            elif out is not None:
+                out[...] = p.numpy()
+            else:
+                out = p.numpy().copy()
+            res.append(out)
+        return res
+
+
+_staging = _Staging()
+
+
 class _DevState:
     """Device copies of a Vars object's six cell fields."""
 
-    def __init__(self, Sol):
-        self.f = {k: to_dev(getattr(Sol, k)) for k in _CELL_FIELDS}
+    def __init__(self, Sol=None, tensors=None):
+        if tensors is None:
+            tensors = [to_dev(getattr(Sol, k)) for k in _CELL_FIELDS]
+        self.f = dict(zip(_CELL_FIELDS, tensors))
 
     def ptrs(self):
         return (_abi.VP * 6)(*[_abi.VP(self.f[k].data_ptr()) for k in _CELL_FIELDS])
@@ -205,6 +282,9 @@ class _DevState:
     def download(self, Sol):
         for k in _CELL_FIELDS:
             from_dev(self.f[k], getattr(Sol, k))
+
+    def pairs(self, Sol):
+        return [(self.f[k], getattr(Sol, k)) for k in _CELL_FIELDS]
 
 
 def _bc_cells(ctx, grid, phys, ds: _DevState, rhoY0):
@@ -216,11 +296,10 @@ class _Projection:
     """Device state of one projection step between assembly and the update."""
 
 
-def _prepare(Sol, mpv, elem, node, ud, th, dt, Sol0, op):
-    """second_projection.py:113-214 on the device: ghost cells, the correction
-    with the old p2, divergence / dt, the regime blend, operator assembly and
-    the scaled right-hand side.  Returns a _Projection holding every device
-    array the solve and the update need."""
+def _upload(Sol, mpv, elem, node, ud, th, dt, Sol0, op):
+    """Host -> device copies of what a projection step reads (Sol, Sol0, the
+    old p2, the background column) and its device work arrays.  Returns a
+    _Projection for _project_prepare."""
     torch = _torch()
     if ud.is_compressible == 0 and ud.is_ArakawaKonor:
         raise NotImplementedError("Arakawa-Konor branch (second_projection.py:145-147) is not runnable "
@@ -229,21 +308,41 @@ def _prepare(Sol, mpv, elem, node, ud, th, dt, Sol0, op):
         raise NotImplementedError("LAMB_BDRY operator variant is not part of this path")
     pr = _Projection()
     pr.grid = grid = grid_of(node, ud)
-    pr.phys = phys = phys_of(ud, th)
+    pr.phys = phys_of(ud, th)
     pr.op = op
-    pr.ctx = ctx = op.ctx
+    pr.ctx = op.ctx
     pr.dt = dt
-    ctx.sync_stream()
+    pr.compressible = int(ud.is_compressible)
+    pr.compressibility = float(ud.compressibility)
+    op.ctx.sync_stream()
     nx, ny = grid.nx, grid.ny
-    pr.dS = to_dev(dSdy_cells(mpv, elem, node))
-    pr.rhoY0 = torch.from_numpy(np.ascontiguousarray(mpv.HydroState.rhoY0, dtype=np.float64)).to("cuda")
-    pr.sol = sol = _DevState(Sol)
-    pr.sol0 = sol0 = _DevState(Sol0) if Sol0 is not None else None
-    S = sol0 if sol0 is not None else sol
-    pr.p2n = p2n = to_dev(mpv.p2_nodes)
+    host = [dSdy_cells(mpv, elem, node), mpv.HydroState.rhoY0, mpv.p2_nodes]
+    host += [getattr(Sol, k) for k in _CELL_FIELDS]
+    if Sol0 is not None:
+        host += [getattr(Sol0, k) for k in _CELL_FIELDS]
+    dev = _staging.upload(host)
+    pr.dS, pr.rhoY0, pr.p2n = dev[:3]
+    pr.sol = _DevState(tensors=dev[3:9])
+    pr.sol0 = _DevState(tensors=dev[9:15]) if Sol0 is not None else None
     cells = lambda: torch.empty(ny + 3, nx + 3, dtype=torch.float64, device="cuda")
     pr.u, pr.v, pr.w = cells(), cells(), cells()
+    pr.rhs = torch.empty(ny + 2, nx + 2, dtype=torch.float64, device="cuda")
+    pr.wplus0, pr.wplus1, pr.nu_c = cells(), cells(), cells()
+    pr.wcenter = torch.empty(ny + 2, nx + 2, dtype=torch.float64, device="cuda")
+    pr.diag_inv = torch.empty_like(pr.wcenter)
+    pr.b = torch.empty(ny, nx, dtype=torch.float64, device="cuda")
+    return pr
+
+
+def _project_prepare(pr):
+    """second_projection.py:113-214 on the device arrays of ``pr``: ghost
+    cells, the correction with the old p2, divergence / dt, the regime blend,
+    operator assembly and the scaled right-hand side (into pr.b)."""
+    ctx, grid, phys, op, dt = pr.ctx, pr.grid, pr.phys, pr.op, pr.dt
+    sol, sol0, p2n = pr.sol, pr.sol0, pr.p2n
+    S = sol0 if sol0 is not None else sol
     F = sol.f
+    ctx.sync_stream()
     # :113-118 ghost cells + coefficients' state
     _bc_cells(ctx, grid, phys, S, pr.rhoY0)
     # :130-132 correction with the old p2, then ghost cells
@@ -254,37 +353,33 @@ def _prepare(Sol, mpv, elem, node, ud, th, dt, Sol0, op):
                                           _abi.ptr(pr.u), _abi.ptr(pr.v), _abi.ptr(pr.w)), "pb_correction_nodes")
     _bc_cells(ctx, grid, phys, sol, pr.rhoY0)
     # :134-139 divergence / dt
-    rhs = torch.empty(ny + 2, nx + 2, dtype=torch.float64, device="cuda")
     ctx.check(ctx.lib.pb_divergence_nodes(ctx.h, C.byref(grid), float(dt), _abi.ptr(F["rho"]),
                                           _abi.ptr(F["rhou"]), _abi.ptr(F["rhov"]), _abi.ptr(F["rhow"]),
-                                          _abi.ptr(F["rhoY"]), _abi.ptr(rhs)), "pb_divergence_nodes")
-    c = float(ud.compressibility)
-    if ud.is_compressible == 0 and c not in (0.0, 1.0):
-        rhs = c * rhs + (1.0 - c) * rhs          # :150, exact elementwise blend
-    pr.rhs = rhs
+                                          _abi.ptr(F["rhoY"]), _abi.ptr(pr.rhs)), "pb_divergence_nodes")
+    c = pr.compressibility
+    if pr.compressible == 0 and c not in (0.0, 1.0):
+        pr.rhs.copy_(c * pr.rhs + (1.0 - c) * pr.rhs)          # :150, exact elementwise blend
     # :115/:118, :141-153, :165-175 operator assembly (wcenter regime-scaled)
-    pr.wplus0, pr.wplus1, pr.nu_c = cells(), cells(), cells()
-    pr.wcenter = torch.empty(ny + 2, nx + 2, dtype=torch.float64, device="cuda")
-    pr.diag_inv = torch.empty_like(pr.wcenter)
-    op.assemble(phys, dt, S.f["rho"], S.f["rhoY"], F["rho"], F["rhoY"], pr.dS, pr.wplus0, pr.wplus1, pr.nu_c,
-                pr.wcenter, pr.diag_inv)
+    op.assemble(pr.phys, dt, S.f["rho"], S.f["rhoY"], F["rho"], F["rhoY"], pr.dS, pr.wplus0, pr.wplus1,
+                pr.nu_c, pr.wcenter, pr.diag_inv)
     # :170, :214 scaled right-hand side
-    pr.b = op.scale_rhs(rhs)
+    op.scale_rhs(pr.rhs, pr.b)
     return pr
 
 
-def _finish(pr, x, info, Sol, Sol0, mpv):
-    """second_projection.py:235-277: counters, scatter of the solution, the
-    correction with dp2, the Exner update, ghost cells; then the side effects
-    are written back to the host objects."""
-    global total_iter, total_calls
+def _prepare(Sol, mpv, elem, node, ud, th, dt, Sol0, op):
+    """Upload and device preparation of one projection step: a _Projection
+    holding every device array the solve and the update need."""
+    return _project_prepare(_upload(Sol, mpv, elem, node, ud, th, dt, Sol0, op))
+
+
+def _project_update(pr, x):
+    """second_projection.py:241-277 on the device: scatter of the solution,
+    the correction with dp2, the Exner update, ghost cells."""
     torch = _torch()
     ctx, grid, phys, F = pr.ctx, pr.grid, pr.phys, pr.sol.f
     nx, ny = grid.nx, grid.ny
-    total_iter += info["iters"]
-    total_calls += 1
-    logging.info(info["iters"])
-    logging.info("Total calls to BiCGStab routine = %i, total iterations = %i" % (total_calls, total_iter))
+    ctx.sync_stream()
     p2_full = torch.empty(ny + 4, nx + 4, dtype=torch.float64, device="cuda")
     ctx.check(ctx.lib.pb_scatter_update_p2(ctx.h, C.byref(grid), _abi.ptr(x), _abi.ptr(p2_full),
                                            _abi.ptr(pr.p2n)), "pb_scatter_update_p2")
@@ -294,17 +389,21 @@ def _finish(pr, x, info, Sol, Sol0, mpv):
                                           _abi.ptr(pr.dS), _abi.ptr(pr.u), _abi.ptr(pr.v), _abi.ptr(pr.w)),
               "pb_correction_nodes")
     _bc_cells(ctx, grid, phys, pr.sol, pr.rhoY0)
-    torch.cuda.synchronize()
-    pr.sol.download(Sol)
-    if pr.sol0 is not None:
-        pr.sol0.download(Sol0)
-    from_dev(pr.p2n, mpv.p2_nodes)
-    mpv.rhs[...] = from_dev(pr.rhs)
-    mpv.wplus[0][...] = from_dev(pr.wplus0)
-    mpv.wplus[1][...] = from_dev(pr.wplus1)
-    mpv.wcenter = from_dev(pr.wcenter)
-    mpv.nu_c = from_dev(pr.nu_c)
-    mpv.u, mpv.v, mpv.w = from_dev(pr.u), from_dev(pr.v), from_dev(pr.w)
+
+
+def _finish(pr, x, info, Sol, Sol0, mpv):
+    """second_projection.py:235-277: counters, the device update, then the
+    side effects are written back to the host objects."""
+    global total_iter, total_calls
+    total_iter += info["iters"]
+    total_calls += 1
+    logging.info(info["iters"])
+    logging.info("Total calls to BiCGStab routine = %i, total iterations = %i" % (total_calls, total_iter))
+    _project_update(pr, x)
+    pairs = pr.sol.pairs(Sol) + (pr.sol0.pairs(Sol0) if pr.sol0 is not None else [])
+    pairs += [(pr.p2n, mpv.p2_nodes), (pr.rhs, mpv.rhs), (pr.wplus0, mpv.wplus[0]), (pr.wplus1, mpv.wplus[1]),
+              (pr.wcenter, None), (pr.nu_c, None), (pr.u, None), (pr.v, None), (pr.w, None)]
+    mpv.wcenter, mpv.nu_c, mpv.u, mpv.v, mpv.w = _staging.download(pairs)[-5:]
 
 
 def euler_backward_non_advective_impl_part(Sol, mpv, elem, node, ud, th, t, dt, alpha_diff, Sol0=None,

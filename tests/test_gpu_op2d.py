@@ -85,15 +85,20 @@ def test_matvec_large_1025x513():
     assert float(op.matvec(chk).abs().max()) < 1e-9 * float(np.max(np.abs(olp.lap2d(x, op_o))))
 
 
+@pytest.mark.parametrize("shape", [(41, 21), (1025, 513), (3, 5)])
 @pytest.mark.parametrize("periodic_x", [True, False])
-def test_bc_cells_and_ghostnodes(periodic_x):
+def test_bc_cells_and_ghostnodes(periodic_x, shape):
+    """Ghost cells and ghost nodes against the oracle: one block of threads
+    (41x21), many blocks (1025x513: 1028 column threads and 2048 ghost-row
+    threads) and the smallest grid the entry point takes (3x5, whose image
+    source rows coincide)."""
     import torch
     from pybella_amd import _abi
     import ctypes as C
     sp = _sp()
     for comp in (False, True):
-        print(f"case bc periodic_x={periodic_x} compressible={comp}", flush=True)
-        Sol, mpv, elem, node, ud, th = W.random_momentum_state(41, 21, seed=9, compressible=comp)
+        print(f"case bc {shape} periodic_x={periodic_x} compressible={comp}", flush=True)
+        Sol, mpv, elem, node, ud, th = W.random_momentum_state(*shape, seed=9, compressible=comp)
         from pybella_amd.state import hydrostatic_state
         hydrostatic_state(mpv, elem, node, th, ud)
         if not periodic_x:
