@@ -255,12 +255,14 @@ class _Staging:
         torch.cuda.current_stream().synchronize()
         res = []
         for _, p, out in staged:
-            if out is not None and out.dtype == np.float64 and out.flags.c_contiguous and out.shape == tuple(p.shape):
+            if out is None:
+                # a new array, as the reference rebinds these fields; the
+                # multi-threaded copy also spreads its first-touch page faults
+                out = np.empty(tuple(p.shape))
+            if out.dtype == np.float64 and out.flags.c_contiguous and out.shape == tuple(p.shape):
                 torch.from_numpy(out).copy_(p)
-            elif out is not None:
-                out[...] = p.numpy()
             else:
-                out = p.numpy().copy()
+                out[...] = p.numpy()
             res.append(out)
         return res
 
