@@ -80,12 +80,24 @@ __device__ __forceinline__ double eps2() { return 4.930380657631324e-32; }
 //     barrier (one step ahead would cost the registers of a third block per
 //     CU), pre(g) where it is used;
 //   0: where they are used (kernels whose accumulators leave no registers).
-template <typename T, int NV, int PF = (sizeof(T) == 4 ? 2 : 1), class In, class PreFn, class Out>
+//   UNR (with PF 2): the march unrolled by two over two register sets, so the
+//     loads issued one step ahead are never copied between sets (a copy
+//     waits for them at the end of the step that issued them); it costs
+//     registers, and pays in the five-dot launch only.
+//   GUARD: plane and weight loads only by the threads that hold a value.
+//     Otherwise (unless PF 2 without UNR) every thread loads, with clamped
+//     indices, so no load sits behind a branch: faster for the matvec and
+//     the five-dot launch, slower for launch 1 (measured).
+template <typename T, int NV, int PF = (sizeof(T) == 4 ? 2 : 1), bool UNR = false, bool GUARD = false,
+          class In, class PreFn, class Out>
 __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out out,
                                        double (&acc)[NV > 0 ? NV : 1]) {
   using PreT = decltype(pre(0L));
   constexpr bool AHEAD = PF == 2;   // node operands one step ahead
-  __shared__ T P[2][Y3 + 2][X3 + 2];
+  constexpr bool UNCOND = !GUARD && (!AHEAD || UNR);   // branch-free plane and weight loads
+  // three node-plane slots: plane k+1 is stored while threads that are still
+  // in step k-1's output stage read plane k-1, so it needs a slot of its own
+  __shared__ T P[3][Y3 + 2][X3 + 2];
   __shared__ T F[3][Y3 + 1][X3 + 1];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tid3();
   // XCD-aware block order: hardware block h runs on XCD h mod 8; each XCD
@@ -103,12 +115,18 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
   // plane NC weights, at most CPT per thread
   constexpr int NQ = (Y3 + 2) * (X3 + 2), QPT = (NQ + NT3 - 1) / NT3;
   constexpr int NC = (Y3 + 1) * (X3 + 1), CPT = (NC + NT3 - 1) / NT3;
-  auto fetch = [&](int k, T (&r)[QPT]) {   // global loads of plane k into registers
+  // Global loads of plane k into registers.  With UNCOND every thread loads
+  // QPT values (indices past the plane clamped to its last value, not
+  // stored), so no load sits behind a branch and the wait counts stay
+  // static; otherwise only the threads with a value load (fewer loads, which
+  // the one-step-ahead FP32 marches prefer).
+  auto fetch = [&](int k, T (&r)[QPT]) {
     const long kk = wrap3(k, o.nz, o.pz);
 #pragma unroll
     for (int e = 0; e < QPT; ++e) {
-      const int q = tid + e * NT3;
-      if (q < NQ) {
+      const int q0 = tid + e * NT3;
+      if (UNCOND || q0 < NQ) {
+        const int q = min(q0, NQ - 1);
         const int jj = q / (X3 + 2), ii = q - jj * (X3 + 2);
         const int gj = wrap3(y0 - 1 + jj, ny, o.py), gi = wrap3(x0 - 1 + ii, nx, o.px);
         r[e] = in((kk * ny + gj) * nx + gi);
@@ -135,7 +153,12 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
       const int cj = q / (X3 + 1), ci = q - cj * (X3 + 1);
       const int cy = y0 + cj, cx = x0 + ci;
       const bool ok = q < NC && cy <= ny && cx <= nx;
-      hw[e] = ok ? o.h[(ck * (ny + 1) + cy) * (nx + 1) + cx] : T(0);
+      if (UNCOND) {
+        const T v = o.h[ok ? (ck * (ny + 1) + cy) * (nx + 1) + cx : 0];   // unconditional load
+        hw[e] = ok ? v : T(0);
+      } else {
+        hw[e] = ok ? o.h[(ck * (ny + 1) + cy) * (nx + 1) + cx] : T(0);
+      }
     }
   };
   // cell fluxes between the node planes in slots s0, s1 of cell plane k+1;
@@ -181,30 +204,36 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
   load_plane(kb, 1);
   T nxt[QPT];
   fetch(kb + 1, nxt);       // plane kb+1 in flight during the first cell plane
-  T h0[CPT], hw[CPT];
+  // two register sets for a step's cell weights and node operands, used in
+  // turn (the march is unrolled by two), so a set loaded one step ahead is
+  // never copied: a copy would wait for the loads at the end of the step
+  // that issued them
+  T h0[CPT], hwA[CPT], hwB[CPT];
 #pragma unroll
-  for (int e = 0; e < CPT; ++e) hw[e] = T(0);
+  for (int e = 0; e < CPT; ++e) hwA[e] = hwB[e] = T(0);
   if (PF != 0) fetch_cells(kb - 1, h0);
-  if (AHEAD) fetch_cells(kb, hw);   // cell plane of step kb
-  NodeOps nd;
-  if (AHEAD) fetch_node(kb, nd);   // node operands of step kb
+  NodeOps ndA, ndB;
+  if (AHEAD) {
+    fetch_cells(kb, hwA);   // cell plane of step kb
+    fetch_node(kb, ndA);    // node operands of step kb
+  }
   __syncthreads();
   cells(kb - 1, h0, 0, 1);
   __syncthreads();
   T Dp, Zp;
   dz(Dp, Zp);
-  for (int k = kb; k < ke; ++k) {
-    const int sk = (k - kb + 1) & 1, sn = sk ^ 1;
-    __syncthreads();          // F and the slot of plane k-1 are free
+  // slots: plane kb-1 in 0, kb in 1, kb+1 in 2, kb+2 in 0, ...  No barrier
+  // before the store: its slot held plane k-2, whose last reads (step k-2)
+  // precede step k-1's barriers, and F's last reads (step k-1's dz) precede
+  // this step's first barrier, which orders them before cells(k) writes F
+  int sk = 1, sn = 2;
+  auto step = [&](int k, T (&hw)[CPT], NodeOps& nd, T (&hwn)[CPT], NodeOps& ndn) {
     store(sn, nxt);           // plane k+1
-    T hw1[CPT];
-    NodeOps nd1;
-    const bool more = k + 1 < ke;
-    if (more) {               // step k+1's loads in flight during this step's compute
+    if (k + 1 < ke) {         // step k+1's loads in flight during this step's compute
       fetch(k + 2, nxt);
       if (AHEAD) {
-        fetch_cells(k + 1, hw1);
-        fetch_node(k + 1, nd1);
+        fetch_cells(k + 1, hwn);
+        fetch_node(k + 1, ndn);
       }
     }
     if (PF == 1) {            // this step's own loads, in flight during the barrier
@@ -233,11 +262,23 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
     }
     Dp = Dn;
     Zp = Zn;
-    if (AHEAD && more) {
-#pragma unroll
-      for (int e = 0; e < CPT; ++e) hw[e] = hw1[e];
-      nd = nd1;
+    sk = sn;
+    sn = sn == 2 ? 0 : sn + 1;
+  };
+  if constexpr (AHEAD && UNR) {
+    for (int k = kb; k < ke; k += 2) {
+      step(k, hwA, ndA, hwB, ndB);
+      if (k + 1 < ke) step(k + 1, hwB, ndB, hwA, ndA);
     }
+  } else if constexpr (AHEAD) {   // set B (next step's) is copied into set A
+    for (int k = kb; k < ke; ++k) {
+      step(k, hwA, ndA, hwB, ndB);
+#pragma unroll
+      for (int e = 0; e < CPT; ++e) hwA[e] = hwB[e];
+      ndA = ndB;
+    }
+  } else {                    // a step's loads are its own: one set suffices
+    for (int k = kb; k < ke; ++k) step(k, hwA, ndA, hwB, ndB);
   }
 }
 
@@ -356,7 +397,7 @@ __global__ __launch_bounds__(NT3) void k3_iter1(Op3<T> o, const KS3* st, const T
   struct Ops {   // this node's v, r, p, t, r~, x, loaded a plane step ahead
     T v, r, p, t, rt, x;
   };
-  march3<T, 1>(
+  march3<T, 1, (sizeof(T) == 4 ? 2 : 1), false, true>(
       o,
       [&](long g) {
         const T v = va[g];
@@ -393,14 +434,16 @@ __global__ __launch_bounds__(NRT) void k3_s1(KS3* st, const double* part, int nb
 }
 
 // launch 2: s = r - alpha v (on the fly), t = A s; partials s.s, t.s, t.t, rt.s, rt.t
+// six waves per SIMD keep three marching blocks on a CU (all 765 blocks of a
+// 257x129x129 launch resident at once): at most 80 VGPRs
 template <typename T>
-__global__ __launch_bounds__(NT3) void k3_iter2(Op3<T> o, const KS3* st, const T* r, const T* v, const T* rt, T* t,
+__global__ __launch_bounds__(NT3) __attribute__((amdgpu_waves_per_eu(6))) void k3_iter2(Op3<T> o, const KS3* st, const T* r, const T* v, const T* rt, T* t,
                                                 double* part) {
   if (st->status != S3_RUN) return;
   const T al = (T)st->alpha;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   // FP64: five FP64 accumulators; its step loads stay where they are used
-  march3<T, 5, sizeof(T) == 4 ? 2 : 0>(
+  march3<T, 5, sizeof(T) == 4 ? 2 : 0, true>(
       o, [&](long g) { return r[g] - al * v[g]; }, [&](long g) { return rt[g]; },
       [&](long g, T y, T s, T rtg, double(&a)[5]) {
         t[g] = y;
