@@ -480,11 +480,13 @@ def main():
     # the learned-preconditioned solve: network initial guess (+ damped Jacobi),
     # flexible GMRES for ``maxiter`` iterations, then plain BiCGStab (two-launch
     # iteration at the fast mode's tile) to the same true-residual tolerance
-    nn_cfg, nn_table = None, {}
+    nn_cfg, nn_table, fallback_after = None, {}, None
     if nn_net is not None:
         op.set_tile(tiles.get("f64fast", tiles[plain])["rows_per_thread"])
-        cands = [dict(maxiter=0, sweeps=2), dict(maxiter=0, sweeps=0), dict(maxiter=10, sweeps=2),
-                 dict(maxiter=40, sweeps=1)]
+        # the learned phase never runs past 2x the plain FP64 iteration count
+        # measured above; plain BiCGStab takes over from there
+        fallback_after = 2 * int(info64["iters"])
+        cands = [dict(maxiter=min(m, fallback_after), sweeps=sw) for m, sw in ((0, 2), (0, 0), (10, 2), (40, 1))]
         for c in cands:
             step("nn", c)
             r = rate_of(lambda: step("nn", c))
@@ -565,6 +567,7 @@ def main():
                                "psinc, seeded 64-mode RHS, true rel. resid <= 1e-8",
                    "solver": SOLVER_NAMES[mode],
                    "nn_config": nn_cfg,
+                   "nn_fallback_after_iters": fallback_after,
                    "iterations": infos[-1]["iters"], "refinements": infos[-1]["outer"],
                    "true_rel_resid": infos[-1]["true_rel_resid"],
                    "f64_iterations": info64["iters"],

@@ -29,9 +29,14 @@ total_iter = 0
 total_calls = 0
 
 # Solver settings used by the drop-in.  mode: "f64" (reference semantics:
-# FP64 BiCGStab, x0 = 0, legacy tolerance), "mixed" (FP32 Krylov in FP64
-# iterative refinement, stop on the FP64 true residual).
-solver_options = {"mode": "f64", "inner_tol": 1e-3, "max_outer": 12, "check_every": 32}
+# FP64 BiCGStab, x0 = 0, legacy tolerance), "f64fast" (two launches per
+# iteration), "mixed" (FP32 Krylov in FP64 iterative refinement, stop on the
+# FP64 true residual), "nn" (the learned preconditioner attached to the
+# grid's operator with operator_for(node, ud).attach_nn(net): nn_budget
+# flexible-GMRES iterations after the network's initial guess and nn_sweeps
+# damped Jacobi sweeps, then plain BiCGStab for up to ud.max_iterations).
+solver_options = {"mode": "f64", "inner_tol": 1e-3, "max_outer": 12, "check_every": 32,
+                  "nn_budget": 0, "nn_sweeps": 2}
 
 _MODES = {"f64": _abi.PB_SOLVE_F64, "f64fast": _abi.PB_SOLVE_F64_FAST, "mixed": _abi.PB_SOLVE_MIXED,
           "nn": _abi.PB_SOLVE_NN}
@@ -415,10 +420,14 @@ def euler_backward_non_advective_impl_part(Sol, mpv, elem, node, ud, th, t, dt, 
     op = operator_for(node, ud)
     pr = _prepare(Sol, mpv, elem, node, ud, th, dt, Sol0, op)
     mode = solver_options.get("mode", "f64")
-    x, info = op.solve(pr.b, tol=float(ud.tol), maxiter=int(ud.max_iterations), mode=mode,
-                       inner_tol=solver_options.get("inner_tol", 1e-3),
+    extra = {}
+    if mode == "nn":   # the learned phase has its own budget; ud.max_iterations caps the fallback
+        extra = dict(fallback_maxiter=int(ud.max_iterations), sweeps=int(solver_options.get("nn_sweeps", 2)))
+    x, info = op.solve(pr.b, tol=float(ud.tol),
+                       maxiter=int(solver_options.get("nn_budget", 0)) if mode == "nn" else int(ud.max_iterations),
+                       mode=mode, inner_tol=solver_options.get("inner_tol", 1e-3),
                        max_outer=solver_options.get("max_outer", 12),
-                       check_every=solver_options.get("check_every", 32))
+                       check_every=solver_options.get("check_every", 32), **extra)
     _finish(pr, x, info, Sol, Sol0, mpv)
     return info
 
