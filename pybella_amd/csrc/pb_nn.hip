@@ -82,12 +82,34 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
     const long src = ok ? (((ba * h + ys) * w + xs) * in_ctot + in_coff + 4 * kg) : 0;
     const int wt = flip ? 8 - tap : tap;
     const bf16* wrow = wg + ((long)wt * nout + (lane & 15)) * cin + 4 * kg;
-    for (int ci0 = 0; ci0 < cin; ci0 += 16) {
-      bf16x4 a = {0, 0, 0, 0};
-      if (ok) a = *(const bf16x4*)(in + src + ci0);
+    // Input-channel chunks of 16 in groups of four: a group's loads are all
+    // issued before its MFMAs (one memory latency per group, not per chunk).
+    // The input loads are unconditional (src is 0, a valid offset, outside
+    // the image; the value is zeroed after).  The MFMA order is that of a
+    // plain chunk-by-chunk loop, so the sums are bitwise the same.
+    const int nci = cin >> 4;
+    int c = 0;
+    for (; c + 4 <= nci; c += 4) {
+      bf16x4 a[4], b[4][NT];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = *(const bf16x4*)(in + src + (c + u) * 16);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) b[u][t] = *(const bf16x4*)(wrow + (long)(t0 + t) * 16 * cin + (c + u) * 16);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (!ok) a[u] = bf16x4{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u][t], acc[t], 0, 0, 0);
+      }
+    }
+    for (; c < nci; ++c) {
+      bf16x4 a = *(const bf16x4*)(in + src + c * 16);
+      if (!ok) a = bf16x4{0, 0, 0, 0};
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const bf16x4 b = *(const bf16x4*)(wrow + (long)(t0 + t) * 16 * cin + ci0);
+        const bf16x4 b = *(const bf16x4*)(wrow + (long)(t0 + t) * 16 * cin + c * 16);
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
       }
     }
