@@ -672,6 +672,23 @@ Acts* get_shape(pb_nn* nn, int nimg, int h, int w, bool train, int* rc) {
   return a;
 }
 
+// resident 256-thread blocks of k_conv3x3<NT> on the whole device (CUs x
+// blocks per CU at the kernel's occupancy), queried once per NT
+long conv_slots(int nt) {
+  static long slots[5] = {0, 0, 0, 0, 0};
+  if (slots[nt] == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    hipError_t e = hipErrorInvalidValue;
+    if (nt == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv3x3<1>, 256, 0);
+    else if (nt == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv3x3<2>, 256, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv3x3<4>, 256, 0);
+    slots[nt] = (e == hipSuccess && cus > 0 && per_cu > 0) ? (long)cus * per_cu : 1;
+  }
+  return slots[nt];
+}
+
 void conv(pb_nn* nn, Acts* a, int l, int level, const bf16* in, int in_ct, int in_co, bf16* out, int out_ct,
           int out_co, bf16* pre, float* out_f32, hipStream_t s) {
   const LayerSpec& L = kLayers[l];
@@ -679,15 +696,26 @@ void conv(pb_nn* nn, Acts* a, int l, int level, const bf16* in, int in_ct, int i
   const long npix = (long)a->nimg * a->lv[level].h * a->lv[level].w;
   const float* bias = nn->params + nn->boff[l];
   const int h = a->lv[level].h, w = a->lv[level].w;
-  // A wave walks 9 taps x cin/16 chunks of dependent loads; on small levels
-  // one output tile per block (grid y over the tiles) gives nt times the waves
-  // to hide that latency (the input loads repeat, from cache).
-  const bool split = npix < 65536 && nt > 1;
-  const dim3 grd((unsigned)((npix + 63) / 64), split ? nt : 1), blk(256);
+  // A wave walks 9 taps x cin/16 chunks of dependent loads.  On small levels
+  // the output tiles are split over the grid's y (fewer tiles per block, more
+  // waves to hide that latency; the input loads repeat, from cache), with the
+  // fewest tiles per block whose grid fits in one round of resident blocks:
+  // a second round of a few blocks would double the launch.
+  const long bx = (npix + 63) / 64;
+  int ntb = nt;
+  if (npix < 65536 && nt > 1) {
+    ntb = 1;
+    for (int c : {1, 2, 4})
+      if (c <= nt && nt % c == 0 && bx * (nt / c) <= conv_slots(c)) {
+        ntb = c;
+        break;
+      }
+  }
+  const dim3 grd((unsigned)bx, nt / ntb), blk(256);
 #define PB_CONV(NT)                                                                                              \
   hipLaunchKernelGGL(k_conv3x3<NT>, grd, blk, 0, s, in, in_ct, in_co, cin, nn->wf[l], 0, bias, L.act, out, out_ct, \
                      out_co, L.cout_v, pre, pad16(L.cout_v), out_f32, a->nimg, h, w)
-  switch (split ? 1 : nt) {
+  switch (ntb) {
     case 1: PB_CONV(1); break;
     case 2: PB_CONV(2); break;
     default: PB_CONV(4); break;
