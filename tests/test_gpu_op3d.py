@@ -70,6 +70,41 @@ def test_first_iterates(k):
     assert rel(x.cpu().numpy().ravel(), r.x) < 1e-9
 
 
+@pytest.mark.parametrize("check_every,k", [(2, 3), (2, 8), (4, 9), (32, 33)])
+def test_iterates_across_chunks(check_every, k):
+    """Each launch applies the scalar step of the launch before it, and a
+    one-block fold applies the last pending step at the end of every chunk of
+    iterations.  Iterates that cross chunk boundaries, stopping inside a chunk
+    or on its last iteration, match the oracle's (a skipped or repeated step
+    would change omega or beta at O(1))."""
+    from pybella_amd.op3d import Operator3D
+    print(f"case op3d iterates across chunks check_every={check_every} k={k}", flush=True)
+    st = _state(33, 17, 19)
+    ref = olp.assemble3d(st, 0.05)
+    op = Operator3D.from_state(st)
+    b = W.fourier_rhs_3d(33, 17, 19, seed=5, nmodes=32, kmax=6).ravel() * ref["dinv"].ravel()
+    x, info = op.solve(_dev(b), tol=1e-30, maxiter=k, mode="f64", check_every=check_every)
+    r = obs.bicgstab(lambda v: olp.lap3d(v, ref), b, tol=1e-30, maxiter=k)
+    assert info["iters"] == k == r.niter
+    assert rel(x.cpu().numpy().ravel(), r.x) < 1e-7
+
+
+@pytest.mark.parametrize("mode", ["f64", "mixed"])
+def test_converged_solve_short_chunks(mode):
+    """Two-iteration chunks: a fold after every second iteration; the solve
+    converges inside a chunk and stops there."""
+    from pybella_amd.op3d import Operator3D
+    print(f"case op3d solve {mode} check_every=2", flush=True)
+    st = _state(33, 17, 19)
+    ref = olp.assemble3d(st, 0.05)
+    op = Operator3D.from_state(st)
+    b = W.fourier_rhs_3d(33, 17, 19, seed=6, nmodes=32, kmax=6).ravel() * ref["dinv"].ravel()
+    x, info = op.solve(_dev(b), tol=1e-10, mode=mode, check_every=2)
+    assert info["info"] == 0 and info["true_rel_resid"] <= 1e-10, info
+    mv = lambda v: olp.lap3d(v, ref)
+    assert obs.true_rel_resid(mv, b, x.cpu().numpy().ravel()) < 2e-10
+
+
 @pytest.mark.parametrize("mode", ["f64", "mixed"])
 def test_converged_solve(mode):
     from pybella_amd.op3d import Operator3D
