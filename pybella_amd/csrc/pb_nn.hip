@@ -25,6 +25,9 @@ using namespace pb2d;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// LDS of a convolution block's weight tile: 16 NT rows x (cin + 4); the
+// launches keep 16 NT cin <= 8192 (checked on the host)
+constexpr int kConvWsh = 8192 + 64 * 4;
 typedef unsigned short bf16;
 
 namespace {
@@ -59,8 +62,9 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
                                                  bf16* pre, int pre_ctot, float* out_f32, int nimg, int h, int w) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long npix = (long)nimg * h * w;
+  // waves past the last pixel stay (the block's barriers need them): their
+  // loads read offset 0 and are zeroed, and they store nothing
   const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
-  if (p0 >= npix) return;
   const long pa = p0 + (lane & 15);
   const int kg = lane >> 4;
   const bool va = pa < npix;
@@ -75,13 +79,27 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
   // over the output channels gives small levels more waves)
   const int t0 = blockIdx.y * NT;
   const int nout = gridDim.y * NT * 16;
+  // A tap's weight tile of the block (16 NT output rows x cin) is staged in
+  // LDS once, with 16-byte loads, instead of every wave gathering it from L2;
+  // rows padded by 4 so the 16 rows a B operand reads fall in distinct banks
+  __shared__ bf16 wsh[kConvWsh];
+  const int wst = cin + 4;
+  const int nq = NT * 16 * (cin / 8);   // 8-element pieces of the tile
   for (int tap = 0; tap < 9; ++tap) {
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
     const int ys = ya + dy, xs = xa + dx;
     const bool ok = va && ys >= 0 && ys < h && xs >= 0 && xs < w;
     const long src = ok ? (((ba * h + ys) * w + xs) * in_ctot + in_coff + 4 * kg) : 0;
     const int wt = flip ? 8 - tap : tap;
-    const bf16* wrow = wg + ((long)wt * nout + (lane & 15)) * cin + 4 * kg;
+    __syncthreads();   // the previous tap's reads of wsh are done
+    for (int q = threadIdx.x; q < nq; q += 256) {
+      const int row = q / (cin / 8), c8 = q - row * (cin / 8);
+      const uint4 v = *(const uint4*)(wg + ((long)wt * nout + t0 * 16 + row) * cin + c8 * 8);
+      *(uint2*)(wsh + row * wst + c8 * 8) = uint2{v.x, v.y};   // LDS rows are 8-byte aligned
+      *(uint2*)(wsh + row * wst + c8 * 8 + 4) = uint2{v.z, v.w};
+    }
+    __syncthreads();
+    const bf16* wrow = wsh + (lane & 15) * wst + 4 * kg;
     // Input-channel chunks of 16 in groups of four: a group's loads are all
     // issued before its MFMAs (one memory latency per group, not per chunk).
     // The input loads are unconditional (src is 0, a valid offset, outside
@@ -95,7 +113,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
       for (int u = 0; u < 4; ++u) {
         a[u] = *(const bf16x4*)(in + src + (c + u) * 16);
 #pragma unroll
-        for (int t = 0; t < NT; ++t) b[u][t] = *(const bf16x4*)(wrow + (long)(t0 + t) * 16 * cin + (c + u) * 16);
+        for (int t = 0; t < NT; ++t) b[u][t] = *(const bf16x4*)(wrow + t * 16 * wst + (c + u) * 16);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -109,7 +127,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
       if (!ok) a = bf16x4{0, 0, 0, 0};
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const bf16x4 b = *(const bf16x4*)(wrow + (long)(t0 + t) * 16 * cin + c * 16);
+        const bf16x4 b = *(const bf16x4*)(wrow + t * 16 * wst + c * 16);
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
       }
     }
@@ -502,11 +520,21 @@ struct LayerSpec {
 };
 
 // 15 convolutions in forward order (see the table in the file header)
-const LayerSpec kLayers[15] = {
+constexpr LayerSpec kLayers[15] = {
     {3, 16, 1},  {16, 16, 1},  {16, 32, 1}, {32, 32, 1}, {32, 64, 1},  {64, 64, 1}, {64, 64, 1}, {64, 64, 1},
     {128, 64, 1}, {64, 64, 1}, {96, 32, 1}, {32, 32, 1}, {48, 16, 1}, {16, 16, 1}, {16, 1, 0}};
 
-inline int pad16(int c) { return (c + 15) / 16 * 16; }
+constexpr int pad16(int c) { return (c + 15) / 16 * 16; }
+
+// a convolution block stages at most 16 NT x cin weights, NT x 16 <= the
+// layer's (padded) output channels: forward 16 NT cin <= cin_p cout_p, and
+// the input gradient the same with the roles swapped
+constexpr bool weight_tiles_fit() {
+  for (const LayerSpec& L : kLayers)
+    if (pad16(L.cin_v) * pad16(L.cout_v) > 8192) return false;
+  return true;
+}
+static_assert(weight_tiles_fit(), "a layer's weight tile exceeds the convolution's LDS stage");
 
 struct Level {
   int h, w;
