@@ -55,7 +55,10 @@ struct View {
 // ---------------------------------------------------------------------------
 // 3x3 convolution (forward or input-gradient), NT output tiles of 16 channels
 // ---------------------------------------------------------------------------
-template <int NT>
+// PX: 16-pixel groups per wave (independent accumulation chains, so a wave
+// has PX loads and MFMAs in flight per chunk; each pixel's sum is in the same
+// order for any PX)
+template <int NT, int PX = 1>
 __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, int in_ctot, int in_coff, int cin,
                                                  const bf16* __restrict__ wg, int flip, const float* __restrict__ bias,
                                                  int act, bf16* out, int out_ctot, int out_coff, int cout_valid,
@@ -64,17 +67,25 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
   const long npix = (long)nimg * h * w;
   // waves past the last pixel stay (the block's barriers need them): their
   // loads read offset 0 and are zeroed, and they store nothing
-  const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
-  const long pa = p0 + (lane & 15);
+  const long p0 = ((long)blockIdx.x * 4 + wave) * 16 * PX;
   const int kg = lane >> 4;
-  const bool va = pa < npix;
-  const int xa = (int)(pa % w);
-  const long t1 = pa / w;
-  const int ya = (int)(t1 % h);
-  const long ba = t1 / h;
-  f32x4 acc[NT];
+  bool va[PX];
+  int xa[PX], ya[PX];
+  long ba[PX];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int g = 0; g < PX; ++g) {
+    const long pa = p0 + g * 16 + (lane & 15);
+    va[g] = pa < npix;
+    xa[g] = (int)(pa % w);
+    const long t1 = pa / w;
+    ya[g] = (int)(t1 % h);
+    ba[g] = t1 / h;
+  }
+  f32x4 acc[PX][NT];
+#pragma unroll
+  for (int g = 0; g < PX; ++g)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[g][t] = f32x4{0.f, 0.f, 0.f, 0.f};
   // the block's output tiles: t0 .. t0+NT-1 of gridDim.y * NT (a grid split
   // over the output channels gives small levels more waves)
   const int t0 = blockIdx.y * NT;
@@ -87,9 +98,14 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
   const int nq = NT * 16 * (cin / 8);   // 8-element pieces of the tile
   for (int tap = 0; tap < 9; ++tap) {
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const int ys = ya + dy, xs = xa + dx;
-    const bool ok = va && ys >= 0 && ys < h && xs >= 0 && xs < w;
-    const long src = ok ? (((ba * h + ys) * w + xs) * in_ctot + in_coff + 4 * kg) : 0;
+    bool ok[PX];
+    long src[PX];
+#pragma unroll
+    for (int g = 0; g < PX; ++g) {
+      const int ys = ya[g] + dy, xs = xa[g] + dx;
+      ok[g] = va[g] && ys >= 0 && ys < h && xs >= 0 && xs < w;
+      src[g] = ok[g] ? (((ba[g] * h + ys) * w + xs) * in_ctot + in_coff + 4 * kg) : 0;
+    }
     const int wt = flip ? 8 - tap : tap;
     __syncthreads();   // the previous tap's reads of wsh are done
     for (int q = threadIdx.x; q < nq; q += 256) {
@@ -108,38 +124,50 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
     const int nci = cin >> 4;
     int c = 0;
     for (; c + 4 <= nci; c += 4) {
-      bf16x4 a[4], b[4][NT];
+      bf16x4 a[PX][4], b[4][NT];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        a[u] = *(const bf16x4*)(in + src + (c + u) * 16);
+#pragma unroll
+        for (int g = 0; g < PX; ++g) a[g][u] = *(const bf16x4*)(in + src[g] + (c + u) * 16);
 #pragma unroll
         for (int t = 0; t < NT; ++t) b[u][t] = *(const bf16x4*)(wrow + t * 16 * wst + (c + u) * 16);
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (!ok) a[u] = bf16x4{0, 0, 0, 0};
+      for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u][t], acc[t], 0, 0, 0);
-      }
+        for (int g = 0; g < PX; ++g) {
+          if (!ok[g]) a[g][u] = bf16x4{0, 0, 0, 0};
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[g][u], b[u][t], acc[g][t], 0, 0, 0);
+        }
     }
     for (; c < nci; ++c) {
-      bf16x4 a = *(const bf16x4*)(in + src + c * 16);
-      if (!ok) a = bf16x4{0, 0, 0, 0};
+      bf16x4 a[PX];
+#pragma unroll
+      for (int g = 0; g < PX; ++g) {
+        a[g] = *(const bf16x4*)(in + src[g] + c * 16);
+        if (!ok[g]) a[g] = bf16x4{0, 0, 0, 0};
+      }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const bf16x4 b = *(const bf16x4*)(wrow + t * 16 * wst + c * 16);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+#pragma unroll
+        for (int g = 0; g < PX; ++g)
+          acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[g], b, acc[g][t], 0, 0, 0);
       }
     }
   }
+#pragma unroll
+  for (int g = 0; g < PX; ++g)
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int co = (t0 + t) * 16 + (lane & 15);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const long p = p0 + 4 * kg + e;
+      const long p = p0 + g * 16 + 4 * kg + e;
       if (p < npix && co < cout_valid) {
-        float v = acc[t][e] + (bias ? bias[co] : 0.f);
+        float v = acc[g][t][e] + (bias ? bias[co] : 0.f);
         const long o = p * out_ctot + out_coff + co;
         if (pre) pre[p * pre_ctot + co] = f2bf(v);   // pre-activations: own buffer, cout channels
         if (act) v = gelu(v);
@@ -729,9 +757,12 @@ void conv(pb_nn* nn, Acts* a, int l, int level, const bf16* in, int in_ct, int i
   // waves to hide that latency; the input loads repeat, from cache), with the
   // fewest tiles per block whose grid fits in one round of resident blocks:
   // a second round of a few blocks would double the launch.
+  // Large levels take two 16-pixel groups per wave instead (two chains of
+  // loads and MFMAs in flight per wave, half the blocks).
+  const bool large = npix >= 65536;
   const long bx = (npix + 63) / 64;
   int ntb = nt;
-  if (npix < 65536 && nt > 1) {
+  if (!large && nt > 1) {
     ntb = 1;
     for (int c : {1, 2, 4})
       if (c <= nt && nt % c == 0 && bx * (nt / c) <= conv_slots(c)) {
@@ -739,14 +770,22 @@ void conv(pb_nn* nn, Acts* a, int l, int level, const bf16* in, int in_ct, int i
         break;
       }
   }
-  const dim3 grd((unsigned)bx, nt / ntb), blk(256);
-#define PB_CONV(NT)                                                                                              \
-  hipLaunchKernelGGL(k_conv3x3<NT>, grd, blk, 0, s, in, in_ct, in_co, cin, nn->wf[l], 0, bias, L.act, out, out_ct, \
-                     out_co, L.cout_v, pre, pad16(L.cout_v), out_f32, a->nimg, h, w)
-  switch (ntb) {
-    case 1: PB_CONV(1); break;
-    case 2: PB_CONV(2); break;
-    default: PB_CONV(4); break;
+  const dim3 grd((unsigned)(large ? (npix + 127) / 128 : bx), nt / ntb), blk(256);
+#define PB_CONV(NT, PX)                                                                                           \
+  hipLaunchKernelGGL((k_conv3x3<NT, PX>), grd, blk, 0, s, in, in_ct, in_co, cin, nn->wf[l], 0, bias, L.act, out,   \
+                     out_ct, out_co, L.cout_v, pre, pad16(L.cout_v), out_f32, a->nimg, h, w)
+  if (large) {
+    switch (ntb) {
+      case 1: PB_CONV(1, 2); break;
+      case 2: PB_CONV(2, 2); break;
+      default: PB_CONV(4, 2); break;
+    }
+  } else {
+    switch (ntb) {
+      case 1: PB_CONV(1, 1); break;
+      case 2: PB_CONV(2, 1); break;
+      default: PB_CONV(4, 1); break;
+    }
   }
 #undef PB_CONV
 }
