@@ -67,7 +67,7 @@ class Operator3D:
         self.ctx.check(self.ctx.lib.pb_op3d_matvec(self.h, _abi.ptr(x), _abi.ptr(y), prec), "pb_op3d_matvec")
         return y
 
-    def solve(self, b, x=None, tol=1e-8, maxiter=6000, mode="mixed", inner_tol=1e-4, max_outer=20,
+    def solve(self, b, x=None, tol=1e-8, maxiter=6000, mode="mixed", inner_tol=1e-5, max_outer=20,
               check_every=32):
         """mode "f64" (reference legacy tolerance) or "mixed" (FP32 inner
         solves in FP64 refinement to true relative residual ``tol``)."""
