@@ -245,19 +245,35 @@ class _Staging:
         torch.cuda.current_stream().synchronize()
         return out
 
-    def download(self, pairs):
-        """(device [y][x] tensor, host array or None) pairs -> the host arrays
-        in reference layout: written into the given array, else new."""
+    def download_start(self, pairs, stream=None):
+        """Queue the device -> pinned copies of (device [y][x] tensor, host
+        array or None) pairs on ``stream`` (torch's current stream if None);
+        returns the pending batch for download_finish.  The device tensors
+        must stay alive until then."""
         torch = _torch()
         self.used = {}
         staged = []
-        for t, out in pairs:
-            d = t.detach()
-            d = d.permute(*reversed(range(d.dim()))).contiguous()
-            p = self._get(d.shape)
-            p.copy_(d, non_blocking=True)
-            staged.append((d, p, out))
-        torch.cuda.current_stream().synchronize()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        with torch.cuda.stream(s):
+            for t, out in pairs:
+                d = t.detach()
+                d = d.permute(*reversed(range(d.dim()))).contiguous()
+                p = self._get(d.shape)
+                p.copy_(d, non_blocking=True)
+                staged.append((d, p, out))
+            ev = torch.cuda.Event()
+            ev.record(s)
+        return staged, ev
+
+    def download(self, pairs):
+        """(device [y][x] tensor, host array or None) pairs -> the host arrays
+        in reference layout: written into the given array, else new."""
+        return self.download_finish(self.download_start(pairs))
+
+    def download_finish(self, pending):
+        torch = _torch()
+        staged, ev = pending
+        ev.synchronize()
         res = []
         for _, p, out in staged:
             if out is None:
@@ -273,6 +289,8 @@ class _Staging:
 
 
 _staging = _Staging()
+_staging_early = _Staging()   # the fields final before the solve, copied during it
+_side_stream = None
 
 
 class _DevState:
@@ -398,9 +416,24 @@ def _project_update(pr, x):
     _bc_cells(ctx, grid, phys, pr.sol, pr.rhoY0)
 
 
-def _finish(pr, x, info, Sol, Sol0, mpv):
+def _early_download(pr, mpv):
+    """rhs, wplus, wcenter and nu_c are final once the preparation has run:
+    their copies to the host start on a side stream (default priority) and
+    run during the solve.  Returns the pending batch for _finish."""
+    global _side_stream
+    torch = _torch()
+    if _side_stream is None:
+        _side_stream = torch.cuda.Stream()
+    _side_stream.wait_stream(torch.cuda.current_stream())
+    pairs = [(pr.rhs, mpv.rhs), (pr.wplus0, mpv.wplus[0]), (pr.wplus1, mpv.wplus[1]), (pr.wcenter, None),
+             (pr.nu_c, None)]
+    return _staging_early.download_start(pairs, _side_stream)
+
+
+def _finish(pr, x, info, Sol, Sol0, mpv, early=None):
     """second_projection.py:235-277: counters, the device update, then the
-    side effects are written back to the host objects."""
+    side effects are written back to the host objects (``early``: the batch
+    _early_download started for this projection, if any)."""
     global total_iter, total_calls
     total_iter += info["iters"]
     total_calls += 1
@@ -408,9 +441,15 @@ def _finish(pr, x, info, Sol, Sol0, mpv):
     logging.info("Total calls to BiCGStab routine = %i, total iterations = %i" % (total_calls, total_iter))
     _project_update(pr, x)
     pairs = pr.sol.pairs(Sol) + (pr.sol0.pairs(Sol0) if pr.sol0 is not None else [])
-    pairs += [(pr.p2n, mpv.p2_nodes), (pr.rhs, mpv.rhs), (pr.wplus0, mpv.wplus[0]), (pr.wplus1, mpv.wplus[1]),
-              (pr.wcenter, None), (pr.nu_c, None), (pr.u, None), (pr.v, None), (pr.w, None)]
-    mpv.wcenter, mpv.nu_c, mpv.u, mpv.v, mpv.w = _staging.download(pairs)[-5:]
+    pairs += [(pr.p2n, mpv.p2_nodes), (pr.u, None), (pr.v, None), (pr.w, None)]
+    if early is None:
+        pairs += [(pr.rhs, mpv.rhs), (pr.wplus0, mpv.wplus[0]), (pr.wplus1, mpv.wplus[1]), (pr.wcenter, None),
+                  (pr.nu_c, None)]
+    res = _staging.download(pairs)
+    if early is not None:
+        res += _staging_early.download_finish(early)
+    mpv.u, mpv.v, mpv.w = res[-8:-5]
+    mpv.wcenter, mpv.nu_c = res[-2:]
 
 
 def euler_backward_non_advective_impl_part(Sol, mpv, elem, node, ud, th, t, dt, alpha_diff, Sol0=None,
@@ -419,6 +458,7 @@ def euler_backward_non_advective_impl_part(Sol, mpv, elem, node, ud, th, t, dt, 
     (second_projection.py:97-277).  Returns the solver info dict."""
     op = operator_for(node, ud)
     pr = _prepare(Sol, mpv, elem, node, ud, th, dt, Sol0, op)
+    early = _early_download(pr, mpv)
     mode = solver_options.get("mode", "f64")
     extra = {}
     if mode == "nn":   # the learned phase has its own budget; ud.max_iterations caps the fallback
@@ -428,7 +468,7 @@ def euler_backward_non_advective_impl_part(Sol, mpv, elem, node, ud, th, t, dt, 
                        mode=mode, inner_tol=solver_options.get("inner_tol", 1e-3),
                        max_outer=solver_options.get("max_outer", 12),
                        check_every=solver_options.get("check_every", 32), **extra)
-    _finish(pr, x, info, Sol, Sol0, mpv)
+    _finish(pr, x, info, Sol, Sol0, mpv, early)
     return info
 
 
