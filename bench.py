@@ -176,6 +176,41 @@ def bench_3d(args, nx=257, ny=129, nz=129):
                       "kernels": kernels, "hbm_peak_GBps": HBM_PEAK_GBS}), flush=True)
 
 
+def projection_step_rate(mode, rows_per_thread, steps=3, nx=1025, ny=513, dt=0.05):
+    """Device-resident projection steps/s at a given tile height: the drop-in's
+    device work (preparation, solve, update) on the 1025x513 rising bubble of
+    ``--workload step``, its state restored from device copies before each
+    step, inside the timed region."""
+    import torch
+    from pybella_amd import second_projection as sp
+    from pybella_amd import workloads as W
+    Sol, Sol0, mpv, elem, node, ud, th = W.rising_bubble_state(nx, ny, dt=dt)
+    op = sp.operator_for(node, ud)
+    op.set_tile(rows_per_thread)
+    pr = sp._upload(Sol, mpv, elem, node, ud, th, dt, Sol0, op)
+    dev0 = [(t, t.clone()) for t in list(pr.sol.f.values()) + list(pr.sol0.f.values()) + [pr.p2n]]
+
+    def step():
+        for t, t0 in dev0:
+            t.copy_(t0)
+        sp._project_prepare(pr)
+        x, info = op.solve(pr.b, tol=float(ud.tol), maxiter=int(ud.max_iterations), mode=mode)
+        sp._project_update(pr, x)
+        if info["info"] != 0:
+            raise SystemExit(f"projection step solve did not converge: {info}")
+        return info
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        info = step()
+    torch.cuda.synchronize()
+    return {"workload": f"projection step (drop-in impl_part device work), {nx}x{ny} rising bubble, dt {dt}",
+            "mode": mode, "rows_per_thread": rows_per_thread,
+            "steps_per_s": round(steps / (time.perf_counter() - t0), 3), "iterations": info["iters"]}
+
+
 def bench_step(args, nx=1025, ny=513, dt=0.05):
     """Projection steps/s: the whole drop-in euler_backward_non_advective_impl_part
     (second_projection.py:97-277) on a 1025x513 rising-bubble state after its
@@ -545,6 +580,12 @@ def main():
             "traffic": traffic, "avg_us": round(kt[dom], 3), "alg_bytes": alg,
             "kernels_us": {KERNEL_NAMES[k]: round(v, 3) for k, v in kt.items()}}
 
+    # time steps/s beside the solve rate: whole projection steps, device-resident
+    step_rate = None
+    if "f64fast" in tiles:
+        step_rate = projection_step_rate("f64fast", tiles["f64fast"]["rows_per_thread"])
+        log("projection step:", step_rate)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -577,7 +618,8 @@ def main():
                    "nn_over_plain": None if plain_rate is None else round(value / plain_rate, 3),
                    "calibration_solves_per_s": {**{k: round(v, 3) for k, v in rates.items()},
                                                 **{"nn " + k: v for k, v in nn_table.items()}},
-                   "tiles": tiles},
+                   "tiles": tiles,
+                   "projection_step": step_rate},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
