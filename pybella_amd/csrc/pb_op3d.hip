@@ -91,8 +91,9 @@ __device__ __forceinline__ double eps2() { return 4.930380657631324e-32; }
 //     Otherwise (unless PF 2 without UNR) every thread loads, with clamped
 //     indices, so no load sits behind a branch: faster for the matvec and
 //     the five-dot launch, slower for launch 1 (measured).
+//   ZC: z planes per block (the launch's grid z is ceil(nz / ZC)).
 template <typename T, int NV, int PF = (sizeof(T) == 4 ? 2 : 1), bool UNR = false, bool GUARD = false,
-          class In, class PreFn, class Out>
+          int ZC = Z3, class In, class PreFn, class Out>
 __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out out,
                                        double (&acc)[NV > 0 ? NV : 1]) {
   using PreT = decltype(pre(0L));
@@ -110,7 +111,7 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
   const int lin = xx < rem ? xx * (base + 1) + kk8 : rem * (base + 1) + (xx - rem) * base + kk8;
   const int bxx = lin % gridDim.x, byz = lin / gridDim.x, byy = byz % gridDim.y, bzz = byz / gridDim.y;
   const int x0 = bxx * X3, y0 = byy * Y3;
-  const int kb = bzz * Z3, ke = min(kb + Z3, o.nz);
+  const int kb = bzz * ZC, ke = min(kb + ZC, o.nz);
   const int nx = o.nx, ny = o.ny;
   const int i = x0 + tx, j = y0 + ty;
   const bool own = i < nx && j < ny;
@@ -509,6 +510,10 @@ __device__ __forceinline__ void sum_rec3(const Rec3<NV>& e, const double* P, int
   }
 }
 
+// z planes per block of launch 1 (FP64 marches 24: see k3_iter1)
+template <typename T>
+constexpr int kZ1 = sizeof(T) == 4 ? Z3 : 24;
+
 // launch 1 of iteration k: first the scalar step of iteration k-1's launch 2
 // (its five dots, when pending), then r_{k-1} = r - alpha v - omega t,
 // x += alpha p + omega (r - alpha v), p_k = r_{k-1} + beta (p - omega v),
@@ -534,7 +539,9 @@ __global__ __launch_bounds__(NT3) void k3_iter1(Op3<T> o, const KS3* Sin, KS3* S
   struct Ops {   // this node's v, r, p, t, r~, x, loaded a plane step ahead
     T v, r, p, t, rt, x;
   };
-  march3<T, 1, (sizeof(T) == 4 ? 2 : 1), false, true>(
+  // FP64: its node operands one step ahead too (112 VGPRs, two blocks per
+  // CU), over 24-plane chunks so the grid fits one round of blocks
+  march3<T, 1, 2, false, true, kZ1<T>>(
       o,
       [&](long g) {
         const T v = va[g];
@@ -725,11 +732,11 @@ int dalloc3(pb_op3d* op, T** p, long count) {
   return PB_OK;
 }
 
-dim3 march_grid(const pb_op3d* op) {
-  return dim3(ceil_div(op->g.nx, X3), ceil_div(op->g.ny, Y3), ceil_div(op->g.nz, Z3));
+dim3 march_grid(const pb_op3d* op, int zc = Z3) {
+  return dim3(ceil_div(op->g.nx, X3), ceil_div(op->g.ny, Y3), ceil_div(op->g.nz, zc));
 }
-int march_blocks(const pb_op3d* op) {
-  const dim3 gr = march_grid(op);
+int march_blocks(const pb_op3d* op, int zc = Z3) {
+  const dim3 gr = march_grid(op, zc);
   return (int)(gr.x * gr.y * gr.z);
 }
 dim3 elem_grid3(long n) { return dim3((unsigned)std::min<long>((n + 255) / 256, 4096)); }
@@ -803,7 +810,9 @@ int krylov3(pb_op3d* op, const T* b, double tol, int maxiter, int legacy, int ch
   const long n = op->n;
   const Op3<T> o = op3<T>(op);
   const dim3 mg = march_grid(op);
-  const int nbm = march_blocks(op);
+  const int nbm = march_blocks(op);   // launch 2's blocks (five-dot records)
+  const dim3 mg1 = march_grid(op, kZ1<T>);
+  const int nb1 = march_blocks(op, kZ1<T>);   // launch 1's blocks (r~.v records)
   const dim3 eg = elem_grid3(n);
   T* t = (T*)op->t[P];
   T* rt = (T*)op->rt[P];
@@ -825,10 +834,10 @@ int krylov3(pb_op3d* op, const T* b, double tol, int maxiter, int legacy, int ch
     for (int c = 0; c < chunk; ++c) {
       const long it = it0 + c + 1;
       const int a = (int)((it - 1) & 1), bset = (int)(it & 1);
-      hipLaunchKernelGGL(k3_iter1<T>, mg, kBlk3, 0, s, o, (const KS3*)S0, S1, (const double*)op->part, nbm,
+      hipLaunchKernelGGL(k3_iter1<T>, mg1, kBlk3, 0, s, o, (const KS3*)S0, S1, (const double*)op->part, nbm,
                          (const T*)op->r[P][a], (const T*)op->p[P][a], (const T*)op->v[P][a], (const T*)t,
                          (const T*)rt, (T*)op->r[P][bset], (T*)op->p[P][bset], (T*)op->v[P][bset], x, op->part1);
-      hipLaunchKernelGGL(k3_iter2<T>, mg, kBlk3, 0, s, o, (const KS3*)S1, S0, (const double*)op->part1, nbm,
+      hipLaunchKernelGGL(k3_iter2<T>, mg, kBlk3, 0, s, o, (const KS3*)S1, S0, (const double*)op->part1, nb1,
                          (const T*)op->r[P][bset], (const T*)op->v[P][bset], (const T*)rt, t, op->part);
     }
     hipLaunchKernelGGL(k3_fold, dim3(1), dim3(NRT), 0, s, S0, (const double*)op->part, nbm);
@@ -1117,7 +1126,8 @@ extern "C" int pb_op3d_time_kernel(pb_op3d* op, int kernel, int precision, int r
                            (float*)op->v[1][1]);
     } else if (kernel == 1) {
       if (P == 0)
-        hipLaunchKernelGGL(k3_iter1<double>, mg, kBlk3, 0, s, op3<double>(op), (const KS3*)op->st, op->st + 1,
+        hipLaunchKernelGGL(k3_iter1<double>, march_grid(op, kZ1<double>), kBlk3, 0, s, op3<double>(op),
+                           (const KS3*)op->st, op->st + 1,
                            (const double*)op->part, nbm, (const double*)op->r[0][0], (const double*)op->p[0][0],
                            (const double*)op->v[0][0], (const double*)op->t[0], (const double*)op->rt[0],
                            (double*)op->r[0][1], (double*)op->p[0][1], (double*)op->v[0][1], (double*)op->x[0],
@@ -1131,11 +1141,13 @@ extern "C" int pb_op3d_time_kernel(pb_op3d* op, int kernel, int precision, int r
     } else {   // reads part1's records and writes its own five-dot records to part
       if (P == 0)
         hipLaunchKernelGGL(k3_iter2<double>, mg, kBlk3, 0, s, op3<double>(op), (const KS3*)op->st, op->st + 1,
-                           (const double*)op->part1, nbm, (const double*)op->r[0][0], (const double*)op->v[0][0],
+                           (const double*)op->part1, march_blocks(op, kZ1<double>), (const double*)op->r[0][0],
+                           (const double*)op->v[0][0],
                            (const double*)op->rt[0], (double*)op->t[0], op->part);
       else
         hipLaunchKernelGGL(k3_iter2<float>, mg, kBlk3, 0, s, op3<float>(op), (const KS3*)op->st, op->st + 1,
-                           (const double*)op->part1, nbm, (const float*)op->r[1][0], (const float*)op->v[1][0],
+                           (const double*)op->part1, march_blocks(op, kZ1<float>), (const float*)op->r[1][0],
+                           (const float*)op->v[1][0],
                            (const float*)op->rt[1], (float*)op->t[1], op->part);
     }
   };
