@@ -112,6 +112,9 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
   const int bxx = lin % gridDim.x, byz = lin / gridDim.x, byy = byz % gridDim.y, bzz = byz / gridDim.y;
   const int x0 = bxx * X3, y0 = byy * Y3;
   const int kb = bzz * ZC, ke = min(kb + ZC, o.nz);
+  // a block whose chunk starts past the last plane (a grid with more z
+  // blocks than ceil(nz / ZC)) loads nothing; its caller's sums stay zero
+  if (kb >= o.nz) return;
   const int nx = o.nx, ny = o.ny;
   const int i = x0 + tx, j = y0 + ty;
   const bool own = i < nx && j < ny;
@@ -1133,7 +1136,8 @@ extern "C" int pb_op3d_time_kernel(pb_op3d* op, int kernel, int precision, int r
                            (double*)op->r[0][1], (double*)op->p[0][1], (double*)op->v[0][1], (double*)op->x[0],
                            op->part1);
       else
-        hipLaunchKernelGGL(k3_iter1<float>, mg, kBlk3, 0, s, op3<float>(op), (const KS3*)op->st, op->st + 1,
+        hipLaunchKernelGGL(k3_iter1<float>, march_grid(op, kZ1<float>), kBlk3, 0, s, op3<float>(op),
+                           (const KS3*)op->st, op->st + 1,
                            (const double*)op->part, nbm, (const float*)op->r[1][0], (const float*)op->p[1][0],
                            (const float*)op->v[1][0], (const float*)op->t[1], (const float*)op->rt[1],
                            (float*)op->r[1][1], (float*)op->p[1][1], (float*)op->v[1][1], (float*)op->x[1],
