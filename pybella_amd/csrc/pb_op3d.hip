@@ -112,9 +112,13 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
   const int bxx = lin % gridDim.x, byz = lin / gridDim.x, byy = byz % gridDim.y, bzz = byz / gridDim.y;
   const int x0 = bxx * X3, y0 = byy * Y3;
   const int kb = bzz * ZC, ke = min(kb + ZC, o.nz);
-  // a block whose chunk starts past the last plane (a grid with more z
-  // blocks than ceil(nz / ZC)) loads nothing; its caller's sums stay zero
-  if (kb >= o.nz) return;
+  // A block whose chunk starts past the last plane (a grid with more z
+  // blocks than ceil(nz / ZC)) loads nothing; its caller's sums stay zero.
+  // Only a march whose chunk differs from the default grid's can meet one
+  // (the check costs the five-dot march registers, so it is left out there).
+  if constexpr (ZC != Z3) {
+    if (kb >= o.nz) return;
+  }
   const int nx = o.nx, ny = o.ny;
   const int i = x0 + tx, j = y0 + ty;
   const bool own = i < nx && j < ny;
