@@ -163,6 +163,23 @@ int pb_op2d_scale_rhs(pb_op2d* op, const double* rhs_isc, double* b_vec);
 int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x,
                       const pb_solve_opts* opts, pb_solve_info* info);
 
+/* The FP64 solve of pb_bicgstab_solve (modes PB_SOLVE_F64 and
+ * PB_SOLVE_F64_FAST only) without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device or pinned memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its Krylov vectors, partial records and state records live in
+ * a static device array of 40 MiB per device (looked up at pb_ctx_create),
+ * which holds grids up to about 650,000 nodes (1025 x 513 fits); a larger
+ * grid returns PB_ERR_ARG.  The iterate is kept in x itself; iterations are
+ * launched directly (no hipGraph) and the state is read back into host stack
+ * memory.  Iterates, x and the info equal pb_bicgstab_solve's in the same
+ * mode and tile height.  Every call on one device shares that array and it
+ * keeps the last call's contents: calls that overlap on one device (two
+ * streams or host threads) race and must be serialised by the caller.
+ * Synchronous; returns the info code as pb_bicgstab_solve. */
+int pb_bicgstab_solve_noalloc(pb_op2d* op, const double* b, double* x,
+                              const pb_solve_opts* opts, pb_solve_info* info);
+
 /* Tile height of the solver's stencil launches: 64 x (4 * rows_per_thread)
  * nodes per 256-thread block; rows_per_thread in {2, 3, 4, 5, 6, 8} (default
  * 4).  The height sets the block count, and so how the grid fills the CUs'

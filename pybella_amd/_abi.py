@@ -83,6 +83,7 @@ SIGNATURES = {
     "pb_op2d_matvec": (I, [VP, VP, VP, I]),
     "pb_op2d_scale_rhs": (I, [VP, VP, VP]),
     "pb_bicgstab_solve": (I, [VP, VP, VP, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+    "pb_bicgstab_solve_noalloc": (I, [VP, VP, VP, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "pb_op2d_time_kernel": (I, [VP, I, I, I, C.POINTER(D)]),
     "pb_op2d_set_tile": (I, [VP, I]),
     "pb_divergence_nodes": (I, [VP, C.POINTER(Grid2D), D, VP, VP, VP, VP, VP, VP]),

@@ -14,6 +14,11 @@ struct pb_ctx {
   hipStream_t stream = nullptr;   // stream all work is issued on
   std::string err;
   int n_cu = 0;
+  // this device's copy of the library's static scratch array (device
+  // address from hipGetSymbolAddress at context creation; null if that failed)
+  double* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  int scratch_err = 0;   // hipError_t of the address lookup
 };
 
 // Record a formatted error on the context and return code.

@@ -843,21 +843,27 @@ int run_chunk(pb_op2d* op, Work<T>& w, int c, bool exact, hipStream_t s) {
   return PB_OK;
 }
 
+__global__ void k_put_state(KState* dst, KState v) { *dst = v; }
+
 // One Krylov run: solves A x = b (b, x internal T / FP64 buffers of ``w``).
-// x0: optional FP64 initial guess (device); result in w.x.
+// x0: optional FP64 initial guess (device); result in w.x.  ``host``: where
+// the state is read back to (two records); null = the operator's pinned
+// mirror, with the chunks replayed as cached hipGraphs.  With a caller's
+// buffer the chunks are launched directly, so the run allocates nothing.
 template <typename T>
 int krylov_run(pb_op2d* op, Work<T>& w, const T* b, const double* x0, double tol, int maxiter, int legacy,
-               int check_every, hipStream_t s, KState* out, bool exact = false) {
+               int check_every, hipStream_t s, KState* out, bool exact = false, KState* host = nullptr) {
   pb_ctx* ctx = op->ctx;
+  KState* hs = host ? host : op->hstate;
   KState init{};
   init.tol = tol;
   init.maxiter = maxiter;
   init.legacy = legacy;
   init.status = ST_RUN;
   init.phase = PH_INIT;
-  op->hstate[0] = init;
-  // the first launch of an iteration reads S0 (fast) or S2 (exact)
-  PB_HIP(ctx, hipMemcpyAsync(w.S + (exact ? 2 : 0), op->hstate, sizeof(KState), hipMemcpyHostToDevice, s));
+  // the first launch of an iteration reads S0 (fast) or S2 (exact); the
+  // record travels as a kernel argument
+  hipLaunchKernelGGL(k_put_state, dim3(1), dim3(1), 0, s, w.S + (exact ? 2 : 0), init);
   {
     const dim3 grd = stencil_grid(op, op->ry);
     const OpDev<T> d = opdev<T>(op);
@@ -873,9 +879,9 @@ int krylov_run(pb_op2d* op, Work<T>& w, const T* b, const double* x0, double tol
   const int chunk = std::max(2, (check_every > 0 ? check_every : 32) & ~1);
   long issued = 1;
   for (;;) {
-    PB_HIP(ctx, hipMemcpyAsync(op->hstate + 1, w.S + chk, sizeof(KState), hipMemcpyDeviceToHost, s));
+    PB_HIP(ctx, hipMemcpyAsync(hs + 1, w.S + chk, sizeof(KState), hipMemcpyDeviceToHost, s));
     PB_HIP(ctx, hipStreamSynchronize(s));
-    const KState h = op->hstate[1];
+    const KState h = hs[1];
     if (st_final(h.status)) {
       *out = h;
       return PB_OK;
@@ -883,26 +889,44 @@ int krylov_run(pb_op2d* op, Work<T>& w, const T* b, const double* x0, double tol
     if (issued > (long)maxiter + 2L * chunk + 2)
       return pb_fail(ctx, PB_ERR_STATE, "bicgstab: %ld iterations issued without a final state (status %d)",
                      issued, h.status);
-    int rc = run_chunk<T>(op, w, chunk, exact, s);
-    if (rc) return rc;
+    if (host) {
+      launch_chunk<T>(op, w, chunk, exact, s);
+      PB_LAUNCH_CHECK(ctx);
+    } else {
+      int rc = run_chunk<T>(op, w, chunk, exact, s);
+      if (rc) return rc;
+    }
     issued += chunk;
   }
 }
 
-int true_residual(pb_op2d* op, const double* b, const double* x, hipStream_t s, double* nrm) {
+// Buffers of a true-residual evaluation: r = b - A x (n), block partials
+// (one per tile of the 4-row tiling), the device sum and its host copy.
+struct TrBufs {
+  double* r;
+  double* PT;
+  double* red;
+  double* hred;
+};
+
+int true_residual_in(pb_op2d* op, const TrBufs& tb, const double* b, const double* x, hipStream_t s, double* nrm) {
   const dim3 grd = stencil_grid(op, 4), blk(TX, TYT);
   const OpDev<double> d = opdev<double>(op);
   const bool c = op->cross, h = op->center;
-  if (c && h) hipLaunchKernelGGL((k_true_res<true, true, 4>), grd, blk, 0, s, d, x, b, op->rtrue, op->PT);
-  else if (c) hipLaunchKernelGGL((k_true_res<true, false, 4>), grd, blk, 0, s, d, x, b, op->rtrue, op->PT);
-  else if (h) hipLaunchKernelGGL((k_true_res<false, true, 4>), grd, blk, 0, s, d, x, b, op->rtrue, op->PT);
-  else hipLaunchKernelGGL((k_true_res<false, false, 4>), grd, blk, 0, s, d, x, b, op->rtrue, op->PT);
-  hipLaunchKernelGGL(k_sum1, dim3(1), dim3(TX, TYT), 0, s, (const double*)op->PT, (int)(grd.x * grd.y), op->red);
+  if (c && h) hipLaunchKernelGGL((k_true_res<true, true, 4>), grd, blk, 0, s, d, x, b, tb.r, tb.PT);
+  else if (c) hipLaunchKernelGGL((k_true_res<true, false, 4>), grd, blk, 0, s, d, x, b, tb.r, tb.PT);
+  else if (h) hipLaunchKernelGGL((k_true_res<false, true, 4>), grd, blk, 0, s, d, x, b, tb.r, tb.PT);
+  else hipLaunchKernelGGL((k_true_res<false, false, 4>), grd, blk, 0, s, d, x, b, tb.r, tb.PT);
+  hipLaunchKernelGGL(k_sum1, dim3(1), dim3(TX, TYT), 0, s, (const double*)tb.PT, (int)(grd.x * grd.y), tb.red);
   PB_LAUNCH_CHECK(op->ctx);
-  PB_HIP(op->ctx, hipMemcpyAsync(op->hred, op->red, sizeof(double), hipMemcpyDeviceToHost, s));
+  PB_HIP(op->ctx, hipMemcpyAsync(tb.hred, tb.red, sizeof(double), hipMemcpyDeviceToHost, s));
   PB_HIP(op->ctx, hipStreamSynchronize(s));
-  *nrm = sqrt(op->hred[0]);
+  *nrm = sqrt(tb.hred[0]);
   return PB_OK;
+}
+
+int true_residual(pb_op2d* op, const double* b, const double* x, hipStream_t s, double* nrm) {
+  return true_residual_in(op, TrBufs{op->rtrue, op->PT, op->red, op->hred}, b, x, s, nrm);
 }
 
 // Iteration cap of one FP32 inner run of the mixed mode: FP32 BiCGStab can
@@ -1086,6 +1110,94 @@ extern "C" int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x, const 
   }
   double tr = 0.0;
   rc = true_residual(op, b, x, s, &tr);
+  if (rc) return rc;
+  out.true_rel_resid = out.bnrm2 > 0 ? tr / out.bnrm2 : tr;
+  *info = out;
+  return out.info;
+}
+
+namespace {
+constexpr long kAlignWords = 32;   // 256-byte alignment of every scratch region
+inline long align_words(long w) { return (w + kAlignWords - 1) / kAlignWords * kAlignWords; }
+constexpr long kStateWords = (long)((3 * sizeof(KState) + sizeof(double) - 1) / sizeof(double));
+
+// Words of static scratch an FP64 solve of ``op`` needs: eight Krylov vectors
+// (r, p, v double-buffered, t, r~), the partial records of the finest tiling
+// (A 1, B 5, exact-update 2, true residual 1 per tile), the true-residual sum
+// and three state records.
+long noalloc_words(const pb_op2d* op) {
+  const dim3 sg = stencil_grid(op, 2);
+  const long nb = (long)sg.x * sg.y;
+  return 8 * align_words(op->n) + align_words(nb) + align_words(5 * nb) + align_words(2 * nb) + align_words(nb) +
+         align_words(2) + align_words(kStateWords);
+}
+}  // namespace
+
+extern "C" int pb_bicgstab_solve_noalloc(pb_op2d* op, const double* b, double* x, const pb_solve_opts* opts,
+                                         pb_solve_info* info) {
+  if (!op) return PB_ERR_ARG;
+  pb_ctx* ctx = op->ctx;
+  if (!b || !x || !opts || !info) return pb_fail(ctx, PB_ERR_ARG, "pb_bicgstab_solve_noalloc: null argument");
+  if (!op->assembled) return pb_fail(ctx, PB_ERR_STATE, "pb_bicgstab_solve_noalloc: operator not assembled");
+  if (opts->mode != PB_SOLVE_F64 && opts->mode != PB_SOLVE_F64_FAST)
+    return pb_fail(ctx, PB_ERR_ARG, "pb_bicgstab_solve_noalloc: mode %d; only PB_SOLVE_F64 and PB_SOLVE_F64_FAST",
+                   opts->mode);
+  if (!(opts->tol >= 0) || opts->maxiter < 1)
+    return pb_fail(ctx, PB_ERR_ARG, "pb_bicgstab_solve_noalloc: bad tol/maxiter");
+  if (!ctx->scratch)
+    return pb_fail(ctx, PB_ERR_STATE, "pb_bicgstab_solve_noalloc: static scratch unavailable on device %d (%s)",
+                   ctx->device, hipGetErrorString((hipError_t)ctx->scratch_err));
+  const long need = noalloc_words(op);
+  const long have = (long)(ctx->scratch_bytes / sizeof(double));
+  if (need > have)
+    return pb_fail(ctx, PB_ERR_ARG,
+                   "pb_bicgstab_solve_noalloc: a %d x %d grid needs %ld bytes of scratch, the static scratch holds "
+                   "%ld (about %ld nodes at most)",
+                   op->g.nx, op->g.ny, need * 8, have * 8, (have - 4096) / 8);
+  // carve the scratch: every region is written before it is read within one
+  // call (vectors by the init launch, partial records by the launch before
+  // their readers, the state record by k_put_state), so nothing is cleared
+  double* base = ctx->scratch;
+  long off = 0;
+  auto take = [&](long words) {
+    double* p = base + off;
+    off += align_words(words);
+    return p;
+  };
+  const dim3 sg = stencil_grid(op, 2);
+  const long nb = (long)sg.x * sg.y;
+  const long n = op->n;
+  Work<double> w;
+  for (int k = 0; k < 2; ++k) {
+    w.r[k] = take(n);
+    w.p[k] = take(n);
+    w.v[k] = take(n);
+  }
+  w.t = take(n);
+  w.rt = take(n);
+  w.x = x;   // the iterate lives in the caller's output
+  w.PA = take(nb);
+  w.PB = take(5 * nb);
+  w.PC = take(2 * nb);
+  double* PT = take(nb);
+  double* red = take(2);
+  w.S = reinterpret_cast<KState*>(take(kStateWords));
+  hipStream_t s = solve_stream(ctx);
+  KState hbuf[2];
+  KState h{};
+  const bool exact = opts->mode == PB_SOLVE_F64;
+  int rc = krylov_run<double>(op, w, b, opts->x0_given ? x : nullptr, opts->tol, opts->maxiter,
+                              opts->x0_given ? 0 : 1, opts->check_every, s, &h, exact, hbuf);
+  if (rc) return rc;
+  pb_solve_info out{};
+  out.info = info_of(h);
+  out.iters = h.iter;
+  out.outer = 1;
+  out.half_step = h.half;
+  out.resid = h.resid;
+  out.bnrm2 = h.bnrm2;
+  double hred = 0.0, tr = 0.0;
+  rc = true_residual_in(op, TrBufs{w.t, PT, red, &hred}, b, x, s, &tr);   // t is free after the run
   if (rc) return rc;
   out.true_rel_resid = out.bnrm2 > 0 ? tr / out.bnrm2 : tr;
   *info = out;

@@ -162,6 +162,20 @@ class Operator2D:
             self.ctx.check(rc, "pb_bicgstab_solve")
         return x, info.as_dict()
 
+    def solve_noalloc(self, b, x, tol=1e-8, maxiter=6000, mode="f64fast", check_every=32, x0_given=False):
+        """The FP64 solve through pb_bicgstab_solve_noalloc: no allocation by
+        the library (the caller passes b and x; scratch is the library's static
+        device array).  mode "f64" or "f64fast"."""
+        opts = _abi.SolveOpts(_MODES[mode], float(tol), int(maxiter), 0.0, 0, int(check_every),
+                              1 if x0_given else 0, 0, 0, 0, 0.0, 0)
+        info = _abi.SolveInfo()
+        self.ctx.sync_stream()
+        rc = self.ctx.lib.pb_bicgstab_solve_noalloc(self.h, _abi.ptr(b), _abi.ptr(x), C.byref(opts),
+                                                    C.byref(info))
+        if rc not in (-10, -11):
+            self.ctx.check(rc, "pb_bicgstab_solve_noalloc")
+        return x, info.as_dict()
+
     def attach_nn(self, net):
         """Use ``net`` (pybella_amd.nn.Network) for mode="nn" solves (None detaches)."""
         self.ctx.check(self.ctx.lib.pb_op2d_attach_nn(self.h, net.h if net is not None else None),
