@@ -52,7 +52,8 @@ extern "C" {
 #define PB_SOLVE_F64 0      /* FP64 BiCGStab, reference semantics: rho = rt.r and ||r||
                                from the stored r (3 launches per iteration)      */
 #define PB_SOLVE_MIXED 1    /* FP32 BiCGStab inside FP64 iterative refinement  */
-#define PB_SOLVE_NN 2       /* learned preconditioner, flexible outer Krylov   */
+#define PB_SOLVE_NN 2       /* FGMRES preconditioned by the multigrid cycle with
+                               the network as its coarse solver                 */
 #define PB_SOLVE_F64_FAST 3 /* FP64 BiCGStab, rho and ||r|| from s/t dots
                                (2 launches per iteration)                      */
 
@@ -94,17 +95,19 @@ typedef struct pb_solve_opts {
   double tol;          /* ud.tol (legacy semantics in PB_SOLVE_F64)           */
   int maxiter;         /* ud.max_iterations (per Krylov run); PB_SOLVE_NN: the
                           flexible-GMRES iteration budget before the fallback */
-  double inner_tol;    /* PB_SOLVE_MIXED / PB_SOLVE_NN: inner relative tol    */
-  int max_outer;       /* PB_SOLVE_MIXED / PB_SOLVE_NN: refinement cap        */
+  double inner_tol;    /* PB_SOLVE_MIXED: inner relative tol                  */
+  int max_outer;       /* PB_SOLVE_MIXED: refinement cap                      */
   int check_every;     /* iterations between host convergence checks (0=auto) */
   int x0_given;        /* 0: x0 = 0 as the reference; 1: use x as the guess   */
   int fallback_maxiter; /* PB_SOLVE_NN: BiCGStab iterations after the budget  */
-  int krylov_m;        /* PB_SOLVE_NN: FGMRES restart length (<=0: 40)         */
-  int sweeps;          /* PB_SOLVE_NN: damped-Jacobi sweeps after the network
-                          (<0: default 1)                                       */
-  double omega;        /* PB_SOLVE_NN: Jacobi damping (<=0: 0.6)               */
-  int use_network;     /* PB_SOLVE_NN: 1 network + Jacobi; 0 Jacobi only (the
-                          same flexible GMRES without the network, for A/B)    */
+  int krylov_m;        /* PB_SOLVE_NN: FGMRES restart length (<=0: 30, max 60) */
+  int sweeps;          /* PB_SOLVE_NN: Jacobi sweeps before and after each
+                          coarse correction of the cycle (<=0: 2, max 8)       */
+  double omega;        /* PB_SOLVE_NN: Jacobi damping (<=0: 0.8)               */
+  int use_network;     /* PB_SOLVE_NN coarse solve: 1 the network on its level;
+                          0 Jacobi sweeps on that level (the same cycle without
+                          the network, for A/B); 2 the cycle continued to the
+                          coarsest level (no network)                          */
 } pb_solve_opts;
 
 typedef struct pb_solve_info {
@@ -238,9 +241,36 @@ int pb_nn_train_step(pb_nn* nn, int nimg, int h, int w, const float* inputs, con
 int pb_nn_apply(pb_nn* nn, const pb_op2d* op, const double* r, double* z);
 /* the two coefficient features (float device arrays of nx*ny) of an operator */
 int pb_nn_features(pb_nn* nn, const pb_op2d* op, float* f1, float* f2);
-/* attach a network to an operator: PB_SOLVE_NN then uses it as the initial
- * guess and the preconditioner of flexible GMRES (NULL detaches) */
+/* attach a network to an operator: PB_SOLVE_NN then uses it as the coarse
+ * solver of the multigrid cycle that preconditions flexible GMRES (NULL
+ * detaches).  pb_nn_destroy detaches the network from every operator. */
 int pb_op2d_attach_nn(pb_op2d* op, pb_nn* nn);
+/* Null space the training loss removes from the output: 0 {constant,
+ * checkerboard} (default), 1 {constant} (a coarse level's operator). */
+int pb_nn_set_loss_nullspace(pb_nn* nn, int mode);
+
+/* ---- the multigrid cycle of PB_SOLVE_NN (no reference counterpart) ----
+ * The scaled 9-point operator couples, without Coriolis, only nodes of equal
+ * parity i + j; the cycle restricts r and checkerboard * r from the fine grid
+ * (two branches, carried as two channels), continues with Galerkin operators
+ * R A P (9-point stencils on unique grids: a periodic axis keeps n - 1 nodes)
+ * and solves one coarse level (the first whose grid, periodic nodes
+ * duplicated, is 8k+1 nodes per axis and at most 129 x 65) with the network.
+ * One application z = M v (v FP64, z FP32 Krylov vectors): coarse 0 the
+ * network, 1 Jacobi sweeps on that level, 2 the cycle continued to the
+ * coarsest level; nu Jacobi sweeps each way (damping omega); branches 1 or 2.
+ * Test and training aid. */
+int pb_op2d_precond(pb_op2d* op, const double* v, float* z, int coarse, int nu, double omega, int branches);
+/* Levels of the hierarchy: dims[4 l + 0..3] = ny, nx, periodic y, periodic x
+ * (unique grid) for l < max_levels; *nn_level = the network's level or -1.
+ * Returns the level count. */
+int pb_op2d_mg_info(pb_op2d* op, int* dims, int max_levels, int* nn_level);
+/* FP64 stencil [9][ny][nx] of a level (device pointer); entry k multiplies
+ * the node at row offset k / 3 - 1, column offset k % 3 - 1. */
+int pb_op2d_mg_stencil(pb_op2d* op, int level, double* st);
+/* The network level's input features on its duplicated grid (device float
+ * arrays of nh * nw values): -(W + E) / C and -(N + S) / C of its stencil. */
+int pb_op2d_mg_features(pb_op2d* op, float* f1, float* f2);
 
 /* ---- Batched uneven 2D solves (ensemble members of any grid sizes) ----
  * No reference counterpart: the reference solves members one after the other

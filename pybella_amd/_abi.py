@@ -99,6 +99,11 @@ SIGNATURES = {
     "pb_nn_apply": (I, [VP, VP, VP, VP]),
     "pb_nn_features": (I, [VP, VP, VP, VP]),
     "pb_op2d_attach_nn": (I, [VP, VP]),
+    "pb_nn_set_loss_nullspace": (I, [VP, I]),
+    "pb_op2d_precond": (I, [VP, VP, VP, I, I, D, I]),
+    "pb_op2d_mg_info": (I, [VP, C.POINTER(I), I, C.POINTER(I)]),
+    "pb_op2d_mg_stencil": (I, [VP, I, VP]),
+    "pb_op2d_mg_features": (I, [VP, VP, VP]),
 }
 
 _lib = None

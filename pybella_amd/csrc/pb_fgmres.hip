@@ -1,205 +1,267 @@
-// Flexible GMRES with the learned preconditioner (gfx950).  No reference
-// counterpart: it replaces the reference's bicgstab call
-// (second_projection.py:230) when a network is attached to the operator.
+// Flexible GMRES preconditioned by the two-branch multigrid cycle with the
+// learned coarse solver (gfx950).  No reference counterpart: it replaces the
+// reference's bicgstab call (second_projection.py:230) in the learned mode.
 //
-// Outer loop in FP64 (iterative refinement): r = b - A x (FP64 true residual,
-// stopping at ||r|| <= tol ||b||).  Inner loop: FGMRES(m) on A d = r with an
-// FP32 Krylov basis V, preconditioned directions Z_j = M(V_j) where M is the
-// network (Saad 1993, "A flexible inner-outer preconditioned GMRES") followed
-// by optional damped-Jacobi sweeps with the exact diagonal of A; classical
-// Gram-Schmidt applied twice per step (Giraud et al. 2005); Givens rotations
-// and the small least-squares solve on the host in FP64.  The first outer step
-// starts from x0 = M(b) (the network's initial guess).  If the iteration
-// budget runs out before the tolerance, plain BiCGStab finishes from the
-// current iterate.
-#include <type_traits>
-#include <vector>
+// FGMRES(m) (Saad 1993, "A flexible inner-outer preconditioned GMRES") in
+// FP64 on the scaled system (D^-1 A) x = b: the basis V and the Hessenberg
+// matrix are FP64; the preconditioned directions Z_j = M(V_j) come from the
+// FP32 multigrid cycle (pb_mg2d.hpp) whose coarse level is solved by the BF16
+// network.  Because the method is flexible, x = x0 + Z y is exact for the
+// Z actually produced, so the FP32 / BF16 preconditioner limits only the
+// convergence rate, not the attainable FP64 residual.
+//
+// Each Arnoldi step is the cycle, w = A Z_j (FP64), classical Gram-Schmidt
+// applied twice (Giraud et al. 2005) and the normalisation; the Givens
+// rotations and the least-squares residual estimate |g_{j+1}| are formed on
+// the device by the normalisation kernel's first thread.  Reductions are
+// per-block partial records that every consuming block re-reduces in the same
+// fixed order (deterministic, no grid-wide fence).  The host reads the
+// estimate one iteration behind the device (pinned copy + event), so it never
+// waits between iterations; it waits at once only when the convergence rate
+// predicts that the step just issued reaches the tolerance.  After a cycle,
+// x is updated on the device, the FP64 true residual is recomputed, and a
+// cycle that did not reduce it is undone.
+#include <algorithm>
+#include <cmath>
 
-#include "pb_op2d.hpp"
+#include "pb_mg2d.hpp"
 
 using namespace pb2d;
 
-int pb_nn_apply_f32(pb_nn* nn, const pb_op2d* op, const float* r, float* z, hipStream_t s);
+void pb_nn_track(pb_nn* nn, pb_op2d* op, bool add);
 
 namespace {
+
+constexpr int kFgBlocks = 128;   // blocks of the reduction kernels
+constexpr int kFgThreads = 256;
+constexpr int kMaxM = 60;        // restart length cap (K = m + 1 <= 64 column sums per record)
+
+}  // namespace
 
 struct FgWork {
   int m = 0;
   long n = 0;
-  float* V = nullptr;     // (m+1) x n
-  float* Z = nullptr;     // m x n
-  float* w = nullptr;
-  float* t = nullptr;
-  float* r32 = nullptr;   // FP32 copy of the outer residual
-  float* diag = nullptr;  // exact diagonal of A (FP32)
-  double* part = nullptr; // dot partials
-  double* h = nullptr;    // device dot results (m+2)
-  double* hh = nullptr;   // pinned host mirror
+  double* V = nullptr;       // (m + 1) x n basis
+  float* Z = nullptr;        // m x n preconditioned directions
+  double* w = nullptr;       // n
+  double* w2 = nullptr;      // n
+  double* part = nullptr;    // [kFgBlocks][m + 1] partial records (three sets)
+  double* part2 = nullptr;
+  double* part3 = nullptr;
+  double* H = nullptr;       // (m + 1) x m rotated Hessenberg, column-major
+  double* cs = nullptr;      // m
+  double* sn = nullptr;      // m
+  double* g = nullptr;       // m + 1
+  double* h1 = nullptr;      // m + 1 (first Gram-Schmidt pass)
+  double* h2 = nullptr;      // m + 1 (second pass)
+  double* est = nullptr;     // m residual estimates |g_{j+1}|
+  double* y = nullptr;       // m least-squares solution
+  double* hest = nullptr;    // pinned host copy of est
+  hipEvent_t ev[kMaxM] = {};
   std::vector<void*> allocs;
 };
 
-std::map<const pb_op2d*, FgWork*> g_work;
-std::map<const pb_op2d*, pb_nn*> g_attached;
+namespace {
 
-constexpr int kDotBlocks = 128;
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
 
-// diag(A)[n] = dinv * (hc - sum of the 4 cells' (Wxx + Wyy)): the coefficient
-// of p_self in the flux form (each cell's gx and gy change by -1 per unit p_self)
-__global__ void k_diag(int nx, int ny, const float* wxx, const float* wyy, const float* hc, const float* dinv,
-                       int center, float* d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= nx || j >= ny) return;
-  const long c00 = (long)j * (nx + 1) + i, c01 = c00 + 1, c10 = c00 + nx + 1, c11 = c10 + 1;
-  const long g = (long)j * nx + i;
-  const float s = (wxx[c00] + wxx[c01] + wxx[c10] + wxx[c11]) + (wyy[c00] + wyy[c01] + wyy[c10] + wyy[c11]);
-  d[g] = dinv[g] * ((center ? hc[g] : 0.f) - s);
+// w = (D^-1 A) z in FP64 for an FP32 direction z
+struct ZIn {
+  using Raw = float;
+  const float* z;
+  __device__ Raw load(long g) const { return z[g]; }
+  __device__ double combine(const Raw& q) const { return (double)q; }
+};
+struct WOut {
+  struct Pre {};
+  double* w;
+  __device__ Pre prefetch(long) const { return Pre{}; }
+  template <class R>
+  __device__ void operator()(int, long g, double y, double, const Pre&, const R&) {
+    w[g] = y;
+  }
+};
+template <bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void kf_matvec(OpDev<double> op, const float* z, double* w) {
+  __shared__ TileLds<double, RY> L;
+  ZIn in{z};
+  WOut out{w};
+  stencil_tile2<double, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, [] { return true; });
 }
 
-// partial dots of w with V[0..k-1]: grid (kDotBlocks, k)
-__global__ void k_mdot(const float* V, long n, const float* w, double* part) {
-  const int i = blockIdx.y;
-  const float* v = V + (long)i * n;
-  double s = 0.0;
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
-    s += (double)v[g] * (double)w[g];
-  __shared__ double sh[256];
-  sh[threadIdx.x] = s;
+__device__ __forceinline__ long gstride() { return (long)gridDim.x * blockDim.x; }
+__device__ __forceinline__ long gfirst() { return blockIdx.x * (long)blockDim.x + threadIdx.x; }
+
+// column sums of a [kFgBlocks][KP] record set, the same order in every block:
+// thread k adds rows 0, 1, ... of column k
+__device__ __forceinline__ void reduce_cols(const double* part, int KP, int K, double* h) {
+  const int tid = threadIdx.x;
+  if (tid < K) {
+    double s = 0.0;
+    for (int b = 0; b < kFgBlocks; ++b) s += part[(long)b * KP + tid];
+    h[tid] = s;
+  }
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[(long)i * gridDim.x + blockIdx.x] = sh[0];
 }
 
-__global__ void k_reduce_parts(const double* part, int nb, int k, double* out) {
-  const int i = blockIdx.x;
-  if (i >= k) return;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(long)i * nb + b];
-  __shared__ double sh[256];
-  sh[threadIdx.x] = s;
+// this block's partial dots of V_0..V_{K-1} with x, written as record
+// blockIdx.x of ``part``; eight vectors per pass over x
+__device__ __forceinline__ void block_dots(const double* V, long n, int K, const double* x, double* part, int KP,
+                                           double* sh) {
+  for (int k0 = 0; k0 < K; k0 += 8) {
+    double acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+    for (long g = gfirst(); g < n; g += gstride()) {
+      const double xg = x[g];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (k0 + q < K) acc[q] += V[(long)(k0 + q) * n + g] * xg;
+    }
+    block_sum_w<8, kFgThreads / kWave>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (k0 + q < K) part[(long)blockIdx.x * KP + k0 + q] = acc[q];
+    }
+    __syncthreads();   // sh is reused by the next pass
+  }
+}
+
+// first Gram-Schmidt pass: partial dots V^T w
+__global__ __launch_bounds__(kFgThreads) void kf_dots(const double* V, long n, int K, const double* w, double* part,
+                                                      int KP) {
+  __shared__ double sh[8 * (kFgThreads / kWave)];
+  block_dots(V, n, K, w, part, KP, sh);
+}
+
+// h1 = V^T w (reduced), w2 = w - V h1, partial dots V^T w2
+__global__ __launch_bounds__(kFgThreads) void kf_orth1(const double* V, long n, int K, int KP, const double* part,
+                                                       const double* w, double* w2, double* part2, double* hout) {
+  __shared__ double h[kMaxM + 4];
+  __shared__ double sh[8 * (kFgThreads / kWave)];
+  reduce_cols(part, KP, K, h);
+  if (blockIdx.x == 0 && (int)threadIdx.x < K) hout[threadIdx.x] = h[threadIdx.x];
+  for (long g = gfirst(); g < n; g += gstride()) {
+    double a = w[g];
+    for (int k = 0; k < K; ++k) a -= h[k] * V[(long)k * n + g];
+    w2[g] = a;
+  }
+  block_dots(V, n, K, w2, part2, KP, sh);
+}
+
+// h2 = V^T w2 (reduced), w = w2 - V h2, partial ||w||^2
+__global__ __launch_bounds__(kFgThreads) void kf_orth2(const double* V, long n, int K, int KP, const double* part2,
+                                                       const double* w2, double* w, double* part3, double* hout) {
+  __shared__ double h[kMaxM + 4];
+  __shared__ double sh[kFgThreads / kWave];
+  reduce_cols(part2, KP, K, h);
+  if (blockIdx.x == 0 && (int)threadIdx.x < K) hout[threadIdx.x] = h[threadIdx.x];
+  double ss[1] = {0.0};
+  for (long g = gfirst(); g < n; g += gstride()) {
+    double a = w2[g];
+    for (int k = 0; k < K; ++k) a -= h[k] * V[(long)k * n + g];
+    w[g] = a;
+    ss[0] += a * a;
+  }
+  block_sum_w<1, kFgThreads / kWave>(ss, sh);
+  if (threadIdx.x == 0) part3[blockIdx.x] = ss[0];
+}
+
+// V_{j+1} = w / ||w||; the first thread of block 0 completes Hessenberg column
+// j (h1 + h2, ||w||), applies the previous Givens rotations, forms rotation j
+// and updates g; est[j] = |g_{j+1}|
+__global__ __launch_bounds__(kFgThreads) void kf_norm(int j, int m, long n, const double* part3, const double* w,
+                                                      double* vnext, const double* h1, const double* h2, double* H,
+                                                      double* cs, double* sn, double* g, double* est) {
+  __shared__ double hn_s;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int b = 0; b < kFgBlocks; ++b) s += part3[b];
+    hn_s = sqrt(s);
+  }
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-    __syncthreads();
+  const double hn = hn_s;
+  const double inv = hn > 0.0 ? 1.0 / hn : 0.0;
+  for (long q = gfirst(); q < n; q += gstride()) vnext[q] = w[q] * inv;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double* col = H + (long)j * (m + 1);
+    for (int i = 0; i <= j; ++i) col[i] = h1[i] + h2[i];
+    col[j + 1] = hn;
+    for (int i = 0; i < j; ++i) {
+      const double a = col[i], b = col[i + 1];
+      col[i] = cs[i] * a + sn[i] * b;
+      col[i + 1] = -sn[i] * a + cs[i] * b;
+    }
+    const double a = col[j], b = col[j + 1];
+    const double r = hypot(a, b);
+    cs[j] = r > 0.0 ? a / r : 1.0;
+    sn[j] = r > 0.0 ? b / r : 0.0;
+    col[j] = r;
+    col[j + 1] = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    est[j] = fabs(g[j + 1]);
   }
-  if (threadIdx.x == 0) out[i] = sh[0];
 }
 
-// w -= sum_i h[i] V[i]; partials of w.w into part (one row)
-__global__ void k_orth_update(const float* V, long n, int k, const double* h, float* w, double* part) {
-  double s = 0.0;
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
-    double acc = (double)w[g];
-    for (int i = 0; i < k; ++i) acc -= h[i] * (double)V[(long)i * n + g];
-    const float wf = (float)acc;
-    w[g] = wf;
-    s += (double)wf * (double)wf;
+__global__ void k_fill_zero_d(long n, double* x) {
+  for (long q = gfirst(); q < n; q += gstride()) x[q] = 0.0;
+}
+
+// V_0 = r / beta, g = (beta, 0, ...)
+__global__ void kf_start(long n, int m, const double* r, double beta, double* v0, double* g) {
+  const double inv = 1.0 / beta;
+  for (long q = gfirst(); q < n; q += gstride()) v0[q] = r[q] * inv;
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i <= m; i += blockDim.x) g[i] = i == 0 ? beta : 0.0;
   }
-  __shared__ double sh[256];
-  sh[threadIdx.x] = s;
+}
+
+// back substitution of the k x k upper-triangular rotated Hessenberg: H y = g
+__global__ void kf_backsub(int k, int m, const double* H, const double* g, double* y) {
+  if (threadIdx.x != 0) return;
+  for (int i = k - 1; i >= 0; --i) {
+    double acc = g[i];
+    for (int c = i + 1; c < k; ++c) acc -= H[(long)c * (m + 1) + i] * y[c];
+    const double d = H[(long)i * (m + 1) + i];
+    y[i] = d != 0.0 ? acc / d : 0.0;
+  }
+}
+
+// x += sign * sum_{i < k} y_i Z_i
+__global__ void kf_update(long n, int k, const double* y, double sign, const float* Z, double* x) {
+  __shared__ double ys[kMaxM];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) ys[i] = sign * y[i];
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
-__global__ void k_scale_to(long n, const float* w, const double* nrm2, float* v) {
-  const double inv = 1.0 / sqrt(*nrm2);
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
-    v[g] = (float)((double)w[g] * inv);
-}
-
-__global__ void k_combine(const float* Z, long n, int k, const double* y, double* x) {
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+  for (long q = gfirst(); q < n; q += gstride()) {
     double acc = 0.0;
-    for (int i = 0; i < k; ++i) acc += y[i] * (double)Z[(long)i * n + g];
-    x[g] += acc;
+    for (int i = 0; i < k; ++i) acc += ys[i] * (double)Z[(long)i * n + q];
+    x[q] += acc;
   }
 }
 
-// z += omega (v - t) / diag
-__global__ void k_jacobi(long n, const float* v, const float* t, const float* d, float omega, float* z) {
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
-    z[g] += omega * (v[g] - t[g]) / d[g];
-}
-
-// z = omega v / diag (first Jacobi step from z = 0)
-__global__ void k_jacobi0(long n, const float* v, const float* d, float omega, float* z) {
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
-    z[g] = omega * v[g] / d[g];
-}
-
-// partial sums of v.t and t.t (one pair per block)
-__global__ void k_dot2(long n, const float* v, const float* t, double* part) {
-  double a = 0.0, b = 0.0;
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
-    const double tv = (double)t[g];
-    a += (double)v[g] * tv;
-    b += tv * tv;
-  }
-  __shared__ double sh[2][256];
-  sh[0][threadIdx.x] = a;
-  sh[1][threadIdx.x] = b;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = sh[0][0];
-    part[2 * blockIdx.x + 1] = sh[1][0];
-  }
-}
-
-// alpha = (v.t) / (t.t) minimises ||v - alpha A z|| (every block re-reduces the
-// partials in the same order, so all blocks see the same alpha); then
-// z = alpha z + omega (v - alpha t) / diag
-__global__ void k_scale_jacobi(long n, const double* part, int nparts, const float* v, const float* t, const float* d,
-                               float omega, float* z) {
-  __shared__ float alpha_s;
-  if (threadIdx.x == 0) {
-    double a = 0.0, b = 0.0;
-    for (int k = 0; k < nparts; ++k) {
-      a += part[2 * k];
-      b += part[2 * k + 1];
-    }
-    alpha_s = b > 0.0 ? (float)(a / b) : 1.f;
-  }
-  __syncthreads();
-  const float alpha = alpha_s;
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
-    z[g] = alpha * z[g] + omega * (v[g] - alpha * t[g]) / d[g];
-}
-
-__global__ void k_cast_df(long n, const double* a, float* b) {
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) b[g] = (float)a[g];
-}
-__global__ void k_cast_fd(long n, const float* a, double* b) {
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) b[g] = (double)a[g];
-}
-__global__ void k_axpy_d(long n, double a, const double* dx, double* x) {
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) x[g] += a * dx[g];
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+void fg_free(FgWork* f) {
+  if (!f) return;
+  for (void* p : f->allocs) (void)hipFree(p);
+  if (f->hest) (void)hipHostFree(f->hest);
+  for (hipEvent_t& e : f->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete f;
 }
 
 int fg_alloc(pb_op2d* op, int m, FgWork** out) {
-  auto it = g_work.find(op);
-  if (it != g_work.end() && it->second->m >= m && it->second->n == op->n) {
-    *out = it->second;
+  if (op->fg && op->fg->m >= m && op->fg->n == op->n) {
+    *out = op->fg;
     return PB_OK;
   }
-  if (it != g_work.end()) {
-    for (void* p : it->second->allocs) (void)hipFree(p);
-    if (it->second->hh) (void)hipHostFree(it->second->hh);
-    delete it->second;
-    g_work.erase(it);
-  }
+  fg_free(op->fg);
+  op->fg = nullptr;
   FgWork* f = new FgWork();
   f->m = m;
   f->n = op->n;
@@ -210,245 +272,182 @@ int fg_alloc(pb_op2d* op, int m, FgWork** out) {
     *p = (std::remove_reference_t<decltype(*p)>)q;
     return 0;
   };
+  const long n = op->n;
   int rc = 0;
-  rc |= A(&f->V, (long)(m + 1) * op->n);
-  rc |= A(&f->Z, (long)m * op->n);
-  rc |= A(&f->w, op->n);
-  rc |= A(&f->t, op->n);
-  rc |= A(&f->r32, op->n);
-  rc |= A(&f->diag, op->n);
-  rc |= A(&f->part, (long)(m + 2) * kDotBlocks);
-  rc |= A(&f->h, m + 2);
+  rc |= A(&f->V, (long)(m + 1) * n);
+  rc |= A(&f->Z, (long)m * n);
+  rc |= A(&f->w, n);
+  rc |= A(&f->w2, n);
+  rc |= A(&f->part, (long)kFgBlocks * (m + 1));
+  rc |= A(&f->part2, (long)kFgBlocks * (m + 1));
+  rc |= A(&f->part3, kFgBlocks);
+  rc |= A(&f->H, (long)(m + 1) * m);
+  rc |= A(&f->cs, m);
+  rc |= A(&f->sn, m);
+  rc |= A(&f->g, m + 1);
+  rc |= A(&f->h1, m + 1);
+  rc |= A(&f->h2, m + 1);
+  rc |= A(&f->est, m);
+  rc |= A(&f->y, m);
   void* hp = nullptr;
-  if (rc || hipHostMalloc(&hp, sizeof(double) * (m + 2), hipHostMallocDefault) != hipSuccess) {
-    for (void* p : f->allocs) (void)hipFree(p);
-    delete f;
+  if (!rc && hipHostMalloc(&hp, sizeof(double) * m, hipHostMallocDefault) != hipSuccess) rc = 1;
+  f->hest = (double*)hp;
+  for (int i = 0; i < m && !rc; ++i)
+    if (hipEventCreateWithFlags(&f->ev[i], hipEventDisableTiming) != hipSuccess) rc = 1;
+  if (rc) {
+    fg_free(f);
     return pb_fail(op->ctx, PB_ERR_HIP, "fgmres: workspace allocation failed");
   }
-  f->hh = (double*)hp;
-  g_work[op] = f;
+  op->fg = f;
   *out = f;
   return PB_OK;
 }
 
-// norm^2 of a float vector into f->h[0] (device), returned on the host
-int fnorm2(pb_op2d* op, FgWork* f, const float* v, hipStream_t s, double* out) {
-  hipLaunchKernelGGL(k_mdot, dim3(kDotBlocks, 1), dim3(256), 0, s, v, op->n, v, f->part);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, (const double*)f->part, kDotBlocks, 1, f->h);
-  PB_HIP(op->ctx, hipMemcpyAsync(f->hh, f->h, sizeof(double), hipMemcpyDeviceToHost, s));
-  PB_HIP(op->ctx, hipStreamSynchronize(s));
-  *out = f->hh[0];
-  return PB_OK;
+void launch_fg_matvec(const pb_op2d* op, const float* z, double* w, hipStream_t s) {
+  const OpDev<double> d = opdev<double>(op);
+  const dim3 grd = stencil_grid(op, 4), blk(TX, TYT);
+  if (op->cross && op->center) hipLaunchKernelGGL((kf_matvec<true, true, 4>), grd, blk, 0, s, d, z, w);
+  else if (op->cross) hipLaunchKernelGGL((kf_matvec<true, false, 4>), grd, blk, 0, s, d, z, w);
+  else if (op->center) hipLaunchKernelGGL((kf_matvec<false, true, 4>), grd, blk, 0, s, d, z, w);
+  else hipLaunchKernelGGL((kf_matvec<false, false, 4>), grd, blk, 0, s, d, z, w);
 }
 
-// z = M(v): the network, then ``sweeps`` damped-Jacobi steps on A z = v; the
-// first sweep also rescales the network output by the alpha that minimises
-// ||v - alpha A z|| (computed on the device, no host round trip).  v, z and
-// f->t must be distinct buffers.  With ``rescale`` the alpha step is done even
-// when sweeps == 0 (the initial guess needs the right magnitude; flexible GMRES
-// directions do not).
-int precond(pb_op2d* op, pb_nn* nn, FgWork* f, const float* v, float* z, int sweeps, float omega, bool rescale,
-            hipStream_t s) {
-  const int nb = elem_blocks(op->n);
-  if (!nn) {   // Jacobi only: z = omega v / diag, then the remaining sweeps
-    hipLaunchKernelGGL(k_jacobi0, dim3(nb), dim3(256), 0, s, op->n, v, (const float*)f->diag, omega, z);
-    for (int k = 1; k < sweeps; ++k) {
-      void* saved = op->ctx->stream;
-      op->ctx->stream = s;
-      int rc = pb_op2d_matvec(op, z, f->t, PB_F32);
-      op->ctx->stream = (hipStream_t)saved;
-      if (rc) return rc;
-      hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(256), 0, s, op->n, v, (const float*)f->t, (const float*)f->diag,
-                         omega, z);
-    }
-    PB_LAUNCH_CHECK(op->ctx);
-    return PB_OK;
-  }
-  int rc = pb_nn_apply_f32(nn, op, v, z, s);
+// One Arnoldi step j: Z_j = M(V_j), then V_{j+1}, Hessenberg column j, est[j].
+int arnoldi_step(pb_op2d* op, pb_nn* nn, FgWork* f, int j, const pbmg::CycleOpts& co, hipStream_t s) {
+  const long n = op->n;
+  const int m = f->m, KP = m + 1, K = j + 1;
+  double* vj = f->V + (long)j * n;
+  float* zj = f->Z + (long)j * n;
+  int rc = pbmg::cycle(op, nn, vj, zj, co, s);
   if (rc) return rc;
-  const int steps = (sweeps == 0 && rescale) ? 1 : sweeps;
-  if (sweeps == 0) omega = 0.f;
-  for (int k = 0; k < steps; ++k) {
-    void* saved = op->ctx->stream;
-    op->ctx->stream = s;
-    rc = pb_op2d_matvec(op, z, f->t, PB_F32);
-    op->ctx->stream = (hipStream_t)saved;
-    if (rc) return rc;
-    if (k == 0) {
-      hipLaunchKernelGGL(k_dot2, dim3(kDotBlocks), dim3(256), 0, s, op->n, v, (const float*)f->t, f->part);
-      hipLaunchKernelGGL(k_scale_jacobi, dim3(nb), dim3(256), 0, s, op->n, (const double*)f->part, kDotBlocks, v,
-                         (const float*)f->t, (const float*)f->diag, omega, z);
-    } else {
-      hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(256), 0, s, op->n, v, (const float*)f->t, (const float*)f->diag,
-                         omega, z);
-    }
-  }
+  launch_fg_matvec(op, zj, f->w, s);
+  const dim3 gb(kFgBlocks), tb(kFgThreads);
+  hipLaunchKernelGGL(kf_dots, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->w, f->part, KP);
+  hipLaunchKernelGGL(kf_orth1, gb, tb, 0, s, (const double*)f->V, n, K, KP, (const double*)f->part,
+                     (const double*)f->w, f->w2, f->part2, f->h1);
+  hipLaunchKernelGGL(kf_orth2, gb, tb, 0, s, (const double*)f->V, n, K, KP, (const double*)f->part2,
+                     (const double*)f->w2, f->w, f->part3, f->h2);
+  hipLaunchKernelGGL(kf_norm, gb, tb, 0, s, j, m, n, (const double*)f->part3, (const double*)f->w,
+                     f->V + (long)(j + 1) * n, (const double*)f->h1, (const double*)f->h2, f->H, f->cs, f->sn, f->g,
+                     f->est);
   PB_LAUNCH_CHECK(op->ctx);
   return PB_OK;
 }
 
-// One FGMRES(m) cycle on A d = r32; adds d to x (FP64).  Returns the number
-// of iterations; *est = final residual estimate relative to ||r32||.
-int fgmres_cycle(pb_op2d* op, pb_nn* nn, FgWork* f, const float* r32, double* x, int m, double tol_rel, int sweeps,
-                 float omega, hipStream_t s, int* iters, double* est) {
+// One FGMRES(m) cycle from the residual in op->rtrue (norm rn): adds Z y to x.
+// Returns the number of Arnoldi steps used in *k.
+int fgmres_cycle(pb_op2d* op, pb_nn* nn, FgWork* f, double* x, int m, double rn, double thr,
+                 const pbmg::CycleOpts& co, hipStream_t s, int* kout) {
   const long n = op->n;
-  const int nb = elem_blocks(n);
-  double beta2 = 0.0;
-  int rc = fnorm2(op, f, r32, s, &beta2);
-  if (rc) return rc;
-  const double beta = sqrt(beta2);
-  *iters = 0;
-  *est = 0.0;
-  if (beta == 0.0) return PB_OK;
-  // V0 = r / beta
-  PB_HIP(op->ctx, hipMemcpyAsync(f->h, &beta2, sizeof(double), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_scale_to, dim3(nb), dim3(256), 0, s, n, r32, (const double*)f->h, f->V);
-  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0);
-  g[0] = beta;
-  int j = 0;
-  for (; j < m; ++j) {
-    float* vj = f->V + (long)j * n;
-    float* zj = f->Z + (long)j * n;
-    rc = precond(op, nn, f, vj, zj, sweeps, omega, false, s);
+  pb_ctx* ctx = op->ctx;
+  hipLaunchKernelGGL(kf_start, dim3(kFgBlocks), dim3(kFgThreads), 0, s, n, f->m, (const double*)op->rtrue, rn, f->V,
+                     f->g);
+  int k = m;
+  double prev = rn;   // estimate one step before the last one read
+  for (int j = 0; j < m; ++j) {
+    int rc = arnoldi_step(op, nn, f, j, co, s);
     if (rc) return rc;
-    void* saved = op->ctx->stream;
-    op->ctx->stream = s;
-    rc = pb_op2d_matvec(op, zj, f->w, PB_F32);
-    op->ctx->stream = (hipStream_t)saved;
-    if (rc) return rc;
-    std::vector<double> hcol(j + 2, 0.0);
-    double wn2 = 0.0;
-    for (int pass = 0; pass < 2; ++pass) {
-      hipLaunchKernelGGL(k_mdot, dim3(kDotBlocks, j + 1), dim3(256), 0, s, (const float*)f->V, n, (const float*)f->w,
-                         f->part);
-      hipLaunchKernelGGL(k_reduce_parts, dim3(j + 1), dim3(256), 0, s, (const double*)f->part, kDotBlocks, j + 1, f->h);
-      hipLaunchKernelGGL(k_orth_update, dim3(kDotBlocks), dim3(256), 0, s, (const float*)f->V, n, j + 1,
-                         (const double*)f->h, f->w, f->part);
-      hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, (const double*)f->part, kDotBlocks, 1,
-                         f->h + j + 1);
-      PB_HIP(op->ctx, hipMemcpyAsync(f->hh, f->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s));
-      PB_HIP(op->ctx, hipStreamSynchronize(s));
-      for (int i = 0; i <= j; ++i) hcol[i] += f->hh[i];
-      wn2 = f->hh[j + 1];
-    }
-    const double hn = sqrt(wn2);
-    hcol[j + 1] = hn;
-    if (hn > 0.0) {
-      hipLaunchKernelGGL(k_scale_to, dim3(nb), dim3(256), 0, s, n, (const float*)f->w, (const double*)(f->h + j + 1),
-                         f->V + (long)(j + 1) * n);
-    }
-    // Givens
-    for (int i = 0; i < j; ++i) {
-      const double a = hcol[i], b = hcol[i + 1];
-      hcol[i] = cs[i] * a + sn[i] * b;
-      hcol[i + 1] = -sn[i] * a + cs[i] * b;
-    }
-    const double a = hcol[j], b = hcol[j + 1];
-    const double r = std::hypot(a, b);
-    cs[j] = r > 0 ? a / r : 1.0;
-    sn[j] = r > 0 ? b / r : 0.0;
-    hcol[j] = r;
-    hcol[j + 1] = 0.0;
-    g[j + 1] = -sn[j] * g[j];
-    g[j] = cs[j] * g[j];
-    for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hcol[i];
-    *est = std::fabs(g[j + 1]) / beta;
-    if (*est <= tol_rel || hn == 0.0) {
-      ++j;
+    PB_HIP(ctx, hipMemcpyAsync(f->hest + j, f->est + j, sizeof(double), hipMemcpyDeviceToHost, s));
+    PB_HIP(ctx, hipEventRecord(f->ev[j], s));
+    if (j == 0) continue;
+    // read step j - 1 while step j runs
+    PB_HIP(ctx, hipEventSynchronize(f->ev[j - 1]));
+    const double e1 = f->hest[j - 1];
+    if (e1 <= thr) {
+      k = j;
       break;
     }
+    // the rate so far predicts step j reaches the tolerance: wait for it now
+    const double rate = prev > 0.0 ? e1 / prev : 1.0;
+    prev = e1;
+    if (e1 * rate <= thr || j + 1 == m) {
+      PB_HIP(ctx, hipEventSynchronize(f->ev[j]));
+      if (f->hest[j] <= thr) {
+        k = j + 1;
+        break;
+      }
+    }
   }
-  const int k = j;
-  // back substitution H(0:k,0:k) y = g
-  std::vector<double> y(k, 0.0);
-  for (int i = k - 1; i >= 0; --i) {
-    double acc = g[i];
-    for (int c = i + 1; c < k; ++c) acc -= H[(size_t)i * m + c] * y[c];
-    y[i] = H[(size_t)i * m + i] != 0.0 ? acc / H[(size_t)i * m + i] : 0.0;
-  }
-  PB_HIP(op->ctx, hipMemcpyAsync(f->h, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, s, (const float*)f->Z, n, k, (const double*)f->h, x);
-  PB_LAUNCH_CHECK(op->ctx);
-  PB_HIP(op->ctx, hipStreamSynchronize(s));
-  *iters = k;
+  hipLaunchKernelGGL(kf_backsub, dim3(1), dim3(64), 0, s, k, f->m, (const double*)f->H, (const double*)f->g, f->y);
+  hipLaunchKernelGGL(kf_update, dim3(elem_blocks(n)), dim3(256), 0, s, n, k, (const double*)f->y, 1.0,
+                     (const float*)f->Z, x);
+  PB_LAUNCH_CHECK(ctx);
+  *kout = k;
   return PB_OK;
 }
 
 }  // namespace
 
-// frees the flexible-GMRES workspace of an operator (called on its destroy)
-void pb_fgmres_release(const pb_op2d* op) {
-  auto it = g_work.find(op);
-  if (it != g_work.end()) {
-    for (void* p : it->second->allocs) (void)hipFree(p);
-    if (it->second->hh) (void)hipHostFree(it->second->hh);
-    delete it->second;
-    g_work.erase(it);
-  }
-  g_attached.erase(op);
+// frees the learned mode's workspace of an operator (called on its destroy)
+void pb_fgmres_release(pb_op2d* op) {
+  fg_free(op->fg);
+  op->fg = nullptr;
+  pbmg::release(op);
+  if (op->nn) pb_nn_track(op->nn, op, false);
+  op->nn = nullptr;
 }
 
 extern "C" int pb_op2d_attach_nn(pb_op2d* op, pb_nn* nn) {
   if (!op) return PB_ERR_ARG;
-  if (nn) g_attached[op] = nn;
-  else g_attached.erase(op);
+  if (op->nn) pb_nn_track(op->nn, op, false);
+  op->nn = nn;
+  if (nn) pb_nn_track(nn, op, true);
   return PB_OK;
 }
 
-// Solve with the attached network: x0 = M(b), then FGMRES(m) refinement
-// cycles; falls back to BiCGStab (fast FP64) past the iteration budget.
-// Called by pb_bicgstab_solve for PB_SOLVE_NN.
+// The learned mode: FGMRES preconditioned by the multigrid cycle whose coarse
+// level the attached network solves, for a budget of opts->maxiter Arnoldi
+// steps.  Called by pb_bicgstab_solve for PB_SOLVE_NN, which falls back to
+// plain BiCGStab when the budget runs out.
 int pb_nn_solve(pb_op2d* op, const double* b, double* x, const pb_solve_opts* opts, pb_solve_info* info,
                 double (*true_res)(pb_op2d*, const double*, const double*, hipStream_t, int*), hipStream_t s) {
   pb_ctx* ctx = op->ctx;
-  pb_nn* nn = nullptr;
-  if (opts->use_network) {
-    auto it = g_attached.find(op);
-    if (it == g_attached.end()) return pb_fail(ctx, PB_ERR_STATE, "PB_SOLVE_NN: no network attached to the operator");
-    nn = it->second;
+  pbmg::CycleOpts co;
+  co.nu_pre = co.nu_post = opts->sweeps > 0 ? std::min(opts->sweeps, 8) : 2;
+  co.omega = opts->omega > 0 ? (float)opts->omega : 0.8f;
+  if (opts->use_network == 1) {
+    if (!op->nn) return pb_fail(ctx, PB_ERR_STATE, "PB_SOLVE_NN: no network attached to the operator");
+    co.coarse = pbmg::COARSE_NET;
+  } else if (opts->use_network == 2) {
+    co.coarse = pbmg::COARSE_MG;
+  } else {
+    co.coarse = pbmg::COARSE_SMOOTH;
   }
-  const int m = opts->krylov_m > 0 ? std::max(4, std::min(200, opts->krylov_m)) : 40;
-  const int sweeps = opts->sweeps >= 0 ? std::min(opts->sweeps, 16) : 1;
-  const float omega = opts->omega > 0 ? (float)opts->omega : 0.6f;
-  if (!opts->use_network && sweeps == 0)
-    return pb_fail(ctx, PB_ERR_ARG, "PB_SOLVE_NN without the network needs at least one Jacobi sweep");
+  const int m = opts->krylov_m > 0 ? std::max(2, std::min(kMaxM, opts->krylov_m)) : 30;
+  int rc = pbmg::build(op, s);
+  if (rc) return rc;
   FgWork* f = nullptr;
-  int rc = fg_alloc(op, m, &f);
+  rc = fg_alloc(op, m, &f);
   if (rc) return rc;
   const long n = op->n;
   const int nb = elem_blocks(n);
-  hipLaunchKernelGGL(k_diag, dim3(ceil_div(op->g.nx, 256), op->g.ny), dim3(256), 0, s, op->g.nx, op->g.ny,
-                     (const float*)op->wxx32, (const float*)op->wyy32, (const float*)op->hc32,
-                     (const float*)op->dinv32, op->center ? 1 : 0, f->diag);
-  const double inner = opts->inner_tol > 0 ? opts->inner_tol : 1e-4;
-  const int budget = opts->maxiter;   // total FGMRES iterations before the fallback
   int rcode = 0;
-  // ||b|| and the initial guess x0 = M(b) (or the caller's x0)
   const double bn = true_res(op, b, op->zero, s, &rcode);
   if (rcode) return rcode;
-  if (!opts->x0_given) {   // V[0], Z[0] are free until the first cycle
-    hipLaunchKernelGGL(k_cast_df, dim3(nb), dim3(256), 0, s, n, b, f->V);
-    rc = precond(op, nn, f, f->V, f->Z, sweeps, omega, true, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_cast_fd, dim3(nb), dim3(256), 0, s, n, (const float*)f->Z, x);
-  }
+  if (!opts->x0_given) hipLaunchKernelGGL(k_fill_zero_d, dim3(nb), dim3(256), 0, s, n, x);
   double rn = true_res(op, b, x, s, &rcode);   // op->rtrue = b - A x
   if (rcode) return rcode;
   const double target = opts->tol * bn;
+  const int budget = opts->maxiter;
   int total = 0, outer = 0;
   while (rn > target && total < budget && outer < 50) {
-    hipLaunchKernelGGL(k_cast_df, dim3(nb), dim3(256), 0, s, n, (const double*)op->rtrue, f->r32);
-    int its = 0;
-    double est = 0.0;
-    const double tol_rel = std::max(inner, 0.5 * target / rn);
-    rc = fgmres_cycle(op, nn, f, f->r32, x, std::min(m, budget - total), tol_rel, sweeps, omega, s, &its, &est);
+    int k = 0;
+    // the estimate is the FP64 least-squares residual; 0.7 leaves room for
+    // its drift from the recomputed true residual
+    rc = fgmres_cycle(op, op->nn, f, x, std::min(m, budget - total), rn, 0.7 * target, co, s, &k);
     if (rc) return rc;
-    total += its;
+    total += k;
     ++outer;
     const double rn_old = rn;
     rn = true_res(op, b, x, s, &rcode);
     if (rcode) return rcode;
-    if (!(rn < rn_old)) break;   // no progress
+    if (!(rn < rn_old)) {   // no progress: undo this cycle's correction and stop
+      hipLaunchKernelGGL(kf_update, dim3(nb), dim3(256), 0, s, n, k, (const double*)f->y, -1.0, (const float*)f->Z,
+                         x);
+      rn = true_res(op, b, x, s, &rcode);
+      if (rcode) return rcode;
+      break;
+    }
   }
   info->outer = outer;
   info->iters = total;

@@ -1,0 +1,822 @@
+// Two-branch multigrid cycle of the 2D nodal operator (gfx950); the design is
+// described in pb_mg2d.hpp.
+//
+// Level 0 runs on the Krylov layout with the operator's own FP32 flux-form
+// weights (the LDS-tiled stencil of pb_op2d.hpp); levels >= 1 are FP32
+// 9-point stencils on unique grids, one thread per node carrying both
+// branches.  The Galerkin stencils are formed in FP64 on the device whenever
+// the operator is re-assembled.
+#include <algorithm>
+
+#include "pb_mg2d.hpp"
+
+using namespace pb2d;
+
+int pb_nn_forward_batch(pb_nn* nn, int nimg, int h, int w, const float* r, const float* f1, const float* f2,
+                        float* z, hipStream_t s);
+
+namespace pbmg {
+namespace {
+
+// index of node i + d along an axis of n nodes: wrapped (periodic) or
+// clamped into range with ok = false (wall; the stencil entry is zero there)
+__device__ __forceinline__ int axis_idx(int i, int n, int periodic, bool& ok) {
+  if (periodic) {
+    ok = true;
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+  }
+  ok = i >= 0 && i < n;
+  return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+// the 3x3 neighbourhood of (j, i), entry k = 3 (dj + 1) + (di + 1)
+__device__ __forceinline__ void nbr9(const Dims& d, int j, int i, int (&idx)[9]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    bool oy;
+    const int jj = axis_idx(j + a - 1, d.ny, d.py, oy);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      bool ox;
+      const int ii = axis_idx(i + b - 1, d.nx, d.px, ox);
+      idx[3 * a + b] = jj * d.nx + ii;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// set-up
+// ---------------------------------------------------------------------------
+
+// Level-0 stencil of D^-1 A on the unique grid from the folded flux weights
+// (pb_op2d.hpp): per op-cell gx = (tr - tl) + (br - bl), gy = (bl - tl) +
+// (br - tr), Fx = Wxx gx + Wyx gy, Fy = Wyy gy + Wxy gx; a node sums its four
+// cells' sx Fx + sy Fy (sx = -1 left cells, sy = -1 upper cells), adds hc p
+// and is scaled by dinv: lap2D_gather_new (laplacian.py:546-746) term by term.
+__global__ void k_expand9(int nxd, Dims u, const double* wxx, const double* wyy, const double* wyx,
+                          const double* wxy, const double* hc, const double* dinv, int cross, int center,
+                          double* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= u.nx || j >= u.ny) return;
+  double c[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) c[k] = 0.0;
+  const int ncx = nxd + 1;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int dcj = q >> 1, dci = q & 1;
+    const double sx = dci ? 1.0 : -1.0, sy = dcj ? 1.0 : -1.0;
+    const long cc = (long)(j + dcj) * ncx + (i + dci);
+    const double a = wxx[cc], b = wyy[cc];
+    const double cyx = cross ? wyx[cc] : 0.0, cxy = cross ? wxy[cc] : 0.0;
+#pragma unroll
+    for (int ca = 0; ca < 2; ++ca) {
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        const double gxc = cb ? 1.0 : -1.0, gyc = ca ? 1.0 : -1.0;
+        c[3 * (dcj + ca) + (dci + cb)] += sx * (a * gxc + cyx * gyc) + sy * (b * gyc + cxy * gxc);
+      }
+    }
+  }
+  const long g = (long)j * nxd + i;
+  if (center) c[4] += hc[g];
+  const double di = dinv[g];
+  const long plane = (long)u.ny * u.nx, o = (long)j * u.nx + i;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) st[k * plane + o] = c[k] * di;
+}
+
+// 1 / diagonal of level 0 in the Krylov layout (duplicated nodes repeat it)
+__global__ void k_id0(int nxd, int nyd, Dims u, const double* st, float* id0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nxd || j >= nyd) return;
+  const int ju = j % u.ny, iu = i % u.nx;
+  id0[(long)j * nxd + i] = (float)(1.0 / st[4L * u.ny * u.nx + (long)ju * u.nx + iu]);
+}
+
+// coarse parents of the fine node 2 J + q (q in [-2, 2]) as offsets from J
+// with their interpolation weights; returns the count (1 or 2)
+__device__ __forceinline__ int parents(int q, int (&o)[2], double (&w)[2]) {
+  if ((q & 1) == 0) {
+    o[0] = q / 2;
+    w[0] = 1.0;
+    return 1;
+  }
+  o[0] = (q - 1) / 2;
+  o[1] = (q + 1) / 2;
+  w[0] = w[1] = 0.5;
+  return 2;
+}
+
+// Galerkin product R A P at coarse node (J, I): R = P^T / 4 gathers the fine
+// nodes 2J + a, 2I + b (a, b in -1..1, weights 1 and 1/2 per axis), each
+// fine stencil entry reaches 2J + a + dj, 2I + b + di, and P sends that node
+// to its coarse parents.  Also writes the FP32 copy and 1 / centre.
+__global__ void k_rap(Dims f, Dims c, const double* stf, double* stc, float* stc32, float* idg) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y;
+  if (I >= c.nx || J >= c.ny) return;
+  double acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  const long pf = (long)f.ny * f.nx;
+  for (int a = -1; a <= 1; ++a) {
+    bool oky;
+    const int jf = axis_idx(2 * J + a, f.ny, f.py, oky);
+    if (!oky) continue;
+    for (int b = -1; b <= 1; ++b) {
+      bool okx;
+      const int iF = axis_idx(2 * I + b, f.nx, f.px, okx);
+      if (!okx) continue;
+      const double wr = 0.25 * (a == 0 ? 1.0 : 0.5) * (b == 0 ? 1.0 : 0.5);
+      const long gf = (long)jf * f.nx + iF;
+      for (int k = 0; k < 9; ++k) {
+        const double sv = stf[k * pf + gf];
+        if (sv == 0.0) continue;
+        const int dj = k / 3 - 1, di = k % 3 - 1;
+        int oy[2], ox[2];
+        double wy[2], wx[2];
+        const int ny_ = parents(a + dj, oy, wy), nx_ = parents(b + di, ox, wx);
+        for (int p = 0; p < ny_; ++p)
+          for (int q = 0; q < nx_; ++q) acc[3 * (oy[p] + 1) + (ox[q] + 1)] += wr * sv * wy[p] * wx[q];
+      }
+    }
+  }
+  const long pc = (long)c.ny * c.nx, o = (long)J * c.nx + I;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    stc[k * pc + o] = acc[k];
+    stc32[k * pc + o] = (float)acc[k];
+  }
+  idg[o] = (float)(1.0 / acc[4]);
+}
+
+// network features on the duplicated grid: the stencil's x and y coupling
+// relative to its centre, -(W + E) / C and -(N + S) / C
+__global__ void k_net_features(Dims u, int nh, int nw, const double* st, float* f1, float* f2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nw || j >= nh) return;
+  const long p = (long)u.ny * u.nx, o = (long)(j % u.ny) * u.nx + (i % u.nx);
+  const double cc = st[4 * p + o];
+  f1[(long)j * nw + i] = (float)(-(st[3 * p + o] + st[5 * p + o]) / cc);
+  f2[(long)j * nw + i] = (float)(-(st[1 * p + o] + st[7 * p + o]) / cc);
+}
+
+// ---------------------------------------------------------------------------
+// level 0: LDS-tiled stencil launches on the Krylov layout
+// ---------------------------------------------------------------------------
+struct Nothing {};
+
+// first two sweeps from z = 0: z1 = omega v / d formed for the halo, z2 =
+// z1 + omega (v - A z1) / d; also stores the FP32 copy of v
+struct FirstIn {
+  struct Raw {
+    double v;
+    float id;
+  };
+  const double* v;
+  const float* id;
+  float omega;
+  __device__ Raw load(long g) const { return Raw{v[g], id[g]}; }
+  __device__ float combine(const Raw& q) const { return omega * (float)q.v * q.id; }
+};
+struct FirstOut {
+  using Pre = Nothing;
+  float *v0, *z;
+  float omega;
+  __device__ Pre prefetch(long) const { return Pre{}; }
+  __device__ void operator()(int, long g, float y, float pc, const Pre&, const FirstIn::Raw& q) {
+    const float vf = (float)q.v;
+    v0[g] = vf;
+    z[g] = pc + omega * (vf - y) * q.id;
+  }
+};
+
+struct VecIn {
+  using Raw = float;
+  const float* x;
+  __device__ Raw load(long g) const { return x[g]; }
+  __device__ float combine(const Raw& q) const { return q; }
+};
+// one sweep: out = z + omega (v - A z) / d
+struct SweepOut {
+  struct Pre {
+    float v, id;
+  };
+  const float *v0, *id0;
+  float* out;
+  float omega;
+  __device__ Pre prefetch(long g) const { return Pre{v0[g], id0[g]}; }
+  template <class R>
+  __device__ void operator()(int, long g, float y, float pc, const Pre& e, const R&) {
+    out[g] = pc + omega * (e.v - y) * e.id;
+  }
+};
+struct ResidOut {
+  using Pre = float;
+  const float* v0;
+  float* r;
+  __device__ Pre prefetch(long g) const { return v0[g]; }
+  template <class R>
+  __device__ void operator()(int, long g, float y, float, const Pre& vg, const R&) {
+    r[g] = vg - y;
+  }
+};
+// z + P e_s + c P e_d formed for every node of the halo (the coarse level-1
+// corrections of the two branches gathered at the node's parents)
+struct ProlongIn {
+  using Raw = float;
+  const float* z;
+  const float* e;   // [2][c.ny][c.nx]
+  Dims u, c;        // level 0 unique grid, level 1
+  int nxd;
+  float dgain;      // 1: add the checkerboard branch, 0: drop it
+  __device__ Raw load(long g) const {
+    const int j = (int)(g / nxd), i = (int)(g - (long)j * nxd);
+    const int ju = j % u.ny, iu = i % u.nx;
+    const int J0 = ju >> 1, I0 = iu >> 1;
+    const int J1 = (ju & 1) ? (J0 + 1 == c.ny && c.py ? 0 : J0 + 1) : J0;
+    const int I1 = (iu & 1) ? (I0 + 1 == c.nx && c.px ? 0 : I0 + 1) : I0;
+    const long pc = (long)c.ny * c.nx;
+    const long a00 = (long)J0 * c.nx + I0, a01 = (long)J0 * c.nx + I1, a10 = (long)J1 * c.nx + I0,
+               a11 = (long)J1 * c.nx + I1;
+    const float es = 0.25f * ((e[a00] + e[a01]) + (e[a10] + e[a11]));
+    const float ed = 0.25f * ((e[pc + a00] + e[pc + a01]) + (e[pc + a10] + e[pc + a11]));
+    const float sgn = ((ju + iu) & 1) ? -dgain : dgain;
+    return z[g] + es + sgn * ed;
+  }
+  __device__ float combine(const Raw& q) const { return q; }
+};
+
+template <bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k0_first(OpDev<float> op, FirstIn in, FirstOut out) {
+  __shared__ TileLds<float, RY> L;
+  stencil_tile2<float, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, [] { return true; });
+}
+template <bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k0_sweep(OpDev<float> op, VecIn in, SweepOut out) {
+  __shared__ TileLds<float, RY> L;
+  stencil_tile2<float, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, [] { return true; });
+}
+template <bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k0_resid(OpDev<float> op, VecIn in, ResidOut out) {
+  __shared__ TileLds<float, RY> L;
+  stencil_tile2<float, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, [] { return true; });
+}
+template <bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k0_prolong_sweep(OpDev<float> op, ProlongIn in, SweepOut out) {
+  __shared__ TileLds<float, RY> L;
+  stencil_tile2<float, CROSS, CENTER, RY>(op, blockIdx.x, blockIdx.y, in, out, L, [] { return true; });
+}
+
+// z = omega v / d, v0 = v (a single pre-sweep)
+__global__ void k0_init(long n, const double* v, const float* id, float omega, float* v0, float* z) {
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+    const float vf = (float)v[g];
+    v0[g] = vf;
+    z[g] = omega * vf * id[g];
+  }
+}
+
+// full weighting of r (level 0, Krylov layout) and of c * r onto level 1
+__global__ void k0_restrict2(Dims u, int nxd, Dims c, const float* r, float dgain, float* vc) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y;
+  if (I >= c.nx || J >= c.ny) return;
+  float s = 0.f, d = 0.f;
+#pragma unroll
+  for (int a = -1; a <= 1; ++a) {
+    bool oky;
+    const int jf = axis_idx(2 * J + a, u.ny, u.py, oky);
+#pragma unroll
+    for (int b = -1; b <= 1; ++b) {
+      bool okx;
+      const int iF = axis_idx(2 * I + b, u.nx, u.px, okx);
+      const float w = (oky && okx) ? 0.25f * (a == 0 ? 1.f : 0.5f) * (b == 0 ? 1.f : 0.5f) : 0.f;
+      const float rv = w * r[(long)jf * nxd + iF];
+      s += rv;
+      d += ((jf + iF) & 1) ? -rv : rv;
+    }
+  }
+  const long o = (long)J * c.nx + I, pc = (long)c.ny * c.nx;
+  vc[o] = s;
+  vc[pc + o] = dgain * d;
+}
+
+// ---------------------------------------------------------------------------
+// levels >= 1: one thread per node, both branches
+// ---------------------------------------------------------------------------
+struct Nb {
+  int idx[9];
+  float s[9];
+};
+
+__device__ __forceinline__ void load_row(const Dims& d, const float* st, long g, Nb& nb) {
+  const int j = (int)(g / d.nx), i = (int)(g - (long)j * d.nx);
+  nbr9(d, j, i, nb.idx);
+  const long p = (long)d.ny * d.nx;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) nb.s[k] = st[k * p + g];
+}
+
+// z = omega v / d, then one sweep (the first two pre-sweeps)
+__global__ void kc_first(Dims d, const float* st, const float* idg, const float* v, float omega, float* z) {
+  const long n = (long)d.ny * d.nx;
+  const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  Nb nb;
+  load_row(d, st, g, nb);
+  float ig[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) ig[k] = idg[nb.idx[k]];
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    const float* vc = v + ch * n;
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc += nb.s[k] * (omega * vc[nb.idx[k]] * ig[k]);
+    const float vg = vc[g];
+    z[ch * n + g] = omega * vg * ig[4] + omega * (vg - acc) * ig[4];
+  }
+}
+
+// z = omega v / d (a single sweep from zero)
+__global__ void kc_init(long n, const float* idg, const float* v, float omega, float* z) {
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+    z[g] = omega * v[g] * idg[g];
+    z[n + g] = omega * v[n + g] * idg[g];
+  }
+}
+
+// out = z + omega (v - A z) / d
+__global__ void kc_sweep(Dims d, const float* st, const float* idg, const float* v, const float* z, float omega,
+                         float* out) {
+  const long n = (long)d.ny * d.nx;
+  const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  Nb nb;
+  load_row(d, st, g, nb);
+  const float ig = idg[g];
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    const float* zc = z + ch * n;
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc += nb.s[k] * zc[nb.idx[k]];
+    out[ch * n + g] = zc[g] + omega * (v[ch * n + g] - acc) * ig;
+  }
+}
+
+// r = v - A z
+__global__ void kc_resid(Dims d, const float* st, const float* v, const float* z, float* r) {
+  const long n = (long)d.ny * d.nx;
+  const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  Nb nb;
+  load_row(d, st, g, nb);
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    const float* zc = z + ch * n;
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc += nb.s[k] * zc[nb.idx[k]];
+    r[ch * n + g] = v[ch * n + g] - acc;
+  }
+}
+
+// full weighting onto the next level, both branches
+__global__ void kc_restrict(Dims f, Dims c, const float* r, float* vc) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y;
+  if (I >= c.nx || J >= c.ny) return;
+  const long pf = (long)f.ny * f.nx;
+  float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+  for (int a = -1; a <= 1; ++a) {
+    bool oky;
+    const int jf = axis_idx(2 * J + a, f.ny, f.py, oky);
+#pragma unroll
+    for (int b = -1; b <= 1; ++b) {
+      bool okx;
+      const int iF = axis_idx(2 * I + b, f.nx, f.px, okx);
+      const float w = (oky && okx) ? 0.25f * (a == 0 ? 1.f : 0.5f) * (b == 0 ? 1.f : 0.5f) : 0.f;
+      const long gf = (long)jf * f.nx + iF;
+      s0 += w * r[gf];
+      s1 += w * r[pf + gf];
+    }
+  }
+  const long o = (long)J * c.nx + I, pc = (long)c.ny * c.nx;
+  vc[o] = s0;
+  vc[pc + o] = s1;
+}
+
+// bilinear value of the coarse field e at fine node (j, i)
+__device__ __forceinline__ float interp(const Dims& c, const float* e, int j, int i) {
+  const int J0 = j >> 1, I0 = i >> 1;
+  const int J1 = (j & 1) ? (J0 + 1 == c.ny && c.py ? 0 : J0 + 1) : J0;
+  const int I1 = (i & 1) ? (I0 + 1 == c.nx && c.px ? 0 : I0 + 1) : I0;
+  return 0.25f * ((e[(long)J0 * c.nx + I0] + e[(long)J0 * c.nx + I1]) +
+                  (e[(long)J1 * c.nx + I0] + e[(long)J1 * c.nx + I1]));
+}
+
+// z' = z + P e, then out = z' + omega (v - A z') / d
+__global__ void kc_prolong_sweep(Dims d, Dims c, const float* st, const float* idg, const float* v, const float* z,
+                                 const float* e, float omega, float* out) {
+  const long n = (long)d.ny * d.nx, pc = (long)c.ny * c.nx;
+  const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  Nb nb;
+  load_row(d, st, g, nb);
+  const float ig = idg[g];
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    const float* zc = z + ch * n;
+    const float* ec = e + ch * pc;
+    float acc = 0.f, zg = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int q = nb.idx[k];
+      const int jq = q / d.nx, iq = q - jq * d.nx;
+      const float zq = zc[q] + interp(c, ec, jq, iq);
+      acc += nb.s[k] * zq;
+      if (k == 4) zg = zq;
+    }
+    out[ch * n + g] = zg + omega * (v[ch * n + g] - acc) * ig;
+  }
+}
+
+// z *= alpha per branch, alpha = (A z . v) / (A z . A z) minimising
+// ||v - alpha A z||: the network predicts the shape of the coarse correction,
+// its scale is fitted here.  One block of 1024 threads (the network's level
+// has at most 129 x 65 nodes).
+constexpr int kAlphaThreads = 1024;
+__global__ __launch_bounds__(kAlphaThreads) void kc_net_alpha(Dims d, const float* st, const float* v, float* z) {
+  __shared__ double sh[4 * (kAlphaThreads / kWave)];
+  __shared__ float alpha[2];
+  const long n = (long)d.ny * d.nx;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};   // (Az.v, Az.Az) per branch
+  for (long g = threadIdx.x; g < n; g += kAlphaThreads) {
+    Nb nb;
+    load_row(d, st, g, nb);
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+      const float* zc = z + ch * n;
+      float az = 0.f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) az += nb.s[k] * zc[nb.idx[k]];
+      acc[2 * ch] += (double)az * (double)v[ch * n + g];
+      acc[2 * ch + 1] += (double)az * (double)az;
+    }
+  }
+  block_sum_w<4, kAlphaThreads / kWave>(acc, sh);
+  if (threadIdx.x == 0) {
+    alpha[0] = acc[1] > 0.0 ? (float)(acc[0] / acc[1]) : 0.f;
+    alpha[1] = acc[3] > 0.0 ? (float)(acc[2] / acc[3]) : 0.f;
+  }
+  __syncthreads();
+  const float a0 = alpha[0], a1 = alpha[1];
+  for (long g = threadIdx.x; g < n; g += kAlphaThreads) {
+    z[g] *= a0;
+    z[n + g] *= a1;
+  }
+}
+
+// the network level's right-hand sides onto the duplicated grid, and back
+__global__ void k_to_net(Dims u, int nh, int nw, const float* v, float* nr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nw || j >= nh) return;
+  const long pu = (long)u.ny * u.nx, pd = (long)nh * nw;
+  const long s = (long)(j % u.ny) * u.nx + (i % u.nx), o = (long)j * nw + i;
+  nr[o] = v[s];
+  nr[pd + o] = v[pu + s];
+}
+__global__ void k_from_net(Dims u, int nh, int nw, const float* nz, float* z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= u.nx || j >= u.ny) return;
+  const long pu = (long)u.ny * u.nx, pd = (long)nh * nw;
+  // a duplicated node takes the mean of its copies
+  const bool dx = u.px && i == 0, dy = u.py && j == 0;
+  for (int ch = 0; ch < 2; ++ch) {
+    const float* q = nz + ch * pd;
+    float a = q[(long)j * nw + i];
+    float cnt = 1.f;
+    if (dx) {
+      a += q[(long)j * nw + (nw - 1)];
+      cnt += 1.f;
+    }
+    if (dy) {
+      a += q[(long)(nh - 1) * nw + i];
+      cnt += 1.f;
+    }
+    if (dx && dy) {
+      a += q[(long)(nh - 1) * nw + (nw - 1)];
+      cnt += 1.f;
+    }
+    z[ch * pu + (long)j * u.nx + i] = a / cnt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+inline int blocks1(long n) { return (int)std::max<long>(1, (n + 255) / 256); }
+
+template <typename T>
+int hmalloc(pb_op2d* op, Hier* H, T** p, long count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)std::max<long>(count, 1));
+  if (e != hipSuccess)
+    return pb_fail(op->ctx, PB_ERR_HIP, "multigrid: hipMalloc(%ld bytes) failed: %s", (long)(sizeof(T) * count),
+                   hipGetErrorString(e));
+  H->allocs.push_back(q);
+  *p = (T*)q;
+  return PB_OK;
+}
+
+bool can_coarsen(int n, int periodic) { return periodic ? (n % 2 == 0 && n / 2 >= 3) : (n % 2 == 1 && (n + 1) / 2 >= 3); }
+int coarse_n(int n, int periodic) { return periodic ? n / 2 : (n + 1) / 2; }
+
+int layout(pb_op2d* op, Hier* H) {
+  const pb_grid2d& g = op->g;
+  Dims d0{g.ny, g.nx, g.bc_y == PB_BC_PERIODIC, g.bc_x == PB_BC_PERIODIC};
+  if (d0.py) d0.ny -= 1;
+  if (d0.px) d0.nx -= 1;
+  H->L[0].d = d0;
+  H->nlev = 1;
+  while (H->nlev < kMaxLevels) {
+    const Dims f = H->L[H->nlev - 1].d;
+    if (!can_coarsen(f.ny, f.py) || !can_coarsen(f.nx, f.px)) break;
+    H->L[H->nlev].d = Dims{coarse_n(f.ny, f.py), coarse_n(f.nx, f.px), f.py, f.px};
+    H->nlev += 1;
+  }
+  if (H->nlev < 2)
+    return pb_fail(op->ctx, PB_ERR_ARG, "multigrid: a %d x %d grid has no coarse level", g.nx, g.ny);
+  // the network level: the first coarse level whose duplicated grid is 8k+1
+  // nodes per axis (>= 9) and at most 129 x 65 nodes
+  H->nn_level = -1;
+  for (int l = 1; l < H->nlev; ++l) {
+    const Dims d = H->L[l].d;
+    const int h = d.ny + (d.py ? 1 : 0), w = d.nx + (d.px ? 1 : 0);
+    if (h >= 9 && w >= 9 && ((h - 1) & 7) == 0 && ((w - 1) & 7) == 0 && (long)h * w <= 129L * 65) {
+      H->nn_level = l;
+      H->nh = h;
+      H->nw = w;
+      break;
+    }
+  }
+  int rc = 0;
+  for (int l = 0; l < H->nlev; ++l) {
+    Level& L = H->L[l];
+    const long n = (long)L.d.ny * L.d.nx;
+    rc |= hmalloc(op, H, &L.st64, 9 * n);
+    if (l >= 1) {
+      rc |= hmalloc(op, H, &L.st, 9 * n);
+      rc |= hmalloc(op, H, &L.idg, n);
+      rc |= hmalloc(op, H, &L.v, 2 * n);
+      rc |= hmalloc(op, H, &L.z, 2 * n);
+      rc |= hmalloc(op, H, &L.t, 2 * n);
+      rc |= hmalloc(op, H, &L.r, 2 * n);
+    }
+  }
+  rc |= hmalloc(op, H, &H->id0, op->n);
+  rc |= hmalloc(op, H, &H->v0, op->n);
+  rc |= hmalloc(op, H, &H->z0, op->n);
+  rc |= hmalloc(op, H, &H->t0, op->n);
+  rc |= hmalloc(op, H, &H->r0, op->n);
+  if (H->nn_level >= 0) {
+    const long nd = (long)H->nh * H->nw;
+    rc |= hmalloc(op, H, &H->nr, 2 * nd);
+    rc |= hmalloc(op, H, &H->nz, 2 * nd);
+    rc |= hmalloc(op, H, &H->f1, nd);
+    rc |= hmalloc(op, H, &H->f2, nd);
+  }
+  return rc ? PB_ERR_HIP : PB_OK;
+}
+
+#define PB_MG_LAUNCH0(KERNEL, RYV, grd, s, ...)                                                        \
+  do {                                                                                                 \
+    const dim3 _blk(TX, TYT);                                                                          \
+    if (op->cross && op->center) hipLaunchKernelGGL((KERNEL<true, true, RYV>), grd, _blk, 0, s, __VA_ARGS__);   \
+    else if (op->cross) hipLaunchKernelGGL((KERNEL<true, false, RYV>), grd, _blk, 0, s, __VA_ARGS__);         \
+    else if (op->center) hipLaunchKernelGGL((KERNEL<false, true, RYV>), grd, _blk, 0, s, __VA_ARGS__);        \
+    else hipLaunchKernelGGL((KERNEL<false, false, RYV>), grd, _blk, 0, s, __VA_ARGS__);                       \
+  } while (0)
+
+constexpr int kRY0 = 4;   // level-0 tile height (rows per thread)
+
+// returns the buffer holding level l's correction (2 channels)
+const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const CycleOpts& o, hipStream_t s, int* rc) {
+  Level& L = H->L[l];
+  const Dims d = L.d;
+  const long n = (long)d.ny * d.nx;
+  const int nb = blocks1(n);
+  float* cur = L.z;
+  float* alt = L.t;
+  auto swap = [&] { std::swap(cur, alt); };
+  if (l == last) {
+    if (o.coarse == COARSE_NET && l == H->nn_level && nn) {
+      const dim3 gn(ceil_div(H->nw, 64), H->nh), gu(ceil_div(d.nx, 64), d.ny);
+      hipLaunchKernelGGL(k_to_net, gn, dim3(64), 0, s, d, H->nh, H->nw, (const float*)L.v, H->nr);
+      *rc = pb_nn_forward_batch(nn, 2, H->nh, H->nw, H->nr, H->f1, H->f2, H->nz, s);
+      if (*rc) return nullptr;
+      hipLaunchKernelGGL(k_from_net, gu, dim3(64), 0, s, d, H->nh, H->nw, (const float*)H->nz, cur);
+      hipLaunchKernelGGL(kc_net_alpha, dim3(1), dim3(kAlphaThreads), 0, s, d, (const float*)L.st, (const float*)L.v,
+                         cur);
+      for (int k = 0; k < o.coarse_sweeps; ++k) {
+        hipLaunchKernelGGL(kc_sweep, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.idg,
+                           (const float*)L.v, (const float*)cur, o.omega, alt);
+        swap();
+      }
+      return cur;
+    }
+    const int sweeps = std::max(1, o.coarse == COARSE_MG ? o.bottom_sweeps : o.coarse_sweeps);
+    hipLaunchKernelGGL(kc_init, dim3(std::min(nb, 1024)), dim3(256), 0, s, n, (const float*)L.idg,
+                       (const float*)L.v, o.omega, cur);
+    for (int k = 1; k < sweeps; ++k) {
+      hipLaunchKernelGGL(kc_sweep, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.idg,
+                         (const float*)L.v, (const float*)cur, o.omega, alt);
+      swap();
+    }
+    return cur;
+  }
+  // pre-smoothing
+  if (o.nu_pre >= 2) {
+    hipLaunchKernelGGL(kc_first, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.idg,
+                       (const float*)L.v, o.omega, cur);
+    for (int k = 2; k < o.nu_pre; ++k) {
+      hipLaunchKernelGGL(kc_sweep, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.idg,
+                         (const float*)L.v, (const float*)cur, o.omega, alt);
+      swap();
+    }
+  } else {
+    hipLaunchKernelGGL(kc_init, dim3(std::min(nb, 1024)), dim3(256), 0, s, n, (const float*)L.idg,
+                       (const float*)L.v, o.omega, cur);
+  }
+  hipLaunchKernelGGL(kc_resid, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.v,
+                     (const float*)cur, L.r);
+  Level& C = H->L[l + 1];
+  hipLaunchKernelGGL(kc_restrict, dim3(ceil_div(C.d.nx, 64), C.d.ny), dim3(64), 0, s, d, C.d, (const float*)L.r,
+                     C.v);
+  const float* e = vcycle(op, H, nn, l + 1, last, o, s, rc);
+  if (!e) return nullptr;
+  // coarse correction and post-smoothing (at least one sweep)
+  hipLaunchKernelGGL(kc_prolong_sweep, dim3(nb), dim3(256), 0, s, d, C.d, (const float*)L.st, (const float*)L.idg,
+                     (const float*)L.v, (const float*)cur, e, o.omega, alt);
+  swap();
+  for (int k = 1; k < o.nu_post; ++k) {
+    hipLaunchKernelGGL(kc_sweep, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.idg,
+                       (const float*)L.v, (const float*)cur, o.omega, alt);
+    swap();
+  }
+  return cur;
+}
+
+}  // namespace
+
+int build(pb_op2d* op, hipStream_t s) {
+  if (!op->mg) {
+    op->mg = new Hier();
+    int rc = layout(op, op->mg);
+    if (rc) {
+      release(op);
+      return rc;
+    }
+  }
+  Hier* H = op->mg;
+  if (H->version == op->version) return PB_OK;
+  const Dims u = H->L[0].d;
+  hipLaunchKernelGGL(k_expand9, dim3(ceil_div(u.nx, 64), u.ny), dim3(64), 0, s, op->g.nx, u, (const double*)op->wxx,
+                     (const double*)op->wyy, (const double*)op->wyx, (const double*)op->wxy, (const double*)op->hc,
+                     (const double*)op->dinv, op->cross ? 1 : 0, op->center ? 1 : 0, H->L[0].st64);
+  hipLaunchKernelGGL(k_id0, dim3(ceil_div(op->g.nx, 64), op->g.ny), dim3(64), 0, s, op->g.nx, op->g.ny, u,
+                     (const double*)H->L[0].st64, H->id0);
+  for (int l = 0; l + 1 < H->nlev; ++l) {
+    const Dims c = H->L[l + 1].d;
+    hipLaunchKernelGGL(k_rap, dim3(ceil_div(c.nx, 64), c.ny), dim3(64), 0, s, H->L[l].d, c,
+                       (const double*)H->L[l].st64, H->L[l + 1].st64, H->L[l + 1].st, H->L[l + 1].idg);
+  }
+  if (H->nn_level >= 0) {
+    const Level& N = H->L[H->nn_level];
+    hipLaunchKernelGGL(k_net_features, dim3(ceil_div(H->nw, 64), H->nh), dim3(64), 0, s, N.d, H->nh, H->nw,
+                       (const double*)N.st64, H->f1, H->f2);
+  }
+  PB_LAUNCH_CHECK(op->ctx);
+  H->version = op->version;
+  return PB_OK;
+}
+
+int cycle(pb_op2d* op, pb_nn* nn, const double* v, float* z, const CycleOpts& o, hipStream_t s) {
+  Hier* H = op->mg;
+  if (!H || H->version != op->version) return pb_fail(op->ctx, PB_ERR_STATE, "multigrid: hierarchy not built");
+  if (o.coarse == COARSE_NET && (!nn || H->nn_level < 0))
+    return pb_fail(op->ctx, PB_ERR_STATE, nn ? "multigrid: no coarse level of this grid fits the network"
+                                             : "multigrid: no network attached");
+  if (o.nu_pre < 1 || o.nu_post < 1) return pb_fail(op->ctx, PB_ERR_ARG, "multigrid: at least one sweep each way");
+  const int last = (o.coarse == COARSE_MG || H->nn_level < 0) ? H->nlev - 1 : H->nn_level;
+  const OpDev<float> d = opdev<float>(op);
+  const dim3 grd = stencil_grid(op, kRY0);
+  const Dims u = H->L[0].d, c1 = H->L[1].d;
+  const int nxd = op->g.nx;
+  const float dgain = o.branches == 2 ? 1.f : 0.f;
+  float* cur = H->z0;
+  float* alt = H->t0;
+  if (o.nu_pre >= 2) {
+    FirstIn in{v, H->id0, o.omega};
+    FirstOut out{H->v0, cur, o.omega};
+    PB_MG_LAUNCH0(k0_first, kRY0, grd, s, d, in, out);
+    for (int k = 2; k < o.nu_pre; ++k) {
+      PB_MG_LAUNCH0(k0_sweep, kRY0, grd, s, d, VecIn{cur}, SweepOut{H->v0, H->id0, alt, o.omega});
+      std::swap(cur, alt);
+    }
+  } else {
+    hipLaunchKernelGGL(k0_init, dim3(elem_blocks(op->n)), dim3(256), 0, s, op->n, v, (const float*)H->id0, o.omega,
+                       H->v0, cur);
+  }
+  PB_MG_LAUNCH0(k0_resid, kRY0, grd, s, d, VecIn{cur}, ResidOut{H->v0, H->r0});
+  hipLaunchKernelGGL(k0_restrict2, dim3(ceil_div(c1.nx, 64), c1.ny), dim3(64), 0, s, u, nxd, c1,
+                     (const float*)H->r0, dgain, H->L[1].v);
+  int rc = PB_OK;
+  const float* e = vcycle(op, H, nn, 1, last, o, s, &rc);
+  if (!e) return rc ? rc : PB_ERR_STATE;
+  ProlongIn pin{cur, e, u, c1, nxd, dgain};
+  float* dst = o.nu_post == 1 ? z : alt;
+  PB_MG_LAUNCH0(k0_prolong_sweep, kRY0, grd, s, d, pin, SweepOut{H->v0, H->id0, dst, o.omega});
+  cur = dst;
+  alt = H->z0 == cur ? H->t0 : H->z0;
+  for (int k = 1; k < o.nu_post; ++k) {
+    float* out = k + 1 == o.nu_post ? z : alt;
+    PB_MG_LAUNCH0(k0_sweep, kRY0, grd, s, d, VecIn{cur}, SweepOut{H->v0, H->id0, out, o.omega});
+    alt = cur;
+    cur = out;
+  }
+  PB_LAUNCH_CHECK(op->ctx);
+  return PB_OK;
+}
+
+void release(pb_op2d* op) {
+  if (!op->mg) return;
+  for (void* p : op->mg->allocs) (void)hipFree(p);
+  delete op->mg;
+  op->mg = nullptr;
+}
+
+}  // namespace pbmg
+
+// ---------------------------------------------------------------------------
+// C ABI: the cycle and its hierarchy, for tests and network training
+// ---------------------------------------------------------------------------
+extern "C" int pb_op2d_precond(pb_op2d* op, const double* v, float* z, int coarse, int nu, double omega,
+                               int branches) {
+  if (!op) return PB_ERR_ARG;
+  if (!v || !z) return pb_fail(op->ctx, PB_ERR_ARG, "pb_op2d_precond: null vector");
+  if (!op->assembled) return pb_fail(op->ctx, PB_ERR_STATE, "pb_op2d_precond: operator not assembled");
+  if (coarse < 0 || coarse > 2 || nu < 1 || nu > 8 || !(omega > 0) || (branches != 1 && branches != 2))
+    return pb_fail(op->ctx, PB_ERR_ARG, "pb_op2d_precond: bad option");
+  hipStream_t s = op->ctx->stream;
+  int rc = pbmg::build(op, s);
+  if (rc) return rc;
+  pbmg::CycleOpts o;
+  o.nu_pre = o.nu_post = nu;
+  o.omega = (float)omega;
+  o.coarse = coarse;
+  o.branches = branches;
+  return pbmg::cycle(op, op->nn, v, z, o, s);
+}
+
+extern "C" int pb_op2d_mg_info(pb_op2d* op, int* dims, int max_levels, int* nn_level) {
+  if (!op || !dims || max_levels < 1) return PB_ERR_ARG;
+  if (!op->assembled) return pb_fail(op->ctx, PB_ERR_STATE, "pb_op2d_mg_info: operator not assembled");
+  int rc = pbmg::build(op, op->ctx->stream);
+  if (rc) return rc;
+  const pbmg::Hier* H = op->mg;
+  for (int l = 0; l < H->nlev && l < max_levels; ++l) {
+    const pbmg::Dims& d = H->L[l].d;
+    dims[4 * l] = d.ny;
+    dims[4 * l + 1] = d.nx;
+    dims[4 * l + 2] = d.py;
+    dims[4 * l + 3] = d.px;
+  }
+  if (nn_level) *nn_level = H->nn_level;
+  return H->nlev;
+}
+
+extern "C" int pb_op2d_mg_stencil(pb_op2d* op, int level, double* st) {
+  if (!op || !st) return PB_ERR_ARG;
+  int rc = pbmg::build(op, op->ctx->stream);
+  if (rc) return rc;
+  const pbmg::Hier* H = op->mg;
+  if (level < 0 || level >= H->nlev) return pb_fail(op->ctx, PB_ERR_ARG, "pb_op2d_mg_stencil: no level %d", level);
+  const pbmg::Dims& d = H->L[level].d;
+  PB_HIP(op->ctx, hipMemcpyAsync(st, H->L[level].st64, sizeof(double) * 9 * (size_t)d.ny * d.nx,
+                                 hipMemcpyDeviceToDevice, op->ctx->stream));
+  return PB_OK;
+}
+
+extern "C" int pb_op2d_mg_features(pb_op2d* op, float* f1, float* f2) {
+  if (!op || !f1 || !f2) return PB_ERR_ARG;
+  int rc = pbmg::build(op, op->ctx->stream);
+  if (rc) return rc;
+  const pbmg::Hier* H = op->mg;
+  if (H->nn_level < 0) return pb_fail(op->ctx, PB_ERR_STATE, "pb_op2d_mg_features: no network level");
+  const size_t nd = (size_t)H->nh * H->nw;
+  PB_HIP(op->ctx, hipMemcpyAsync(f1, H->f1, sizeof(float) * nd, hipMemcpyDeviceToDevice, op->ctx->stream));
+  PB_HIP(op->ctx, hipMemcpyAsync(f2, H->f2, sizeof(float) * nd, hipMemcpyDeviceToDevice, op->ctx->stream));
+  return PB_OK;
+}

@@ -441,12 +441,18 @@ __global__ void k_scale_from(const double* part, int nchunk, long npix_img, floa
 }
 
 // null-space coefficients (mu, kappa) of each image: least squares on {1, c}
-__global__ void k_null_coef(const double* part, int nchunk, long npix_img, int w, int h, double* coef) {
+// (mode 0) or on {1} alone (mode 1, kappa = 0)
+__global__ void k_null_coef(const double* part, int nchunk, long npix_img, int w, int h, int mode, double* coef) {
   const int b = threadIdx.x;
   double sz = 0.0, scz = 0.0;
   for (int c = 0; c < nchunk; ++c) {
     sz += part[((long)b * nchunk + c) * 2];
     scz += part[((long)b * nchunk + c) * 2 + 1];
+  }
+  if (mode == 1) {
+    coef[2 * b] = sz / (double)npix_img;
+    coef[2 * b + 1] = 0.0;
+    return;
   }
   // sum of the checkerboard over the grid
   const long ne = ((long)h * w + 1) / 2, no = (long)h * w / 2;
@@ -615,6 +621,8 @@ struct pb_nn {
   float* f1 = nullptr;
   float* f2 = nullptr;
   long feat_n = 0, feat_cap = 0, feat_version = -1;
+  std::vector<pb_op2d*> attached;   // operators this network is attached to
+  int null_mode = 0;                // training loss: 0 removes {1, checkerboard}, 1 removes {1}
 };
 
 namespace {
@@ -985,26 +993,40 @@ int op_features(pb_nn* nn, const pb_op2d* op, hipStream_t s) {
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
-// internal entry used by the flexible Krylov solver: z = M(r) for one image
-// on the operator's grid (r, z FP32 vectors of op->n values)
-// ---------------------------------------------------------------------------
-int pb_nn_apply_f32(pb_nn* nn, const pb_op2d* op, const float* r, float* z, hipStream_t s) {
-  if ((op->g.nx & 1) == 0 || (op->g.ny & 1) == 0)
-    return pb_fail(nn->ctx, PB_ERR_ARG, "network preconditioner: grid dimensions must be odd (2^k + 1 nodes)");
-  int rc = op_features(nn, op, s);
-  if (rc) return rc;
-  Acts* a = get_shape(nn, 1, op->g.ny, op->g.nx, false, &rc);
+// z = rms(r) * net(r / rms(r), f1, f2) for a batch of ``nimg`` images of
+// h x w nodes that share one pair of feature fields: r, z float
+// [nimg][h][w], f1, f2 float [h][w] (device).  Used by the multigrid cycle,
+// whose coarse level hands the network both branches at once.
+int pb_nn_forward_batch(pb_nn* nn, int nimg, int h, int w, const float* r, const float* f1, const float* f2,
+                        float* z, hipStream_t s) {
+  if ((h & 1) == 0 || (w & 1) == 0 || ((h - 1) & 7) || ((w - 1) & 7) || h < 9 || w < 9)
+    return pb_fail(nn->ctx, PB_ERR_ARG, "network: a %d x %d grid is not 8k + 1 nodes per axis", h, w);
+  int rc = 0;
+  Acts* a = get_shape(nn, nimg, h, w, false, &rc);
   if (!a) return rc;
-  const long np = op->n;
+  const long np = (long)h * w;
   rms_scale<float>(a, r, s);
-  hipLaunchKernelGGL(k_input<float>, dim3(gs(np)), dim3(256), 0, s, r, (const float*)a->scale, (const float*)nn->f1,
-                     (const float*)nn->f2, 0, np, 1, a->X0);
+  hipLaunchKernelGGL(k_input<float>, dim3(gs(np * nimg)), dim3(256), 0, s, r, (const float*)a->scale, f1, f2, 0, np,
+                     nimg, a->X0);
   forward(nn, a, s);
-  hipLaunchKernelGGL(k_output<float>, dim3(gs(np)), dim3(256), 0, s, (const float*)a->zout, (const float*)a->scale, np,
-                     1, z);
+  hipLaunchKernelGGL(k_output<float>, dim3(gs(np * nimg)), dim3(256), 0, s, (const float*)a->zout,
+                     (const float*)a->scale, np, nimg, z);
   PB_LAUNCH_CHECK(nn->ctx);
   return PB_OK;
+}
+
+// attach bookkeeping: an operator records its network, the network the
+// operators it is attached to, so that destroying either side detaches
+void pb_nn_track(pb_nn* nn, pb_op2d* op, bool add) {
+  if (!nn) return;
+  auto& v = nn->attached;
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i] == op) {
+      if (add) return;
+      v.erase(v.begin() + (long)i);
+      return;
+    }
+  if (add) v.push_back(op);
 }
 
 extern "C" int pb_nn_create(pb_ctx* ctx, unsigned long long seed, pb_nn** out) {
@@ -1070,6 +1092,9 @@ extern "C" int pb_nn_destroy(pb_nn* nn) {
     (void)hipStreamSynchronize(nn->ctx->stream);
     (void)hipStreamSynchronize(nn->ctx->own_stream);
   }
+  for (pb_op2d* op : nn->attached)   // no operator keeps a dangling network
+    if (op->nn == nn) op->nn = nullptr;
+  nn->attached.clear();
   for (Acts* a : nn->shapes) {
     for (void* p : a->allocs) (void)hipFree(p);
     delete a;
@@ -1126,7 +1151,8 @@ extern "C" int pb_nn_train_step(pb_nn* nn, int nimg, int h, int w, const float* 
   if (out_z) PB_HIP(nn->ctx, hipMemcpyAsync(out_z, a->zout, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
   if (train) {
     hipLaunchKernelGGL(k_img_sums<float>, dim3(64, nimg), dim3(256), 0, s, (const float*)a->zout, np, w, 1, a->part, 64);
-    hipLaunchKernelGGL(k_null_coef, dim3(1), dim3(nimg), 0, s, (const double*)a->part, 64, np, w, h, a->coef);
+    hipLaunchKernelGGL(k_null_coef, dim3(1), dim3(nimg), 0, s, (const double*)a->part, 64, np, w, h, nn->null_mode,
+                       a->coef);
     const int nbl = 1024;
     hipLaunchKernelGGL(k_loss, dim3(nbl), dim3(256), 0, s, (const float*)a->zout, target, (const double*)a->coef,
                        img_weight, np, w, nimg, (float)(1.0 / (double)n), a->dOUT, a->lpart);
@@ -1175,6 +1201,15 @@ extern "C" int pb_nn_apply(pb_nn* nn, const pb_op2d* op, const double* r, double
                      np, 1, z);
   PB_LAUNCH_CHECK(nn->ctx);
   PB_HIP(nn->ctx, hipStreamSynchronize(s));
+  return PB_OK;
+}
+
+// Null space the training loss removes from the output before comparing it
+// with the target: 0 = {constant, checkerboard} (the 9-point operator's
+// pseudo-incompressible null space), 1 = {constant} (a coarse level's).
+extern "C" int pb_nn_set_loss_nullspace(pb_nn* nn, int mode) {
+  if (!nn || (mode != 0 && mode != 1)) return pb_fail(nn ? nn->ctx : nullptr, PB_ERR_ARG, "bad null-space mode");
+  nn->null_mode = mode;
   return PB_OK;
 }
 

@@ -199,7 +199,7 @@ extern "C" int pb_op2d_create(pb_ctx* ctx, const pb_grid2d* grid, pb_op2d** out)
   return PB_OK;
 }
 
-void pb_fgmres_release(const pb_op2d* op);   // pb_fgmres.hip
+void pb_fgmres_release(pb_op2d* op);   // pb_fgmres.hip
 
 extern "C" int pb_op2d_destroy(pb_op2d* op) {
   if (!op) return PB_OK;

@@ -332,6 +332,11 @@ struct Work {
 
 }  // namespace pb2d
 
+namespace pbmg {
+struct Hier;
+}
+struct FgWork;
+
 struct pb_op2d {
   pb_ctx* ctx = nullptr;
   pb_grid2d g{};
@@ -366,6 +371,11 @@ struct pb_op2d {
   pb2d::KState* hstate = nullptr;   // pinned host mirror
   double* hred = nullptr;           // pinned host mirror of red
   std::vector<void*> allocs;
+  // learned-preconditioner mode: multigrid hierarchy, flexible-GMRES
+  // workspace and the attached network (owned / referenced by this operator)
+  pbmg::Hier* mg = nullptr;
+  FgWork* fg = nullptr;
+  pb_nn* nn = nullptr;
 };
 
 namespace pb2d {
