@@ -47,8 +47,8 @@ SOLVER_NAMES = {
     "mixed": "plain BiCGStab, FP32 Krylov (FP64 iterate) in FP64 iterative refinement",
     "f64": "plain BiCGStab FP64 (reference semantics, 3 launches/iteration)",
     "f64fast": "plain BiCGStab FP64 (rho, ||r|| from s/t dots, 2 launches/iteration)",
-    "nn": "learned preconditioner: BF16 network initial guess + damped Jacobi, flexible GMRES budget, "
-          "then plain BiCGStab FP64 to the same true-residual tolerance",
+    "nn": "learned preconditioner: FP64 flexible GMRES preconditioned by the FP32 two-branch multigrid cycle "
+          "whose coarse level (65x33 nodes) the BF16 network solves; plain BiCGStab FP64 only past the budget",
 }
 
 
@@ -63,10 +63,38 @@ def build_workload(nx=1025, ny=513, seed=20261202):
     return (Sol, mpv, elem, node, ud, th), rhs
 
 
-def cpu_baseline(state, rhs, full_iters, sample_iters, dt=0.05):
-    """Time the C restatement (oracle) on ``sample_iters`` iterations of the same
-    solve and scale to the full iteration count."""
+def _strip_null(v2d):
+    """v minus its least-squares constant and checkerboard components."""
+    ny, nx = v2d.shape
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    B = np.stack([np.ones(ny * nx), np.where((ii + jj) % 2 == 0, 1.0, -1.0).ravel()], axis=1)
+    c, *_ = np.linalg.lstsq(B, v2d.ravel(), rcond=None)
+    return (v2d.ravel() - B @ c).reshape(ny, nx)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_checks(state, rhs, full_iters, sample_iters, gpu_x, dt=0.05):
+    """The checker leg (oracle only): assemble the system with the NumPy
+    oracle, solve it with the serial C restatement of the reference's solve
+    (oracle/c/pb_oracle.c, FP64 BiCGStab with the reference's semantics) to
+    relative residual 1e-8, and report every GPU solution's relative L2
+    distance to it modulo the null space {1, checkerboard} and its FP64 true
+    residual under the oracle's operator.  The same solve's per-iteration time
+    is the CPU baseline: one core (the reference's serial loop) on a bounded
+    sample of iterations, and all host cores (NumPy's vectorised
+    lap2D_gather_new restatement, BLAS threads = the cores available)."""
     import copy
+    from oracle import bicgstab as obs
     from oracle import cport
     from oracle import laplacian as olp
     from oracle import projection as opj
@@ -81,19 +109,35 @@ def cpu_baseline(state, rhs, full_iters, sample_iters, dt=0.05):
     t_asm = time.perf_counter() - t0
     c = cport.COp(op)
     t0 = time.perf_counter()
-    _, info, it, half, _ = c.bicgstab(b, tol=1e-30, maxiter=sample_iters)
+    xo, info_o, it_o, _, _ = c.bicgstab(b, tol=1e-8, maxiter=6000)
+    t_oracle = time.perf_counter() - t0
+    A = lambda v: olp.lap2d(v, op)
+    ref = _strip_null(xo.reshape(op["ny"], op["nx"]))
+    parity = {"oracle": "oracle/c/pb_oracle.c serial FP64 BiCGStab (reference semantics), tol 1e-8",
+              "oracle_iters": it_o, "oracle_info": info_o,
+              "oracle_true_rel_resid": obs.true_rel_resid(A, b, xo), "oracle_seconds": round(t_oracle, 2)}
+    for name, xg in gpu_x.items():
+        xs = _strip_null(xg)
+        parity[f"rel_l2_{name}"] = float(np.linalg.norm(xs - ref) / np.linalg.norm(ref))
+        parity[f"true_rel_resid_{name}_under_oracle_operator"] = obs.true_rel_resid(A, b, xg.ravel())
+    # CPU baseline, one core: the C restatement on a bounded sample
+    t0 = time.perf_counter()
+    _, _, it, _, _ = c.bicgstab(b, tol=1e-30, maxiter=sample_iters)
     t_it = (time.perf_counter() - t0) / max(it, 1)
     t_solve = t_asm + t_it * full_iters
-    cpu = "unknown"
+    cores = len(os.sched_getaffinity(0))
+    # all cores: the vectorised NumPy restatement's iteration (matvec + BLAS-1
+    # work of one BiCGStab iteration), threads = the cores available
+    t_np = None
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=cores):
+            t0 = time.perf_counter()
+            res = obs.bicgstab(A, b, tol=1e-30, maxiter=20)
+            t_np = (time.perf_counter() - t0) / max(res.niter, 1)
+    except ImportError:
         pass
-    return {
+    cpu = {
         "value": 1.0 / t_solve,
         "unit": "solves/s",
         "cores": 1,
@@ -101,8 +145,34 @@ def cpu_baseline(state, rhs, full_iters, sample_iters, dt=0.05):
         "sample": (f"{it} BiCGStab iterations (serial C restatement of lap2D_gather_new + Templates "
                    f"BiCGStab, FP64) of the same 1025x513 solve, {t_it * 1e3:.2f} ms/iteration, scaled to "
                    f"the {full_iters} iterations the FP64 solve needs, plus {t_asm:.2f} s NumPy assembly; "
-                   f"CPU: {cpu}"),
+                   f"CPU: {_cpu_model()}"),
     }
+    if t_np is not None:
+        cpu["all_cores"] = {"value": 1.0 / (t_asm + t_np * full_iters), "unit": "solves/s", "cores": cores,
+                            "kind": "port",
+                            "sample": (f"20 iterations of the vectorised NumPy restatement (BLAS threads {cores}), "
+                                       f"{t_np * 1e3:.2f} ms/iteration, scaled to {full_iters} iterations")}
+    return parity, cpu
+
+
+def extra_workloads(args):
+    """The 3D and batched workloads' summaries for the default line, each
+    from its own bench.py process (--workload 3d / batch) run here."""
+    import subprocess
+    out = {}
+    for wl, key, extra in (("3d", "workload_3d", []), ("batch", "batch", ["--members-per-size", "16"])):
+        cmd = [sys.executable, os.path.abspath(__file__), "--workload", wl, "--steps", "2", *extra]
+        log("running", " ".join(cmd[1:]))
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            out[key] = {"error": r.stderr[-500:]}
+            continue
+        line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
+        rec = json.loads(line)
+        rec["wall_s"] = round(time.perf_counter() - t0, 1)
+        out[key] = rec
+    return out
 
 
 # Algorithmic words per node of the 3D kernels (cell weights ~ one word per
@@ -119,7 +189,6 @@ def bench_3d(args, nx=257, ny=129, nz=129):
     zero-mean RHS (|k| <= 12), mixed FP32 Krylov in FP64 refinement to true
     relative residual 1e-8; per-kernel device times and algorithmic GB/s."""
     import torch
-    from oracle import laplacian as olp
     from pybella_amd import workloads as W
     from pybella_amd.op3d import Operator3D, grid3_of, phys3_of
     log(f"building 3D workload {nx}x{ny}x{nz} ...")
@@ -206,9 +275,9 @@ def projection_step_rate(mode, rows_per_thread, steps=3, nx=1025, ny=513, dt=0.0
     for _ in range(steps):
         info = step()
     torch.cuda.synchronize()
-    return {"workload": f"projection step (drop-in impl_part device work), {nx}x{ny} rising bubble, dt {dt}",
+    return {"workload": f"projection (drop-in impl_part device work), {nx}x{ny} rising bubble, dt {dt}",
             "mode": mode, "rows_per_thread": rows_per_thread,
-            "steps_per_s": round(steps / (time.perf_counter() - t0), 3), "iterations": info["iters"]}
+            "projections_per_s": round(steps / (time.perf_counter() - t0), 3), "iterations": info["iters"]}
 
 
 def bench_step(args, nx=1025, ny=513, dt=0.05):
@@ -400,7 +469,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", default="auto", choices=["auto", "nn", "f64", "f64fast", "mixed"],
                     help="auto = the learned-preconditioned solve timed beside the fastest plain mode")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle checks and the CPU baseline")
+    ap.add_argument("--no-extra-workloads", action="store_true",
+                    help="skip the 3D and batch summaries of the default line")
     ap.add_argument("--cpu-sample-iters", type=int, default=300)
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--kernel-only", type=int, default=None,
@@ -481,16 +552,17 @@ def main():
         nn_net.load(default_weights_path())
         op.attach_nn(nn_net)
 
-    def step(mode, nn_cfg=None):
+    def step(mode, cfg=None):
         op.assemble(phys, dt, rho, rhoY, rho, rhoY, dS)
         op.scale_rhs(rhs_d, b)
         if mode == "nn":
-            _, info = op.solve(b, x, tol=1e-8, mode="nn", fallback_maxiter=6000, **nn_cfg)
+            _, info = op.solve(b, x, tol=1e-8, mode="nn", fallback_maxiter=6000, **cfg)
         else:
             _, info = op.solve(b, x, tol=1e-8, maxiter=6000, mode=mode, inner_tol=1e-3)
         return info
 
-    def rate_of(fn, reps=2):
+    def rate_of(fn, reps=3):
+        fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -499,10 +571,10 @@ def main():
         return reps / (time.perf_counter() - t0)
 
     from pybella_amd.autotune import tune_tile
-    # plain FP64 reference-semantics solve (iteration count, parity anchor)
+    # plain FP64 reference-semantics solve: the iteration count the learned
+    # mode's budget is measured against
     info64 = step("f64")
     log("f64 (exact) solve:", info64)
-    # calibrate every plain mode (tile height autotuned per mode)
     rates, tiles, infos_cal = {}, {}, {}
     plain_modes = ["f64", "f64fast", "mixed"] if args.mode in ("auto", "nn") else [args.mode]
     for m in plain_modes:
@@ -512,25 +584,20 @@ def main():
         tiles[m] = {"rows_per_thread": best, "seconds": {str(k): round(v, 5) for k, v in table.items()}}
         log(f"mode {m}: {rates[m]:.2f} solves/s, tile rows/thread {best}, {infos_cal[m]}")
     plain = max(rates, key=rates.get)
-    # the learned-preconditioned solve: network initial guess (+ damped Jacobi),
-    # flexible GMRES for ``maxiter`` iterations, then plain BiCGStab (two-launch
-    # iteration at the fast mode's tile) to the same true-residual tolerance
+    op.set_tile(tiles[plain]["rows_per_thread"])
+
+    # the learned mode: FGMRES preconditioned by the two-branch multigrid
+    # cycle whose coarse level the network solves; plain BiCGStab takes over
+    # only past 2x the plain FP64 iteration count (the fallback budget)
     nn_cfg, nn_table, fallback_after = None, {}, None
     if nn_net is not None:
-        op.set_tile(tiles.get("f64fast", tiles[plain])["rows_per_thread"])
-        # the learned phase never runs past 2x the plain FP64 iteration count
-        # measured above; plain BiCGStab takes over from there
         fallback_after = 2 * int(info64["iters"])
-        cands = [dict(maxiter=min(m, fallback_after), sweeps=sw) for m, sw in ((0, 2), (0, 0), (10, 2), (40, 1))]
-        for c in cands:
-            step("nn", c)
-            r = rate_of(lambda: step("nn", c))
-            nn_table[json.dumps(c, sort_keys=True)] = round(r, 3)
-            log(f"mode nn {c}: {r:.2f} solves/s")
+        for sw in (1, 2):
+            c = dict(maxiter=fallback_after, sweeps=sw, krylov_m=30)
+            nn_table[json.dumps(c, sort_keys=True)] = round(rate_of(lambda: step("nn", c)), 3)
+            log(f"mode nn {c}: {nn_table[json.dumps(c, sort_keys=True)]:.2f} solves/s")
         nn_cfg = json.loads(max(nn_table, key=nn_table.get))
     mode = "nn" if nn_cfg is not None else plain
-    if mode != "nn":
-        op.set_tile(tiles[mode]["rows_per_thread"])
 
     def timed(m, cfg):
         for _ in range(args.warmup):
@@ -548,48 +615,72 @@ def main():
     el, infos = timed(mode, nn_cfg)
     log("timed:", mode, nn_cfg, infos[-1])
     value = args.steps / el
+    x_main = x.clone()
     plain_rate, plain_info = None, None
+    comparators = {}
     if mode == "nn":   # the plain BiCGStab it is compared with, timed the same way
-        op.set_tile(tiles[plain]["rows_per_thread"])
         el_p, infos_p = timed(plain, None)
         plain_rate, plain_info = args.steps / el_p, infos_p[-1]
         log("timed plain:", plain, plain_info)
-        op.set_tile(tiles.get("f64fast", tiles[plain])["rows_per_thread"])
+        # the same FGMRES and cycle without the network (its level smoothed by
+        # Jacobi sweeps instead), and the cycle continued to the coarsest level
+        for name, use in (("cycle_without_network", False), ("multigrid_to_coarsest_no_network", "mg")):
+            c = dict(nn_cfg, use_network=use)
+            r = rate_of(lambda: step("nn", c))
+            inf = step("nn", c)
+            comparators[name] = {"solves_per_s": round(r, 3), "iterations": inf["iters"],
+                                 "true_rel_resid": inf["true_rel_resid"], "fallback": inf["fallback"]}
+            log(name, comparators[name])
+    x_plain = x.clone()
+    op.set_tile(tiles[plain]["rows_per_thread"])
 
-    # roofline of the solver kernels (HIP events, same stream)
-    prec = "f32" if mode == "mixed" else "f64"
-    wbytes = 4 if prec == "f32" else 8
+    # roofline of the plain BiCGStab's dominant launch: HIP events around every
+    # launch inside a real solve, and on pattern-filled vectors beside it
     n = grid.nx * grid.ny
-    kt = {k: op.time_kernel(k, prec, args.kernel_reps) for k in (0, 1, 2, 3)}   # at the tuned tile
-    per_iter = {k: kt[k] for k in (1, 2)}
-    dom = max(per_iter, key=per_iter.get)
-    alg = KERNEL_WORDS[dom] * wbytes * n + KERNEL_XBYTES.get(dom, 0) * n
-    achieved = alg / (kt[dom] * 1e-6) / 1e9
-    traffic = None
-    try:   # HBM bytes per launch from the committed rocprofv3 counter summary
+    kt = {k: op.time_kernel(k, "f64", args.kernel_reps) for k in (0, 1, 2, 3)}
+    op.assemble(phys, dt, rho, rhoY, rho, rhoY, dS)
+    op.scale_rhs(rhs_d, b)
+    in_solve = {}
+    for k in (1, 2):
+        us, cnt, _ = op.time_in_solve(b, k)
+        in_solve[k] = (us, cnt)
+    dom = max((1, 2), key=lambda k: in_solve[k][0])
+    alg = KERNEL_WORDS[dom] * 8 * n + KERNEL_XBYTES.get(dom, 0) * n
+    achieved = alg / (in_solve[dom][0] * 1e-6) / 1e9
+    achieved_syn = alg / (kt[dom] * 1e-6) / 1e9
+    roof = {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_us": round(in_solve[dom][0], 3), "timed_in_solve_launches": in_solve[dom][1], "alg_bytes": alg,
+            "synthetic_state": {"avg_us": round(kt[dom], 3), "achieved": round(achieved_syn, 1),
+                                "frac": round(achieved_syn / HBM_PEAK_GBS, 4)},
+            "kernels_us_in_solve": {KERNEL_NAMES[k]: round(v[0], 3) for k, v in in_solve.items()},
+            "kernels_us_synthetic": {KERNEL_NAMES[k]: round(v, 3) for k, v in kt.items()}}
+    try:   # HBM bytes per launch: rocprofv3 FETCH_SIZE / WRITE_SIZE of this kernel (profiles/)
         with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_counters.json")) as f:
             hc = json.load(f)
-        tile_now = tiles["f64fast" if mode == "nn" and "f64fast" in tiles else (plain if mode == "nn" else mode)]
-        rec = hc["kernels"].get(f"{KERNEL_NAMES[dom]}/{prec}/{tile_now['rows_per_thread']}")
+        rec = hc["kernels"].get(f"{KERNEL_NAMES[dom]}/f64/{tiles[plain]['rows_per_thread']}")
         if rec:
-            traffic = int(1024 * (rec["fetch_kib"] * hc["fetch_scale"] + rec["write_kib"] * hc["write_scale"]))
+            roof["traffic"] = int(1024 * (rec["fetch_kib"] * hc["fetch_scale"] + rec["write_kib"] * hc["write_scale"]))
+            roof["traffic_source"] = "profiles/hbm_counters.json (rocprofv3 --pmc of the same kernel and tile)"
     except (OSError, KeyError, ValueError):
-        traffic = None
-    roof = {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic, "avg_us": round(kt[dom], 3), "alg_bytes": alg,
-            "kernels_us": {KERNEL_NAMES[k]: round(v, 3) for k, v in kt.items()}}
+        pass
 
-    # time steps/s beside the solve rate: whole projection steps, device-resident
-    step_rate = None
+    # projections (the drop-in's whole device work per call) per second
+    proj_rate = None
     if "f64fast" in tiles:
-        step_rate = projection_step_rate("f64fast", tiles["f64fast"]["rows_per_thread"])
-        log("projection step:", step_rate)
+        proj_rate = projection_step_rate("f64fast", tiles["f64fast"]["rows_per_thread"])
+        log("projection step:", proj_rate)
 
+    parity = None
     cpu = None
     if not args.no_cpu_baseline:
-        log("cpu baseline ...")
-        cpu = cpu_baseline(state, rhs, info64["iters"], args.cpu_sample_iters)
+        log("oracle solve and cpu baseline ...")
+        parity, cpu = oracle_checks(state, rhs, info64["iters"], args.cpu_sample_iters,
+                                    {"nn": x_main.cpu().numpy(), plain: x_plain.cpu().numpy()})
+        log("parity:", parity)
+    extra = {}
+    if not args.no_extra_workloads:
+        extra = extra_workloads(args)
 
     out = {
         "metric": "solves/s at rel. resid 1e-8, 1024x512 nodal solve: NN-precond vs plain BiCGStab",
@@ -602,7 +693,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"mixed": "f32", "nn": "f64+bf16"}.get(mode, "f64"),
+        "dtype": {"mixed": "f32", "nn": "f64+f32+bf16"}.get(mode, "f64"),
         "data": "synthetic",
         "config": {"workload": "isolated 2D nodal 9-point solve, 1025x513 nodes, stratified theta 300->330 K, "
                                "psinc, seeded 64-mode RHS, true rel. resid <= 1e-8",
@@ -610,16 +701,20 @@ def main():
                    "nn_config": nn_cfg,
                    "nn_fallback_after_iters": fallback_after,
                    "iterations": infos[-1]["iters"], "refinements": infos[-1]["outer"],
+                   "fallback_used": infos[-1].get("fallback", 0),
                    "true_rel_resid": infos[-1]["true_rel_resid"],
                    "f64_iterations": info64["iters"],
                    "plain_bicgstab": None if plain_rate is None else {
                        "solver": SOLVER_NAMES[plain], "solves_per_s": round(plain_rate, 3),
                        "iterations": plain_info["iters"], "true_rel_resid": plain_info["true_rel_resid"]},
                    "nn_over_plain": None if plain_rate is None else round(value / plain_rate, 3),
+                   "comparators": comparators,
+                   "parity_vs_oracle": parity,
                    "calibration_solves_per_s": {**{k: round(v, 3) for k, v in rates.items()},
                                                 **{"nn " + k: v for k, v in nn_table.items()}},
                    "tiles": tiles,
-                   "projection_step": step_rate},
+                   "projections": proj_rate,
+                   **extra},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

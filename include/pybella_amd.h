@@ -198,6 +198,15 @@ int pb_op2d_set_tile(pb_op2d* op, int rows_per_thread);
  * iteration; kernel 3 is the FP64 true-residual stencil. */
 int pb_op2d_time_kernel(pb_op2d* op, int kernel, int precision, int reps, double* avg_us);
 
+/* Average device time (microseconds) of the A (kernel 1) or B (kernel 2)
+ * launch of the PB_SOLVE_F64_FAST iteration inside a real solve of b to
+ * tolerance tol (x0 = 0): every launch of that kernel in the live iterations
+ * is bracketed by HIP events on the solve stream (iterations launched
+ * directly, no graph); *launches = the number timed; x receives the
+ * solution.  Synchronous.  Measurement aid; no reference counterpart. */
+int pb_op2d_time_in_solve(pb_op2d* op, const double* b, double* x, double tol, int kernel,
+                          double* avg_us, int* launches);
+
 /* divergence of (rhou, rhov)*Y at nodes -> rhs (isc), divided by dt.
  * Zeroes rhou/rhov/rhow in the two cell rows next to y walls first. */
 int pb_divergence_nodes(pb_ctx* ctx, const pb_grid2d* grid, double dt,
@@ -255,7 +264,7 @@ int pb_nn_set_loss_nullspace(pb_nn* nn, int mode);
  * (two branches, carried as two channels), continues with Galerkin operators
  * R A P (9-point stencils on unique grids: a periodic axis keeps n - 1 nodes)
  * and solves one coarse level (the first whose grid, periodic nodes
- * duplicated, is 8k+1 nodes per axis and at most 129 x 65) with the network.
+ * duplicated, is 8k+1 nodes per axis and at most 65 x 33) with the network.
  * One application z = M v (v FP64, z FP32 Krylov vectors): coarse 0 the
  * network, 1 Jacobi sweeps on that level, 2 the cycle continued to the
  * coarsest level; nu Jacobi sweeps each way (damping omega); branches 1 or 2.
@@ -308,6 +317,12 @@ int pb_op3d_matvec(pb_op3d* op, const void* x, void* y, int precision);
  * (legacy tolerance) and PB_SOLVE_MIXED (FP32 inner solves in FP64
  * refinement).  Synchronous; returns the info code as pb_bicgstab_solve. */
 int pb_op3d_solve(pb_op3d* op, const double* b, double* x, const pb_solve_opts* opts, pb_solve_info* info);
+/* The tile of every block of a 3D stencil march launched with gx * gy * gz
+ * blocks of 64 x 8 nodes and one z chunk each (the launch's gz is
+ * ceil(nz / chunk)): out[3 h + 0..2] = x tile, y tile, z chunk of hardware
+ * block h, in the XCD-aware order the kernels use.  Host only, no GPU call;
+ * a check of the block mapping.  No reference counterpart. */
+int pb_op3d_march_map(int gx, int gy, int gz, int* out);
 /* Average device time (microseconds) of a 3D kernel: 0 matvec, 1 iteration
  * launch 1 (x, r, p updates + v = A p + rt.v), 2 launch 2 (t = A s + 5 dots).
  * Measurement aid; no reference counterpart. */

@@ -70,6 +70,7 @@ SIGNATURES = {
     "pb_op3d_matvec": (I, [VP, VP, VP, I]),
     "pb_op3d_solve": (I, [VP, VP, VP, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "pb_op3d_time_kernel": (I, [VP, I, I, I, C.POINTER(D)]),
+    "pb_op3d_march_map": (I, [I, I, I, C.POINTER(I)]),
     "pb_ctx_create": (I, [I, C.POINTER(VP)]),
     "pb_ctx_destroy": (I, [VP]),
     "pb_ctx_set_stream": (I, [VP, VP]),
@@ -85,6 +86,7 @@ SIGNATURES = {
     "pb_bicgstab_solve": (I, [VP, VP, VP, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "pb_bicgstab_solve_noalloc": (I, [VP, VP, VP, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "pb_op2d_time_kernel": (I, [VP, I, I, I, C.POINTER(D)]),
+    "pb_op2d_time_in_solve": (I, [VP, VP, VP, D, I, C.POINTER(D), C.POINTER(I)]),
     "pb_op2d_set_tile": (I, [VP, I]),
     "pb_divergence_nodes": (I, [VP, C.POINTER(Grid2D), D, VP, VP, VP, VP, VP, VP]),
     "pb_correction_nodes": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), D, D, VP, VP, VP, VP,

@@ -14,8 +14,9 @@
 // applied twice (Giraud et al. 2005) and the normalisation; the Givens
 // rotations and the least-squares residual estimate |g_{j+1}| are formed on
 // the device by the normalisation kernel's first thread.  Reductions are
-// per-block partial records that every consuming block re-reduces in the same
-// fixed order (deterministic, no grid-wide fence).  The host reads the
+// per-block partial records (wave-shuffle block sums) summed in a fixed order
+// by a one-block-per-column kernel, or, for the norm, by every block of the
+// normalisation (deterministic, no grid-wide fence).  The host reads the
 // estimate one iteration behind the device (pinned copy + event), so it never
 // waits between iterations; it waits at once only when the convergence rate
 // predicts that the step just issued reaches the tolerance.  After a cycle,
@@ -32,7 +33,7 @@ void pb_nn_track(pb_nn* nn, pb_op2d* op, bool add);
 
 namespace {
 
-constexpr int kFgBlocks = 128;   // blocks of the reduction kernels
+constexpr int kFgBlocks = 1024;  // blocks of the vector passes (each writes one partial record)
 constexpr int kFgThreads = 256;
 constexpr int kMaxM = 60;        // restart length cap (K = m + 1 <= 64 column sums per record)
 
@@ -45,9 +46,8 @@ struct FgWork {
   float* Z = nullptr;        // m x n preconditioned directions
   double* w = nullptr;       // n
   double* w2 = nullptr;      // n
-  double* part = nullptr;    // [kFgBlocks][m + 1] partial records (three sets)
-  double* part2 = nullptr;
-  double* part3 = nullptr;
+  double* part = nullptr;    // [m + 1][kFgBlocks] partial records, one column per block
+  double* part3 = nullptr;   // [kFgBlocks] partial ||w||^2
   double* H = nullptr;       // (m + 1) x m rotated Hessenberg, column-major
   double* cs = nullptr;      // m
   double* sn = nullptr;      // m
@@ -94,21 +94,10 @@ __global__ __launch_bounds__(NTHR) void kf_matvec(OpDev<double> op, const float*
 __device__ __forceinline__ long gstride() { return (long)gridDim.x * blockDim.x; }
 __device__ __forceinline__ long gfirst() { return blockIdx.x * (long)blockDim.x + threadIdx.x; }
 
-// column sums of a [kFgBlocks][KP] record set, the same order in every block:
-// thread k adds rows 0, 1, ... of column k
-__device__ __forceinline__ void reduce_cols(const double* part, int KP, int K, double* h) {
-  const int tid = threadIdx.x;
-  if (tid < K) {
-    double s = 0.0;
-    for (int b = 0; b < kFgBlocks; ++b) s += part[(long)b * KP + tid];
-    h[tid] = s;
-  }
-  __syncthreads();
-}
-
-// this block's partial dots of V_0..V_{K-1} with x, written as record
-// blockIdx.x of ``part``; eight vectors per pass over x
-__device__ __forceinline__ void block_dots(const double* V, long n, int K, const double* x, double* part, int KP,
+// this block's partial dots of V_0..V_{K-1} with x, stored as entry
+// blockIdx.x of rows 0..K-1 of ``part`` ([K][kFgBlocks]); eight vectors per
+// pass over x, wave-shuffle block sums
+__device__ __forceinline__ void block_dots(const double* V, long n, int K, const double* x, double* part,
                                            double* sh) {
   for (int k0 = 0; k0 < K; k0 += 8) {
     double acc[8];
@@ -124,41 +113,51 @@ __device__ __forceinline__ void block_dots(const double* V, long n, int K, const
     if (threadIdx.x == 0) {
 #pragma unroll
       for (int q = 0; q < 8; ++q)
-        if (k0 + q < K) part[(long)blockIdx.x * KP + k0 + q] = acc[q];
+        if (k0 + q < K) part[(long)(k0 + q) * kFgBlocks + blockIdx.x] = acc[q];
     }
     __syncthreads();   // sh is reused by the next pass
   }
 }
 
 // first Gram-Schmidt pass: partial dots V^T w
-__global__ __launch_bounds__(kFgThreads) void kf_dots(const double* V, long n, int K, const double* w, double* part,
-                                                      int KP) {
+__global__ __launch_bounds__(kFgThreads) void kf_dots(const double* V, long n, int K, const double* w, double* part) {
   __shared__ double sh[8 * (kFgThreads / kWave)];
-  block_dots(V, n, K, w, part, KP, sh);
+  block_dots(V, n, K, w, part, sh);
 }
 
-// h1 = V^T w (reduced), w2 = w - V h1, partial dots V^T w2
-__global__ __launch_bounds__(kFgThreads) void kf_orth1(const double* V, long n, int K, int KP, const double* part,
-                                                       const double* w, double* w2, double* part2, double* hout) {
+// h[k] = sum of row k of part (one block per row, fixed order: thread t adds
+// entries t, t + 256, ... then a wave-shuffle block sum)
+__global__ __launch_bounds__(kFgThreads) void kf_colsum(const double* part, double* h) {
+  __shared__ double sh[kFgThreads / kWave];
+  const double* row = part + (long)blockIdx.x * kFgBlocks;
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < kFgBlocks; b += kFgThreads) v[0] += row[b];
+  block_sum_w<1, kFgThreads / kWave>(v, sh);
+  if (threadIdx.x == 0) h[blockIdx.x] = v[0];
+}
+
+// w2 = w - V h1, partial dots V^T w2 (the second Gram-Schmidt pass)
+__global__ __launch_bounds__(kFgThreads) void kf_orth1(const double* V, long n, int K, const double* h1,
+                                                       const double* w, double* w2, double* part) {
   __shared__ double h[kMaxM + 4];
   __shared__ double sh[8 * (kFgThreads / kWave)];
-  reduce_cols(part, KP, K, h);
-  if (blockIdx.x == 0 && (int)threadIdx.x < K) hout[threadIdx.x] = h[threadIdx.x];
+  if ((int)threadIdx.x < K) h[threadIdx.x] = h1[threadIdx.x];
+  __syncthreads();
   for (long g = gfirst(); g < n; g += gstride()) {
     double a = w[g];
     for (int k = 0; k < K; ++k) a -= h[k] * V[(long)k * n + g];
     w2[g] = a;
   }
-  block_dots(V, n, K, w2, part2, KP, sh);
+  block_dots(V, n, K, w2, part, sh);
 }
 
-// h2 = V^T w2 (reduced), w = w2 - V h2, partial ||w||^2
-__global__ __launch_bounds__(kFgThreads) void kf_orth2(const double* V, long n, int K, int KP, const double* part2,
-                                                       const double* w2, double* w, double* part3, double* hout) {
+// w = w2 - V h2, partial ||w||^2
+__global__ __launch_bounds__(kFgThreads) void kf_orth2(const double* V, long n, int K, const double* h2,
+                                                       const double* w2, double* w, double* part3) {
   __shared__ double h[kMaxM + 4];
   __shared__ double sh[kFgThreads / kWave];
-  reduce_cols(part2, KP, K, h);
-  if (blockIdx.x == 0 && (int)threadIdx.x < K) hout[threadIdx.x] = h[threadIdx.x];
+  if ((int)threadIdx.x < K) h[threadIdx.x] = h2[threadIdx.x];
+  __syncthreads();
   double ss[1] = {0.0};
   for (long g = gfirst(); g < n; g += gstride()) {
     double a = w2[g];
@@ -176,12 +175,12 @@ __global__ __launch_bounds__(kFgThreads) void kf_orth2(const double* V, long n, 
 __global__ __launch_bounds__(kFgThreads) void kf_norm(int j, int m, long n, const double* part3, const double* w,
                                                       double* vnext, const double* h1, const double* h2, double* H,
                                                       double* cs, double* sn, double* g, double* est) {
+  __shared__ double sh[kFgThreads / kWave];
   __shared__ double hn_s;
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int b = 0; b < kFgBlocks; ++b) s += part3[b];
-    hn_s = sqrt(s);
-  }
+  double ss[1] = {0.0};
+  for (int b = threadIdx.x; b < kFgBlocks; b += kFgThreads) ss[0] += part3[b];
+  block_sum_w<1, kFgThreads / kWave>(ss, sh);
+  if (threadIdx.x == 0) hn_s = sqrt(ss[0]);
   __syncthreads();
   const double hn = hn_s;
   const double inv = hn > 0.0 ? 1.0 / hn : 0.0;
@@ -279,7 +278,6 @@ int fg_alloc(pb_op2d* op, int m, FgWork** out) {
   rc |= A(&f->w, n);
   rc |= A(&f->w2, n);
   rc |= A(&f->part, (long)kFgBlocks * (m + 1));
-  rc |= A(&f->part2, (long)kFgBlocks * (m + 1));
   rc |= A(&f->part3, kFgBlocks);
   rc |= A(&f->H, (long)(m + 1) * m);
   rc |= A(&f->cs, m);
@@ -315,18 +313,20 @@ void launch_fg_matvec(const pb_op2d* op, const float* z, double* w, hipStream_t 
 // One Arnoldi step j: Z_j = M(V_j), then V_{j+1}, Hessenberg column j, est[j].
 int arnoldi_step(pb_op2d* op, pb_nn* nn, FgWork* f, int j, const pbmg::CycleOpts& co, hipStream_t s) {
   const long n = op->n;
-  const int m = f->m, KP = m + 1, K = j + 1;
+  const int m = f->m, K = j + 1;
   double* vj = f->V + (long)j * n;
   float* zj = f->Z + (long)j * n;
   int rc = pbmg::cycle(op, nn, vj, zj, co, s);
   if (rc) return rc;
   launch_fg_matvec(op, zj, f->w, s);
   const dim3 gb(kFgBlocks), tb(kFgThreads);
-  hipLaunchKernelGGL(kf_dots, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->w, f->part, KP);
-  hipLaunchKernelGGL(kf_orth1, gb, tb, 0, s, (const double*)f->V, n, K, KP, (const double*)f->part,
-                     (const double*)f->w, f->w2, f->part2, f->h1);
-  hipLaunchKernelGGL(kf_orth2, gb, tb, 0, s, (const double*)f->V, n, K, KP, (const double*)f->part2,
-                     (const double*)f->w2, f->w, f->part3, f->h2);
+  hipLaunchKernelGGL(kf_dots, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->w, f->part);
+  hipLaunchKernelGGL(kf_colsum, dim3(K), tb, 0, s, (const double*)f->part, f->h1);
+  hipLaunchKernelGGL(kf_orth1, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h1, (const double*)f->w,
+                     f->w2, f->part);
+  hipLaunchKernelGGL(kf_colsum, dim3(K), tb, 0, s, (const double*)f->part, f->h2);
+  hipLaunchKernelGGL(kf_orth2, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h2, (const double*)f->w2,
+                     f->w, f->part3);
   hipLaunchKernelGGL(kf_norm, gb, tb, 0, s, j, m, n, (const double*)f->part3, (const double*)f->w,
                      f->V + (long)(j + 1) * n, (const double*)f->h1, (const double*)f->h2, f->H, f->cs, f->sn, f->g,
                      f->est);

@@ -1291,6 +1291,92 @@ extern "C" int pb_op2d_time_kernel(pb_op2d* op, int kernel, int precision, int r
   return pb_fail(op->ctx, PB_ERR_ARG, "pb_op2d_time_kernel: unknown precision %d", precision);
 }
 
+// Average device time of the A (kernel 1) or B (kernel 2) launch inside a real
+// FP64 two-launch solve of b (x0 = 0, tolerance tol): the solve's iterations
+// are launched directly, each launch of the chosen kernel between two HIP
+// events on the solve stream; *launches = how many were timed.  The solve's
+// result is the fast mode's (same kernels and order); x receives it.
+extern "C" int pb_op2d_time_in_solve(pb_op2d* op, const double* b, double* x, double tol, int kernel,
+                                     double* avg_us, int* launches) {
+  if (!op) return PB_ERR_ARG;
+  pb_ctx* ctx = op->ctx;
+  if (!b || !x || !avg_us || !launches || (kernel != 1 && kernel != 2) || !(tol > 0))
+    return pb_fail(ctx, PB_ERR_ARG, "pb_op2d_time_in_solve: bad argument");
+  if (!op->assembled) return pb_fail(ctx, PB_ERR_STATE, "pb_op2d_time_in_solve: operator not assembled");
+  int rc = alloc_work(op);
+  if (rc) return rc;
+  hipStream_t s = solve_stream(ctx);
+  Work<double>& w = op->w64;
+  KState init{};
+  init.tol = tol;
+  init.maxiter = 20000;
+  init.legacy = 1;
+  init.status = ST_RUN;
+  init.phase = PH_INIT;
+  hipLaunchKernelGGL(k_put_state, dim3(1), dim3(1), 0, s, w.S, init);
+  {
+    const dim3 grd = stencil_grid(op, op->ry);
+    const OpDev<double> d = opdev<double>(op);
+    InitIn<double> in{nullptr, 0};
+    InitOut<double> o{b, w.r[0], w.rt, w.p[0], w.v[0], w.t, w.x, nullptr, 0, 0.0, 0.0};
+    PB_DISPATCH(k_init5, double, op, grd, s, d, in, o, w.PB);
+  }
+  std::vector<hipEvent_t> ev;
+  auto record = [&](hipEvent_t* e) -> int {
+    PB_HIP(ctx, hipEventCreate(e));
+    PB_HIP(ctx, hipEventRecord(*e, s));
+    ev.push_back(*e);
+    return PB_OK;
+  };
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
+  auto timed = [&](bool is_a, int k) -> int {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool t = (kernel == 1) == is_a;
+    if (t && (rc = record(&e0))) return rc;
+    if (is_a) launch_a<double>(op, w, k, s);
+    else launch_b<double>(op, w, k, s);
+    if (t) {
+      if ((rc = record(&e1))) return rc;
+      pairs.emplace_back(e0, e1);
+    }
+    return PB_OK;
+  };
+  if ((rc = timed(true, 1))) return rc;
+  KState h{};
+  const int chunk = 32;
+  for (long issued = 1;; issued += chunk) {
+    PB_HIP(ctx, hipMemcpyAsync(op->hstate + 1, w.S + 1, sizeof(KState), hipMemcpyDeviceToHost, s));
+    PB_HIP(ctx, hipStreamSynchronize(s));
+    h = op->hstate[1];
+    if (st_final(h.status)) break;
+    if (issued > 20000 + 2L * chunk)
+      return pb_fail(ctx, PB_ERR_STATE, "pb_op2d_time_in_solve: no final state after %ld iterations", issued);
+    for (int i = 0; i < chunk; ++i) {   // B(k), A(k + 1) from an odd k: the parity pattern of launch_chunk
+      const int k = 1 + i;
+      if ((rc = timed(false, k))) return rc;
+      if ((rc = timed(true, k + 1))) return rc;
+    }
+    PB_LAUNCH_CHECK(ctx);
+  }
+  // the first h.iter launches of either kernel are live iterations; the
+  // launch that applies the last x update and those past the exit return early
+  int ntimed = 0;
+  double total_ms = 0.0;
+  for (size_t q = 0; q < pairs.size() && (long)q < (long)h.iter; ++q) {
+    float ms = 0.f;
+    PB_HIP(ctx, hipEventElapsedTime(&ms, pairs[q].first, pairs[q].second));
+    total_ms += ms;
+    ++ntimed;
+  }
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  PB_HIP(ctx, hipMemcpyAsync(x, w.x, sizeof(double) * op->n, hipMemcpyDeviceToDevice, s));
+  PB_HIP(ctx, hipStreamSynchronize(s));
+  if (h.status != ST_CONV) return pb_fail(ctx, PB_ERR_STATE, "pb_op2d_time_in_solve: solve ended with status %d", h.status);
+  *launches = ntimed;
+  *avg_us = ntimed ? 1e3 * total_ms / ntimed : 0.0;
+  return PB_OK;
+}
+
 extern "C" int pb_op2d_set_tile(pb_op2d* op, int rows_per_thread) {
   if (!op) return PB_ERR_ARG;
   if (rows_per_thread < 2 || rows_per_thread > 8 || rows_per_thread == 7)

@@ -382,6 +382,44 @@ __global__ void kc_resid(Dims d, const float* st, const float* v, const float* z
   }
 }
 
+// All ``sweeps`` Jacobi sweeps from z = 0 on a small level in one block, the
+// iterate double-buffered in LDS (the coarsest level's solve).
+constexpr int kBlockSweepMax = 2048;   // nodes
+constexpr int kBlockSweepThreads = 1024;
+__global__ __launch_bounds__(kBlockSweepThreads) void kc_sweeps_block(Dims d, const float* st, const float* idg,
+                                                                      const float* v, int sweeps, float omega,
+                                                                      float* zout) {
+  __shared__ float zb[2][2 * kBlockSweepMax];
+  const int n = d.ny * d.nx;
+  for (int g = threadIdx.x; g < n; g += kBlockSweepThreads) {
+    zb[0][g] = omega * v[g] * idg[g];
+    zb[0][n + g] = omega * v[n + g] * idg[g];
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int k = 1; k < sweeps; ++k) {
+    for (int g = threadIdx.x; g < n; g += kBlockSweepThreads) {
+      Nb nb;
+      load_row(d, st, g, nb);
+      const float ig = idg[g];
+#pragma unroll
+      for (int ch = 0; ch < 2; ++ch) {
+        const float* zc = zb[cur] + ch * n;
+        float acc = 0.f;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc += nb.s[q] * zc[nb.idx[q]];
+        zb[cur ^ 1][ch * n + g] = zc[g] + omega * (v[ch * n + g] - acc) * ig;
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  for (int g = threadIdx.x; g < n; g += kBlockSweepThreads) {
+    zout[g] = zb[cur][g];
+    zout[n + g] = zb[cur][n + g];
+  }
+}
+
 // full weighting onto the next level, both branches
 __global__ void kc_restrict(Dims f, Dims c, const float* r, float* vc) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y;
@@ -442,16 +480,18 @@ __global__ void kc_prolong_sweep(Dims d, Dims c, const float* st, const float* i
   }
 }
 
-// z *= alpha per branch, alpha = (A z . v) / (A z . A z) minimising
-// ||v - alpha A z||: the network predicts the shape of the coarse correction,
-// its scale is fitted here.  One block of 1024 threads (the network's level
-// has at most 129 x 65 nodes).
+// z *= alpha per branch, alpha = (z . v) / (z . A z), the energy-norm
+// (Galerkin) scale of the direction z: the network predicts the shape of the
+// coarse correction, its scale is fitted here.  A residual fit (A z . v) /
+// (A z . A z) would weight the high-frequency part of A z and shrink the
+// smooth modes the coarse level is there for.  One block of 1024 threads (the
+// network's level has at most 65 x 33 nodes).
 constexpr int kAlphaThreads = 1024;
 __global__ __launch_bounds__(kAlphaThreads) void kc_net_alpha(Dims d, const float* st, const float* v, float* z) {
   __shared__ double sh[4 * (kAlphaThreads / kWave)];
   __shared__ float alpha[2];
   const long n = (long)d.ny * d.nx;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};   // (Az.v, Az.Az) per branch
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};   // (z.v, z.Az) per branch
   for (long g = threadIdx.x; g < n; g += kAlphaThreads) {
     Nb nb;
     load_row(d, st, g, nb);
@@ -461,14 +501,15 @@ __global__ __launch_bounds__(kAlphaThreads) void kc_net_alpha(Dims d, const floa
       float az = 0.f;
 #pragma unroll
       for (int k = 0; k < 9; ++k) az += nb.s[k] * zc[nb.idx[k]];
-      acc[2 * ch] += (double)az * (double)v[ch * n + g];
-      acc[2 * ch + 1] += (double)az * (double)az;
+      const double zg = (double)zc[g];
+      acc[2 * ch] += zg * (double)v[ch * n + g];
+      acc[2 * ch + 1] += zg * (double)az;
     }
   }
   block_sum_w<4, kAlphaThreads / kWave>(acc, sh);
   if (threadIdx.x == 0) {
-    alpha[0] = acc[1] > 0.0 ? (float)(acc[0] / acc[1]) : 0.f;
-    alpha[1] = acc[3] > 0.0 ? (float)(acc[2] / acc[3]) : 0.f;
+    alpha[0] = acc[1] != 0.0 ? (float)(acc[0] / acc[1]) : 0.f;
+    alpha[1] = acc[3] != 0.0 ? (float)(acc[2] / acc[3]) : 0.f;
   }
   __syncthreads();
   const float a0 = alpha[0], a1 = alpha[1];
@@ -549,12 +590,13 @@ int layout(pb_op2d* op, Hier* H) {
   if (H->nlev < 2)
     return pb_fail(op->ctx, PB_ERR_ARG, "multigrid: a %d x %d grid has no coarse level", g.nx, g.ny);
   // the network level: the first coarse level whose duplicated grid is 8k+1
-  // nodes per axis (>= 9) and at most 129 x 65 nodes
+  // nodes per axis (>= 9) and at most 65 x 33 nodes, where the network's
+  // receptive field (about +-60 nodes) spans the whole grid
   H->nn_level = -1;
   for (int l = 1; l < H->nlev; ++l) {
     const Dims d = H->L[l].d;
     const int h = d.ny + (d.py ? 1 : 0), w = d.nx + (d.px ? 1 : 0);
-    if (h >= 9 && w >= 9 && ((h - 1) & 7) == 0 && ((w - 1) & 7) == 0 && (long)h * w <= 129L * 65) {
+    if (h >= 9 && w >= 9 && ((h - 1) & 7) == 0 && ((w - 1) & 7) == 0 && (long)h * w <= 65L * 33) {
       H->nn_level = l;
       H->nh = h;
       H->nw = w;
@@ -627,6 +669,11 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
       return cur;
     }
     const int sweeps = std::max(1, o.coarse == COARSE_MG ? o.bottom_sweeps : o.coarse_sweeps);
+    if (n <= kBlockSweepMax) {
+      hipLaunchKernelGGL(kc_sweeps_block, dim3(1), dim3(kBlockSweepThreads), 0, s, d, (const float*)L.st,
+                         (const float*)L.idg, (const float*)L.v, sweeps, o.omega, cur);
+      return cur;
+    }
     hipLaunchKernelGGL(kc_init, dim3(std::min(nb, 1024)), dim3(256), 0, s, n, (const float*)L.idg,
                        (const float*)L.v, o.omega, cur);
     for (int k = 1; k < sweeps; ++k) {

@@ -28,6 +28,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // LDS of a convolution block's weight tile: 16 NT rows x (cin + 4); the
 // launches keep 16 NT cin <= 8192 (checked on the host)
 constexpr int kConvWsh = 8192 + 64 * 4;
+// grids of at most this many pixels (all images) take k_conv3x3_small
+constexpr long kSmallConvPix = 32768;
 typedef unsigned short bf16;
 
 namespace {
@@ -174,6 +176,86 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
         if (out) out[o] = f2bf(v);
         if (out_f32 && co == 0) out_f32[p] = v;
       }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution on small grids (forward, one 16-channel output tile per
+// block).  A small launch costs its chain of dependent memory latencies, so
+// all nine taps' weights of the tile are staged in LDS at once (one barrier
+// instead of two per tap), and tap t+1's input chunks are loaded while tap
+// t's MFMAs run.  Per output the MFMAs are issued in k_conv3x3's order (tap
+// by tap, input chunks in order), so the sums are bitwise those of k_conv3x3.
+// ---------------------------------------------------------------------------
+template <int CIN>
+__global__ __launch_bounds__(256) void k_conv3x3_small(const bf16* __restrict__ in, int in_ctot, int in_coff,
+                                                       const bf16* __restrict__ wg, const float* __restrict__ bias,
+                                                       int act, bf16* out, int out_ctot, int out_coff, int cout_valid,
+                                                       bf16* pre, int pre_ctot, float* out_f32, int nimg, int h,
+                                                       int w) {
+  constexpr int NCI = CIN / 16;
+  constexpr int WST = CIN + 4;   // rows padded by 4 bf16 against bank conflicts
+  __shared__ bf16 wsh[9 * 16 * WST];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long npix = (long)nimg * h * w;
+  const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
+  const int kg = lane >> 4;
+  const long pa = p0 + (lane & 15);
+  const bool va = pa < npix;
+  const int xa = (int)(pa % w);
+  const long t1 = pa / w;
+  const int ya = (int)(t1 % h);
+  const long ba = t1 / h;
+  const int t0 = blockIdx.y;
+  const int nout = gridDim.y * 16;
+  // the tile's weights of all nine taps, 16-byte global loads
+  constexpr int NQ = 9 * 16 * (CIN / 8);
+  for (int q = threadIdx.x; q < NQ; q += 256) {
+    const int tap = q / (16 * (CIN / 8)), rem = q - tap * 16 * (CIN / 8);
+    const int row = rem / (CIN / 8), c8 = rem - row * (CIN / 8);
+    const uint4 v = *(const uint4*)(wg + ((long)tap * nout + t0 * 16 + row) * CIN + c8 * 8);
+    bf16* d = wsh + (tap * 16 + row) * WST + c8 * 8;
+    *(uint2*)d = uint2{v.x, v.y};
+    *(uint2*)(d + 4) = uint2{v.z, v.w};
+  }
+  // a tap's input chunks (unconditional loads; zeroed outside the image)
+  auto fetch = [&](int tap, bf16x4 (&a)[NCI]) -> bool {
+    const int ys = ya + tap / 3 - 1, xs = xa + tap % 3 - 1;
+    const bool ok = va && ys >= 0 && ys < h && xs >= 0 && xs < w;
+    const long src = ok ? (((ba * h + ys) * w + xs) * in_ctot + in_coff + 4 * kg) : 0;
+#pragma unroll
+    for (int c = 0; c < NCI; ++c) a[c] = *(const bf16x4*)(in + src + c * 16);
+    return ok;
+  };
+  bf16x4 a[2][NCI];
+  bool ok[2];
+  ok[0] = fetch(0, a[0]);
+  __syncthreads();
+  f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) {
+    const int cur = tap & 1;
+    if (tap < 8) ok[cur ^ 1] = fetch(tap + 1, a[cur ^ 1]);
+    const bf16* wrow = wsh + (tap * 16 + (lane & 15)) * WST + 4 * kg;
+#pragma unroll
+    for (int c = 0; c < NCI; ++c) {
+      const bf16x4 av = ok[cur] ? a[cur][c] : bf16x4{0, 0, 0, 0};
+      const bf16x4 b = *(const bf16x4*)(wrow + c * 16);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b, acc, 0, 0, 0);
+    }
+  }
+  const int co = t0 * 16 + (lane & 15);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const long p = p0 + 4 * kg + e;
+    if (p < npix && co < cout_valid) {
+      float v = acc[e] + (bias ? bias[co] : 0.f);
+      const long o = p * out_ctot + out_coff + co;
+      if (pre) pre[p * pre_ctot + co] = f2bf(v);
+      if (act) v = gelu(v);
+      if (out) out[o] = f2bf(v);
+      if (out_f32 && co == 0) out_f32[p] = v;
     }
   }
 }
@@ -769,6 +851,22 @@ void conv(pb_nn* nn, Acts* a, int l, int level, const bf16* in, int in_ct, int i
   // loads and MFMAs in flight per wave, half the blocks).
   const bool large = npix >= 65536;
   const long bx = (npix + 63) / 64;
+  if (npix <= kSmallConvPix) {   // one output tile per block, all taps' weights staged at once
+    const dim3 gs_((unsigned)bx, nt), bs_(256);
+#define PB_CONVS(CI)                                                                                         \
+  hipLaunchKernelGGL((k_conv3x3_small<CI>), gs_, bs_, 0, s, in, in_ct, in_co, nn->wf[l], bias, L.act, out, out_ct, \
+                     out_co, L.cout_v, pre, pad16(L.cout_v), out_f32, a->nimg, h, w)
+    switch (cin) {
+      case 16: PB_CONVS(16); break;
+      case 32: PB_CONVS(32); break;
+      case 48: PB_CONVS(48); break;
+      case 64: PB_CONVS(64); break;
+      case 96: PB_CONVS(96); break;
+      default: PB_CONVS(128); break;
+    }
+#undef PB_CONVS
+    return;
+  }
   int ntb = nt;
   if (!large && nt > 1) {
     ntb = 1;

@@ -66,6 +66,20 @@ __device__ __forceinline__ int wrap3(int i, int n, int per) {
   return i;
 }
 
+// Tile of hardware block h in a march launch of gx * gy * gz blocks, in the
+// XCD-aware order: block h runs on XCD h mod 8, and each XCD takes a
+// contiguous band of (x tile, y tile, z chunk) tiles so that halo planes and
+// rows it shares stay in its L2.  Bijective on [0, gx gy gz).
+struct MarchTile {
+  int bx, by, bz;
+};
+__host__ __device__ inline MarchTile march_tile(int h, int gx, int gy, int gz) {
+  const int nbk = gx * gy * gz, xx = h % 8, kk8 = h / 8, base = nbk / 8, rem = nbk % 8;
+  const int lin = xx < rem ? xx * (base + 1) + kk8 : rem * (base + 1) + (xx - rem) * base + kk8;
+  const int byz = lin / gx;
+  return MarchTile{lin % gx, byz % gy, byz / gy};
+}
+
 __device__ __forceinline__ int tid3() { return threadIdx.x + threadIdx.y * X3; }
 __device__ __forceinline__ int bid3() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
 __device__ __forceinline__ int nblk3() { return gridDim.x * gridDim.y * gridDim.z; }
@@ -104,13 +118,9 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
   __shared__ T P[3][Y3 + 2][X3 + 2];
   __shared__ T F[3][Y3 + 1][X3 + 1];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tid3();
-  // XCD-aware block order: hardware block h runs on XCD h mod 8; each XCD
-  // takes a contiguous band of (x, y, z-chunk) tiles so shared halo planes
-  // and rows stay in its L2
-  const int nbk = nblk3(), h = bid3(), xx = h % 8, kk8 = h / 8, base = nbk / 8, rem = nbk % 8;
-  const int lin = xx < rem ? xx * (base + 1) + kk8 : rem * (base + 1) + (xx - rem) * base + kk8;
-  const int bxx = lin % gridDim.x, byz = lin / gridDim.x, byy = byz % gridDim.y, bzz = byz / gridDim.y;
-  const int x0 = bxx * X3, y0 = byy * Y3;
+  const MarchTile mt = march_tile(bid3(), gridDim.x, gridDim.y, gridDim.z);
+  const int bzz = mt.bz;
+  const int x0 = mt.bx * X3, y0 = mt.by * Y3;
   const int kb = bzz * ZC, ke = min(kb + ZC, o.nz);
   // A block whose chunk starts past the last plane (a grid with more z
   // blocks than ceil(nz / ZC)) loads nothing; its caller's sums stay zero.
@@ -157,7 +167,7 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
     store(slot, r);
   };
   auto fetch_cells = [&](int k, T (&hw)[CPT]) {   // weights of the cell plane between node planes k, k+1
-    const long ck = k + 1;
+    const long ck = min(k + 1, o.nz);   // cell planes 0..nz; clamped like the node planes
 #pragma unroll
     for (int e = 0; e < CPT; ++e) {
       const int q = tid + e * NT3;
@@ -183,7 +193,7 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
         T hh = hw[e];
         if (PF == 0) {
           const int cy = y0 + cj, cx = x0 + ci;
-          hh = (cy <= ny && cx <= nx) ? o.h[((long)(k + 1) * (ny + 1) + cy) * (nx + 1) + cx] : T(0);
+          hh = (cy <= ny && cx <= nx) ? o.h[((long)min(k + 1, o.nz) * (ny + 1) + cy) * (nx + 1) + cx] : T(0);
         }
         const T a00 = P[s0][cj][ci], a01 = P[s0][cj][ci + 1], a10 = P[s0][cj + 1][ci], a11 = P[s0][cj + 1][ci + 1];
         const T b00 = P[s1][cj][ci], b01 = P[s1][cj][ci + 1], b10 = P[s1][cj + 1][ci], b11 = P[s1][cj + 1][ci + 1];
@@ -746,6 +756,16 @@ int march_blocks(const pb_op3d* op, int zc = Z3) {
   const dim3 gr = march_grid(op, zc);
   return (int)(gr.x * gr.y * gr.z);
 }
+// A march launch must have exactly the tiles of its kernel's chunk of zc
+// planes: fewer z blocks would leave planes out, more would start chunks
+// past the last plane (whose loads the march does not guard).
+int march_check(const pb_op3d* op, const dim3& grd, int zc) {
+  const dim3 want = march_grid(op, zc);
+  if (grd.x != want.x || grd.y != want.y || grd.z != want.z)
+    return pb_fail(op->ctx, PB_ERR_ARG, "3D march: grid (%u, %u, %u) does not tile %d x %d x %d nodes in %d-plane chunks",
+                   grd.x, grd.y, grd.z, op->g.nx, op->g.ny, op->g.nz, zc);
+  return PB_OK;
+}
 dim3 elem_grid3(long n) { return dim3((unsigned)std::min<long>((n + 255) / 256, 4096)); }
 const dim3 kBlk3(X3, Y3);
 
@@ -820,6 +840,8 @@ int krylov3(pb_op3d* op, const T* b, double tol, int maxiter, int legacy, int ch
   const int nbm = march_blocks(op);   // launch 2's blocks (five-dot records)
   const dim3 mg1 = march_grid(op, kZ1<T>);
   const int nb1 = march_blocks(op, kZ1<T>);   // launch 1's blocks (r~.v records)
+  if (int rc = march_check(op, mg, Z3)) return rc;       // launch 2 marches Z3-plane chunks
+  if (int rc = march_check(op, mg1, kZ1<T>)) return rc;  // launch 1 marches kZ1<T>-plane chunks
   const dim3 eg = elem_grid3(n);
   T* t = (T*)op->t[P];
   T* rt = (T*)op->rt[P];
@@ -911,6 +933,7 @@ int info3(const KS3& h) {
 // ||b - A x|| in FP64 into op->r64 (and op->b32 as FP32)
 int resid3(pb_op3d* op, const double* x, const double* b, hipStream_t s, double* nrm) {
   const int nbm = march_blocks(op);
+  if (int rc = march_check(op, march_grid(op), Z3)) return rc;
   hipLaunchKernelGGL(k3_resid, march_grid(op), kBlk3, 0, s, op3<double>(op), x, b, op->r64, (float*)op->b32,
                      op->part);
   PB_LAUNCH_CHECK(op->ctx);
@@ -982,10 +1005,26 @@ extern "C" int pb_op3d_assemble(pb_op3d* op, const pb_phys* phys, double dt, con
   return PB_OK;
 }
 
+// The tile (x tile, y tile, z chunk) every hardware block of a march launch
+// of gx * gy * gz blocks processes: out[3 h .. 3 h + 2].  Host only (no GPU
+// call); it runs the kernels' own mapping, for checking it.
+extern "C" int pb_op3d_march_map(int gx, int gy, int gz, int* out) {
+  if (gx < 1 || gy < 1 || gz < 1 || !out) return PB_ERR_ARG;
+  const int nb = gx * gy * gz;
+  for (int h = 0; h < nb; ++h) {
+    const MarchTile t = march_tile(h, gx, gy, gz);
+    out[3 * h] = t.bx;
+    out[3 * h + 1] = t.by;
+    out[3 * h + 2] = t.bz;
+  }
+  return PB_OK;
+}
+
 extern "C" int pb_op3d_matvec(pb_op3d* op, const void* x, void* y, int precision) {
   if (!op || !x || !y) return pb_fail(op ? op->ctx : nullptr, PB_ERR_ARG, "pb_op3d_matvec: null argument");
   if (!op->assembled) return pb_fail(op->ctx, PB_ERR_STATE, "pb_op3d_matvec: operator not assembled");
   hipStream_t s = op->ctx->stream;
+  if (int rc = march_check(op, march_grid(op), Z3)) return rc;
   if (precision == PB_F64)
     hipLaunchKernelGGL(k3_matvec<double>, march_grid(op), kBlk3, 0, s, op3<double>(op), (const double*)x, (double*)y);
   else
@@ -1089,6 +1128,9 @@ extern "C" int pb_op3d_time_kernel(pb_op3d* op, int kernel, int precision, int r
   const long n = op->n;
   const dim3 eg = elem_grid3(n), mg = march_grid(op);
   const int P = precision == PB_F64 ? 0 : 1;
+  if ((rc = march_check(op, mg, Z3))) return rc;
+  if ((rc = march_check(op, march_grid(op, P == 0 ? kZ1<double> : kZ1<float>), P == 0 ? kZ1<double> : kZ1<float>)))
+    return rc;
   // vectors r, p, v, t, rt, x of both sets hold a constant pattern; the state
   // carries small fixed scalars so repeated launches stay finite
   if (P == 0) {

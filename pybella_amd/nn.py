@@ -19,8 +19,8 @@ from . import _abi
 
 
 def default_weights_path() -> str:
-    """The committed weights of the network trained by pybella_amd.nn_train."""
-    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "precond_513x257.npz")
+    """The committed weights of the coarse solver trained by pybella_amd.nn_train."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "coarse_65x33.npz")
 
 
 class Network:

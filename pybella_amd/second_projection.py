@@ -31,12 +31,13 @@ total_calls = 0
 # Solver settings used by the drop-in.  mode: "f64" (reference semantics:
 # FP64 BiCGStab, x0 = 0, legacy tolerance), "f64fast" (two launches per
 # iteration), "mixed" (FP32 Krylov in FP64 iterative refinement, stop on the
-# FP64 true residual), "nn" (the learned preconditioner attached to the
-# grid's operator with operator_for(node, ud).attach_nn(net): nn_budget
-# flexible-GMRES iterations after the network's initial guess and nn_sweeps
-# damped Jacobi sweeps, then plain BiCGStab for up to ud.max_iterations).
+# FP64 true residual), "nn" (the learned preconditioner: the network attached
+# to the grid's operator with operator_for(node, ud).attach_nn(net) solves the
+# coarse level of the multigrid cycle that preconditions flexible GMRES, for
+# at most nn_budget iterations with nn_sweeps Jacobi sweeps each way per
+# level; then plain BiCGStab for up to ud.max_iterations).
 solver_options = {"mode": "f64", "inner_tol": 1e-3, "max_outer": 12, "check_every": 32,
-                  "nn_budget": 0, "nn_sweeps": 2}
+                  "nn_budget": 300, "nn_sweeps": 2}
 
 _MODES = {"f64": _abi.PB_SOLVE_F64, "f64fast": _abi.PB_SOLVE_F64_FAST, "mixed": _abi.PB_SOLVE_MIXED,
           "nn": _abi.PB_SOLVE_NN}
@@ -240,6 +241,17 @@ class Operator2D:
         self.ctx.check(self.ctx.lib.pb_op2d_time_kernel(self.h, int(kernel), prec, int(reps), C.byref(us)),
                        "pb_op2d_time_kernel")
         return us.value
+
+    def time_in_solve(self, b, kernel, tol=1e-8):
+        """Average microseconds of the fast iteration's A (1) or B (2) launch
+        inside a real FP64 solve of b; returns (avg_us, launches timed, x)."""
+        torch = _torch()
+        x = torch.empty_like(b)
+        us, n = C.c_double(), C.c_int()
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.lib.pb_op2d_time_in_solve(self.h, _abi.ptr(b), _abi.ptr(x), float(tol), int(kernel),
+                                                          C.byref(us), C.byref(n)), "pb_op2d_time_in_solve")
+        return us.value, n.value, x
 
     def close(self):
         if getattr(self, "h", None):

@@ -8,6 +8,7 @@ import subprocess
 import sys
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -79,3 +80,26 @@ def test_bench_refuses_several_gpus():
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+@pytest.mark.parametrize("zc", [12, 16, 24])
+def test_march_blocks_cover_planes_once(lib, zc):
+    """The 3D march's block -> (x tile, y tile, z chunk) mapping, run on the
+    host from the library: for 257 x 129 x 129 nodes in zc-plane chunks
+    (grid z = ceil(129 / zc)) it is a bijection onto the tiles, every chunk
+    starts inside the grid, and the chunks of each (x, y) tile cover planes
+    0..128 exactly once."""
+    nx, ny, nz = 257, 129, 129
+    gx, gy, gz = -(-nx // 64), -(-ny // 8), -(-nz // zc)
+    nb = gx * gy * gz
+    out = (C.c_int * (3 * nb))()
+    assert lib.pb_op3d_march_map(gx, gy, gz, out) == 0
+    tiles = np.array(out[:]).reshape(nb, 3)
+    assert len({tuple(t) for t in tiles}) == nb
+    assert tiles[:, 0].max() == gx - 1 and tiles[:, 1].max() == gy - 1 and tiles[:, 2].max() == gz - 1
+    assert np.all(tiles[:, 2] * zc < nz)
+    cover = np.zeros((gx, gy, nz), dtype=int)
+    for bx, by, bz in tiles:
+        kb = bz * zc
+        cover[bx, by, kb:min(kb + zc, nz)] += 1
+    assert np.all(cover == 1)

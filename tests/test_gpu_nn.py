@@ -127,35 +127,163 @@ def _trained():
     return net
 
 
-@pytest.mark.parametrize("budget,sweeps", [(0, 2), (20, 1), (40, 0)])
+@pytest.mark.parametrize("budget,sweeps", [(0, 2), (200, 1), (200, 2)])
 def test_nn_solve_reaches_tolerance(budget, sweeps):
-    """The learned-preconditioned solve (network guess, flexible GMRES budget,
-    plain BiCGStab after it) reaches true relative residual 1e-8."""
+    """The learned mode (FGMRES preconditioned by the multigrid cycle whose
+    coarse level the network solves, plain BiCGStab after the budget)
+    reaches true relative residual 1e-8 and the plain solution, modulo the
+    null space {1, checkerboard}; a zero budget is the plain fallback alone."""
     print(f"case nn solve 257x129 budget {budget} sweeps {sweeps}", flush=True)
     op, b = _stratified_system(257, 129)
     net = _trained()
     op.attach_nn(net)
     x, info = op.solve(b, tol=1e-8, mode="nn", maxiter=budget, sweeps=sweeps, fallback_maxiter=6000)
     assert info["info"] == 0 and info["true_rel_resid"] <= 1e-8, info
-    # the same solution as plain BiCGStab, modulo the null space {1, checkerboard}
+    assert info["fallback"] == (1 if budget == 0 else 0), info
     xp, infop = op.solve(b, tol=1e-12, mode="f64", maxiter=6000)
     d = (x - xp).cpu().numpy()
     d = onn.project_null(d[None])[0]
     assert np.linalg.norm(d) <= 1e-5 * np.linalg.norm(onn.project_null(xp.cpu().numpy()[None])[0])
 
 
-def test_network_gain_over_jacobi_only():
-    """A/B: the same flexible GMRES (60 iterations, no fallback), with the
-    trained network + one Jacobi sweep vs one Jacobi sweep alone; the network
-    must lower the true residual reached by at least 5x."""
-    print("case nn A/B 513x257", flush=True)
+def test_network_gain_over_the_cycle_without_it():
+    """A/B on the metric's grid: the same FGMRES and multigrid cycle with the
+    network as the coarse solver, and with the network's level smoothed by
+    Jacobi sweeps instead (the method without the network).  The network
+    must cut the iterations at least 2x."""
+    print("case nn A/B 1025x513 network vs no network", flush=True)
+    op, b = _stratified_system(1025, 513)
+    net = _trained()
+    op.attach_nn(net)
+    _, with_nn = op.solve(b, tol=1e-8, mode="nn", maxiter=400, fallback_maxiter=1)
+    _, without = op.solve(b, tol=1e-8, mode="nn", maxiter=400, fallback_maxiter=1, use_network=False)
+    print("with network", with_nn, "without", without, flush=True)
+    assert with_nn["info"] == 0 and without["info"] == 0
+    assert 2 * with_nn["iters"] <= without["iters"], (with_nn["iters"], without["iters"])
+
+
+def test_learned_mode_beats_plain_bicgstab():
+    """A/B at 513 x 257: the learned mode takes fewer iterations and less
+    wall time than plain FP64 BiCGStab (two-launch iteration) to the same
+    true residual 1e-8 (median of 5 timed solves each, after one warm-up)."""
+    import time
+    import torch
+    print("case nn vs f64fast 513x257", flush=True)
     op, b = _stratified_system(513, 257)
     net = _trained()
     op.attach_nn(net)
-    _, with_nn = op.solve(b, tol=1e-8, mode="nn", maxiter=60, sweeps=1, fallback_maxiter=1)
-    _, without = op.solve(b, tol=1e-8, mode="nn", maxiter=60, sweeps=1, fallback_maxiter=1, use_network=False)
-    print("with network", with_nn, "without", without, flush=True)
-    assert with_nn["true_rel_resid"] * 5 < without["true_rel_resid"]
+
+    def timed(**kw):
+        op.solve(b, tol=1e-8, **kw)
+        ts, info = [], None
+        for _ in range(5):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, info = op.solve(b, tol=1e-8, **kw)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), info
+
+    t_nn, i_nn = timed(mode="nn", maxiter=400, fallback_maxiter=6000)
+    t_pl, i_pl = timed(mode="f64fast", maxiter=6000)
+    print(f"nn {t_nn * 1e3:.2f} ms {i_nn['iters']} it; f64fast {t_pl * 1e3:.2f} ms {i_pl['iters']} it", flush=True)
+    assert i_nn["info"] == 0 and i_pl["info"] == 0 and i_nn["fallback"] == 0
+    assert i_nn["iters"] < i_pl["iters"] and t_nn < t_pl
+
+
+def test_destroyed_network_detaches():
+    """Destroying an attached network detaches it from the operator: the next
+    learned-mode solve reports PB_ERR_STATE instead of reading freed memory."""
+    from pybella_amd import _abi
+    print("case nn destroy while attached", flush=True)
+    op, b = _stratified_system(129, 65)
+    net = _net(2)
+    op.attach_nn(net)
+    net.close()
+    op._net = None
+    with pytest.raises(_abi.HipError, match=r"no network attached.*code -3"):
+        op.solve(b, tol=1e-8, mode="nn", maxiter=50, fallback_maxiter=1)
+    x, info = op.solve(b, tol=1e-8, mode="nn", maxiter=100, fallback_maxiter=1, use_network=False)
+    assert info["info"] == 0
+
+
+def test_cycle_with_network_matches_oracle():
+    """One application of the cycle whose coarse level the network solves,
+    against the multigrid oracle with the NumPy network (bf16 rounding at the
+    GPU's storage points) as the coarse solve: the duplicated-grid hand-over,
+    the features, the least-squares scale and the two sweeps after it.
+    Relative L2 <= 2e-2 (bf16 storage)."""
+    import copy
+    import torch
+    from oracle import laplacian as olp
+    from oracle import mg
+    from oracle import projection as opj
+    from pybella_amd import workloads as W
+    print("case nn cycle vs oracle 257x129", flush=True)
+    op, b = _stratified_system(257, 129)
+    net = _trained()
+    op.attach_nn(net)
+    Sol, mpv, elem, node, ud, th = W.stratified_state(257, 129)
+    S2, m2 = copy.deepcopy(Sol), copy.deepcopy(mpv)
+    opj.operator_coefficients_nodes(elem, node, S2, m2, ud, th, 0.05)
+    m2.wcenter[...] *= ud.compressibility
+    cor = opj.multiply_inverse_coriolis(m2, S2, m2, ud, elem, node, 0.05, attrs=("u", "v", "w"), get_coeffs=True)
+    op_o = olp.op2d_arrays(m2, node, cor, olp.precon_diag_prepare(m2, elem, node, ud), ud)
+    levels, nnl = op.mg_info()
+    L = [mg.expand9(op_o)]
+    while len(L) <= nnl:
+        L.append(mg.galerkin(L[-1]))
+    P = net.get_params().astype(np.float64)
+    lv = L[nnl]
+
+    def net_solve(v):   # v (2, ny, nx) on the unique grid -> scaled, smoothed correction
+        vd = mg.to_dup(v[0], lv.py, lv.px), mg.to_dup(v[1], lv.py, lv.px)
+        C = lv.st[4]
+        f1 = mg.to_dup(-(lv.st[3] + lv.st[5]) / C, lv.py, lv.px)
+        f2 = mg.to_dup(-(lv.st[1] + lv.st[7]) / C, lv.py, lv.px)
+        X = np.zeros((2,) + vd[0].shape + (3,))
+        sc = []
+        for c in range(2):
+            s = np.sqrt(np.mean(vd[c] ** 2))
+            sc.append(s)
+            X[c, ..., 0], X[c, ..., 1], X[c, ..., 2] = vd[c] / s, f1, f2
+        z, _ = onn.forward(P, X, bf16=True)
+        out = []
+        for c in range(2):
+            zd = z[c] * sc[c]
+            zu = zd[:, :-1].copy()
+            zu[:, 0] = 0.5 * (zd[:, 0] + zd[:, -1])
+            az = mg.apply(lv, zu)
+            zu *= np.sum(zu * v[c]) / np.sum(zu * az)
+            out.append(mg.jacobi(lv, v[c], zu, 2, 0.8))
+        return out
+
+    rng = np.random.default_rng(5)
+    vin = rng.standard_normal((129, 257))
+    vin[:, -1] = vin[:, 0]
+    z = op.precond(torch.from_numpy(vin).cuda(), coarse="net").double().cpu().numpy()
+    # oracle: the two-branch fine level by hand, both branches down to the network level together
+    lv0 = L[0]
+    v = mg.to_unique(vin, lv0.py, lv0.px)
+    zz = mg.jacobi(lv0, v, None, 2, 0.8)
+    r = v - mg.apply(lv0, zz)
+    c = mg.checkerboard(lv0.ny, lv0.nx)
+    vs = [mg.restrict(lv0, r), mg.restrict(lv0, c * r)]
+
+    def down_up(l, vv):
+        if l == nnl:
+            return net_solve(np.stack(vv))
+        zs = [mg.jacobi(L[l], q, None, 2, 0.8) for q in vv]
+        rs = [q - mg.apply(L[l], zq) for q, zq in zip(vv, zs)]
+        es = down_up(l + 1, [mg.restrict(L[l], rq) for rq in rs])
+        return [mg.jacobi(L[l], q, zq + mg.prolong(L[l], e), 2, 0.8) for q, zq, e in zip(vv, zs, es)]
+
+    es = down_up(1, vs)
+    zz = zz + mg.prolong(lv0, es[0]) + c * mg.prolong(lv0, es[1])
+    ref = mg.to_dup(mg.jacobi(lv0, v, zz, 2, 0.8), lv0.py, lv0.px)
+    err = np.linalg.norm(z - ref) / np.linalg.norm(ref)
+    print("cycle with network vs oracle, rel L2", err, flush=True)
+    assert err <= 2e-2
 
 
 def test_features_follow_reassembly():
@@ -182,10 +310,11 @@ def test_features_follow_reassembly():
 
 def test_dropin_learned_mode():
     """The drop-in projection step with the learned preconditioner attached to
-    the grid's operator (mode "nn": network guess + 2 damped Jacobi sweeps,
-    then plain BiCGStab capped at ud.max_iterations) reaches the tolerance and
-    updates the state like the FP64 mode: momenta and rhoX within 1e-5 of
-    their magnitude (both solves stop at relative residual 1e-8)."""
+    the grid's operator (mode "nn": FGMRES with the network's multigrid cycle,
+    plain BiCGStab capped at ud.max_iterations only past its budget) reaches
+    the tolerance and updates the state like the FP64 mode: momenta and rhoX
+    within 1e-5 of their magnitude (both solves stop at relative residual
+    1e-8)."""
     import copy
     from pybella_amd import second_projection as sp
     from pybella_amd import workloads as W
@@ -199,14 +328,15 @@ def test_dropin_learned_mode():
         sp.solver_options["mode"] = "f64"
         ia = sp.euler_backward_non_advective_impl_part(A[0], A[2], elem, node, ud, th, 0.0, 0.05, 1.0, Sol0=A[1])
         op.attach_nn(_trained())
-        sp.solver_options.update(mode="nn", nn_budget=0, nn_sweeps=2)
+        sp.solver_options.update(mode="nn")
         ib = sp.euler_backward_non_advective_impl_part(B[0], B[2], elem, node, ud, th, 0.0, 0.05, 1.0, Sol0=B[1])
     finally:
         sp.solver_options.clear()
         sp.solver_options.update(saved)
         op.attach_nn(None)
     print("f64", ia, "nn", ib, flush=True)
-    assert ia["info"] == 0 and ib["info"] == 0 and ib["fallback"] == 1, (ia, ib)
+    assert ia["info"] == 0 and ib["info"] == 0 and ib["fallback"] == 0, (ia, ib)
+    assert ib["iters"] < ia["iters"] // 10, (ia, ib)
     assert ib["true_rel_resid"] <= 1e-8
     for k in ("rhou", "rhov", "rhoX"):
         a, r = getattr(B[0], k), getattr(A[0], k)

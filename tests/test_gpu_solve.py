@@ -82,6 +82,40 @@ def test_converged_f64(nx, ny):
     tr = obs.true_rel_resid(lambda v: olp.lap2d(v, op_o), b, x.ravel())
     assert tr <= 1.05e-8
     assert abs(info["true_rel_resid"] - tr) <= 1e-3 * tr + 1e-15
+    # the solutions themselves, modulo the null space {1, checkerboard}: two
+    # solves stopped at relative residual 1e-8 lie ~1e-7 from the exact one
+    a, r = strip_null(x), strip_null(ref.x.reshape(ny, nx))
+    assert np.linalg.norm(a - r) / np.linalg.norm(r) <= 2e-6
+
+
+@pytest.mark.parametrize("mode", ["f64", "nn"])
+def test_converged_parity_1025x513(mode):
+    """The metric's grid: the GPU solution (FP64 BiCGStab, or the learned
+    mode with its FP32 / BF16 preconditioner) at tolerance 1e-10 against the
+    C restatement's BiCGStab (oracle/c/pb_oracle.c, serial, FP64) at 1e-10,
+    modulo the null space: relative L2 <= 1e-8."""
+    import torch
+    from oracle import cport
+    from pybella_amd.nn import Network, default_weights_path
+    sp = _sp()
+    print(f"case converged parity 1025x513 {mode}", flush=True)
+    (Sol, mpv, elem, node, ud, th, dt), op_o, rhs, b = system(1025, 513)
+    xo, info_o, it_o, _, _ = cport.COp(op_o).bicgstab(b, tol=1e-10, maxiter=20000)
+    assert obs.true_rel_resid(lambda v: olp.lap2d(v, op_o), b, xo) <= 1.1e-10
+    op = sp.operator_for(node, ud)
+    net = None
+    if mode == "nn":
+        net = Network(0)
+        net.load(default_weights_path())
+        op.attach_nn(net)
+    try:
+        x, info = sp.solve_isolated(Sol, mpv, elem, node, ud, th, dt, rhs, mode=mode, tol=1e-10, maxiter=20000)
+    finally:
+        if net is not None:
+            op.attach_nn(None)
+    assert info["info"] == 0 and info["true_rel_resid"] <= 1e-10, info
+    a, r = strip_null(x), strip_null(xo.reshape(513, 1025))
+    assert np.linalg.norm(a - r) / np.linalg.norm(r) <= 1e-8
 
 
 @pytest.mark.parametrize("nx,ny", [(129, 65), (257, 129)])

@@ -6,19 +6,23 @@ from pybella_amd import workloads as W
 
 
 def test_device_rhs_is_consistent():
-    from pybella_amd.nn_train import fourier_rhs_dev
+    from pybella_amd.nn_train import fourier_rhs_batch
     g = torch.Generator()
     g.manual_seed(3)
-    f = fourier_rhs_dev(g, 33, 17, nmodes=16, kmax=6, device="cpu").numpy().T   # -> host [x][y]
-    assert f.shape == (35, 19)
-    assert not f[0].any() and not f[-1].any() and not f[:, 0].any() and not f[:, -1].any()
-    inner = f[1:-1, 1:-1]
+    F = fourier_rhs_batch(g, 3, 33, 17, nmodes=16, kmax=6, device="cpu").numpy()
+    assert F.shape == (3, 19, 35)
     Wn = W.left_null_vectors(33, 17)
-    for v in Wn:
-        assert abs((v * inner).sum()) < 1e-10 * np.abs(inner).sum()
-    # the periodic end columns of each mode coincide (integer kx) before projection,
-    # so they coincide after it as well
-    assert np.allclose(inner[0], inner[-1], atol=1e-12)
+    for fb in F:
+        f = fb.T   # -> host [x][y]
+        assert not f[0].any() and not f[-1].any() and not f[:, 0].any() and not f[:, -1].any()
+        inner = f[1:-1, 1:-1]
+        assert np.abs(inner).max() > 0.1
+        for v in Wn:
+            assert abs((v * inner).sum()) < 1e-10 * np.abs(inner).sum()
+        # the periodic end columns of each mode coincide (integer kx) before
+        # projection, so they coincide after it as well
+        assert np.allclose(inner[0], inner[-1], atol=1e-12)
+    assert not np.allclose(F[0], F[1])
 
 
 def test_numpy_rhs_is_consistent():
