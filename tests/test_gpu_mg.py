@@ -48,11 +48,6 @@ def case(kind, nx, ny):
     return Sol, mpv, elem, node, ud, th
 
 
-def walls_case(nx, ny):
-    Sol, mpv, elem, node, ud, th = make_case("xwall", nx, ny)[:6]
-    return Sol, mpv, elem, node, ud, th
-
-
 @pytest.mark.parametrize("kind,nx,ny", [("strat", 257, 129), ("coriolis", 129, 65), ("xwall", 65, 33),
                                         ("strat", 1025, 513)])
 def test_hierarchy_stencils(kind, nx, ny):
@@ -69,8 +64,8 @@ def test_hierarchy_stencils(kind, nx, ny):
         got = op.mg_stencil(l).cpu().numpy()
         scale = np.max(np.abs(lv.st))
         assert np.max(np.abs(got - lv.st)) <= 1e-12 * scale, (l, np.max(np.abs(got - lv.st)) / scale)
-    if nx == 1025:
-        assert nnl == 3 and levels[3][:2] == (65, 128)
+    if nx == 1025:   # the network's level: 33 x 64 unique nodes, 33 x 65 with the periodic column
+        assert nnl == 4 and levels[4][:2] == (33, 64)
     op.close()
 
 

@@ -1,12 +1,13 @@
 """GPU parity of the learned-preconditioner network against the NumPy oracle.
 
 * forward: GPU (bf16 storage, FP32 accumulation) vs the oracle run in float64
-  with bf16 rounding at the same storage points: relative L2 <= 2e-2 (the
-  remaining difference is accumulation order flipping bf16 roundings);
+  with bf16 rounding at the same storage points: relative L2 <= 5e-3
+  (measured <= 9e-4; the remaining difference is accumulation order flipping
+  bf16 roundings);
 * gradients: GPU parameter gradients (bf16 gradient tensors) vs the oracle's
-  float64 backward on the same rounded forward: per-layer cosine >= 0.99 and
-  relative L2 <= 0.1; the oracle's backward is itself pinned by finite
-  differences in the CPU tests.
+  float64 backward on the same rounded forward: per-layer cosine >= 0.9999
+  and relative L2 <= 0.02 (measured >= 0.99996 and <= 0.009); the oracle's
+  backward is itself pinned by finite differences in the CPU tests.
 """
 import numpy as np
 import pytest
@@ -41,7 +42,7 @@ def test_forward_matches_oracle(shape):
     zr, _ = onn.forward(P, X.astype(np.float64), bf16=True)
     err = np.linalg.norm(z - zr) / np.linalg.norm(zr)
     print("rel L2", err)
-    assert err < 2e-2
+    assert err < 5e-3
 
 
 def test_gradients_match_oracle():
@@ -69,7 +70,7 @@ def test_gradients_match_oracle():
     for l, part, cos, rel in stats:
         if l == 14 and part == "b":
             continue   # sum of a projected gradient: exactly zero in exact arithmetic
-        assert cos > 0.99 and rel < 0.1, (l, part, cos, rel)
+        assert cos >= 0.9999 and rel <= 0.02, (l, part, cos, rel)
 
 
 def test_training_reduces_loss():
@@ -313,8 +314,9 @@ def test_dropin_learned_mode():
     the grid's operator (mode "nn": FGMRES with the network's multigrid cycle,
     plain BiCGStab capped at ud.max_iterations only past its budget) reaches
     the tolerance and updates the state like the FP64 mode: momenta and rhoX
-    within 1e-5 of their magnitude (both solves stop at relative residual
-    1e-8)."""
+    within 5e-5 of their magnitude (both solves stop at relative residual
+    1e-8, which leaves the pressures ~1e-7 apart and their gradient, the
+    momentum correction, ~1e-5 apart in max norm)."""
     import copy
     from pybella_amd import second_projection as sp
     from pybella_amd import workloads as W
@@ -340,4 +342,4 @@ def test_dropin_learned_mode():
     assert ib["true_rel_resid"] <= 1e-8
     for k in ("rhou", "rhov", "rhoX"):
         a, r = getattr(B[0], k), getattr(A[0], k)
-        assert np.max(np.abs(a - r)) / np.max(np.abs(r)) < 1e-5, k
+        assert np.max(np.abs(a - r)) / np.max(np.abs(r)) < 5e-5, k
