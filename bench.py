@@ -462,6 +462,38 @@ def bench_batch(args, sizes=((129, 65), (257, 129), (513, 257), (1025, 513))):
     print(json.dumps(out), flush=True)
 
 
+def only_run(args, op, phys, dt, rho, rhoY, dS, rhs_d, b, x):
+    """Warm-up and K timed solves of one mode (assembly + scaling + solve per
+    step, as in the default line), for profiler runs."""
+    import torch
+    cfg = dict(mode="f64fast", maxiter=6000)
+    if args.only in ("nn", "mg"):
+        from pybella_amd.nn import Network, default_weights_path
+        net = Network(seed=0)
+        net.load(default_weights_path())
+        op.attach_nn(net)
+        cfg = dict(mode="nn", maxiter=2000, sweeps=2, krylov_m=30, fallback_maxiter=6000,
+                   use_network=True if args.only == "nn" else "mg")
+    else:
+        op.set_tile(5)
+
+    def step():
+        op.assemble(phys, dt, rho, rhoY, rho, rhoY, dS)
+        op.scale_rhs(rhs_d, b)
+        return op.solve(b, x, tol=1e-8, **cfg)[1]
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"only": args.only, "solves_per_s": round(args.steps / el, 3),
+                      "ms_per_solve": round(1e3 * el / args.steps, 3), "iterations": infos[-1]["iters"],
+                      "true_rel_resid": infos[-1]["true_rel_resid"]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -472,6 +504,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle checks and the CPU baseline")
     ap.add_argument("--no-extra-workloads", action="store_true",
                     help="skip the 3D and batch summaries of the default line")
+    ap.add_argument("--only", default=None, choices=["nn", "f64fast", "mg"],
+                    help="time only this solve of the 2D workload (no calibration; for profiler runs): the "
+                         "learned mode, plain BiCGStab, or the cycle continued to the coarsest level")
     ap.add_argument("--cpu-sample-iters", type=int, default=300)
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--kernel-only", type=int, default=None,
@@ -543,6 +578,10 @@ def main():
         print(json.dumps({"kernel": KERNEL_NAMES[args.kernel_only], "precision": args.precision,
                           "tile_rows_per_thread": args.tile, "avg_us": us, "alg_bytes": alg,
                           "alg_GBps": alg / (us * 1e-6) / 1e9}))
+        return
+
+    if args.only is not None:
+        only_run(args, op, phys, dt, rho, rhoY, dS, rhs_d, b, x)
         return
 
     nn_net = None
