@@ -80,6 +80,7 @@ typedef struct pb_phys {
   int is_compressible;     /* ud.is_compressible                         */
   double compressibility;  /* ud.compressibility                         */
   int is_arakawa_konor;    /* ud.is_ArakawaKonor                         */
+  double wind[3];          /* ud.u/v/w_wind_speed (time-step kernels)     */
 } pb_phys;
 
 /* 3D node grid: nx * ny * nz inner nodes (y the gravity axis), spacings,
@@ -135,6 +136,12 @@ const char* pb_last_error(const pb_ctx* ctx);
  * background column mpv.HydroState.rhoY0 (ny+3), used when !is_compressible. */
 int pb_bc_cells(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys,
                 double* const fields[6], const double* rhoY0_bg);
+
+/* the same fill along one axis only (axis 0: x, 1: y; -1: both, as
+ * pb_bc_cells), as the advection's split steps call it (boundary.py:30-39).
+ * Without gravity (phys->g == 0) y is filled like x: wrap or mirror. */
+int pb_bc_cells_axis(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys,
+                     double* const fields[6], const double* rhoY0_bg, int axis);
 
 /* node ghost fill of a (ny+4)x(nx+4) node array (periodic_plus_one / reflect) */
 int pb_ghostnodes(pb_ctx* ctx, const pb_grid2d* grid, double* p);
@@ -225,6 +232,65 @@ int pb_correction_nodes(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys,
  * p2_full (nodes) receives the increment. */
 int pb_scatter_update_p2(pb_ctx* ctx, const pb_grid2d* grid, const double* x,
                          double* p2_full, double* p2_nodes);
+
+/* ---- explicit parts of the time step ----
+ * Face-flux arrays: x faces (ny+3) x (nx+4) in [y][face] (the reference's
+ * flux[0] as it lies), y faces (ny+4) x (nx+3) in [face][x] (the transpose of
+ * the reference's flux[1]).  flux[6] = rho, rhou, rhov, rhow, rhoY, rhoX face
+ * fluxes, rhoY being the advective flux.  Faces 0 and n of a line are never
+ * written: the caller zeroes them once.  cells[6] = rho, rhou, rhov, rhow,
+ * rhoY, rhoX as for pb_bc_cells.  All asynchronous on the context stream
+ * except pb_cfl_maxima. */
+
+/* rhoY-weighted advective face fluxes into the rhoY entries of both flux
+ * sets (inner faces and lines only).  Replaces recompute_advective_fluxes,
+ * numerical_flux.py:5-48. */
+int pb_advective_fluxes(pb_ctx* ctx, const pb_grid2d* grid, const double* rho, const double* rhou,
+                        const double* rhov, const double* rhoY, double* fx_rhoY, double* fy_rhoY);
+
+/* One advection split step along axis (0 x, 1 y): ghost cells along the
+ * axis, the second-order recovery of face states (limiter NONE) and the
+ * upwind (HLL) fluxes of rho, rhou, rhov, rhow, rhoX, then, unless rk, the
+ * conservative update with lmbda = dt/dx, then ghost cells along the axis
+ * again.  rk: the one-stage Runge-Kutta variant (recovery without the
+ * time-centring term, no update).  Replaces explicit_step_and_flux,
+ * explicit.py:75-165, with recovery.do (recovery.py:6-210) and hll_solver
+ * (numerical_flux.py:50-99). */
+int pb_advect_split(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys, int axis, double lmbda,
+                    int rk, double* const cells[6], double* const flux[6], const double* rhoY0_bg);
+
+/* The update loop of advect_rk: q += lx (fx_l - fx_r), then q += ly (fy_l -
+ * fy_r), then every ghost cell (explicit.py:219-237). */
+int pb_advect_rk_update(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys, double lx, double ly,
+                        double* const cells[6], double* const fx[6], double* const fy[6],
+                        const double* rhoY0_bg);
+
+/* Buoyancy, then the inverse Coriolis / buoyancy matrix on the momenta
+ * relative to the wind, then every ghost cell.  Replaces
+ * euler_backward_non_advective_expl_part, second_projection.py:72-90. */
+int pb_expl_part(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys, double dt,
+                 double* const cells[6], const double* dSdy, const double* rhoY0_bg);
+
+/* The explicit forward step: rhs (isc) = nodal divergence (momenta zeroed
+ * next to y walls first), doubled on wall node rows; momenta and rhoX
+ * updated from the old p2_nodes, buoyancy and Coriolis; p2_nodes +=
+ * compressibility * (-dt dpidP rhs); node and cell ghosts.  S0c: the cell
+ * background column (ny+3).  Replaces euler_forward_non_advective,
+ * second_projection.py:22-69. */
+int pb_euler_forward(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys, double dt,
+                     double* const cells[6], double* p2_nodes, double* rhs, const double* S0c,
+                     const double* dSdy, const double* rhoY0_bg);
+
+/* Device work size (doubles) of pb_cfl_maxima for a grid. */
+long pb_cfl_work_words(const pb_grid2d* grid);
+
+/* maxima[6] (host) = max over every cell of |u|, |v|, |w|, |u|+c, |v|+c,
+ * |w|+c with c = sqrt(gamm rhoY^gamm / rho) * Minv, Minv = 1/sqrt(Msq): the
+ * reductions of dynamic_timestep (cfl.py:5-35).  work: pb_cfl_work_words
+ * doubles of device memory.  Synchronous. */
+int pb_cfl_maxima(pb_ctx* ctx, const pb_grid2d* grid, double gamm, double Minv, const double* rho,
+                  const double* rhou, const double* rhov, const double* rhow, const double* rhoY,
+                  double* work, double* maxima);
 
 /* ---- learned preconditioner (no reference counterpart: pyBELLA has none) ----
  * Fixed architecture: 4-level convolutional encoder-decoder, 3x3 convolutions,

@@ -46,12 +46,15 @@ def _set_boundary(Sol, pads, btype, idx):
     Sol.rhoX[...] = np.pad(Sol.rhoX[idx], pads, btype)
 
 
-def set_explicit_boundary_data(Sol, elem, ud, th, mpv):
-    """Cell ghost fill (boundary.py:7-132): wrap/mirror on the axis without
-    gravity, hydrostatic image rows on the gravity axis (y)."""
+def set_explicit_boundary_data(Sol, elem, ud, th, mpv, step=None):
+    """Cell ghost fill (boundary.py:7-132): wrap/mirror on an axis without
+    gravity, hydrostatic image rows on the gravity axis (y).  ``step`` (an
+    axis) restricts the fill to that axis, as the advection's split steps
+    call it (boundary.py:30-39 with the arrays flipped so that the split axis
+    is last; unflipped here, the same axis is filled)."""
     ndim = elem.ndim
     igs = elem.igs
-    for dim in range(ndim):
+    for dim in (range(ndim) if step is None else (step,)):
         pads = [(0, 0)] * ndim
         pads[dim] = (igs[dim], igs[dim])
         pads = tuple(pads)

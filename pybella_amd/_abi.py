@@ -35,7 +35,7 @@ class Phys(C.Structure):
     _fields_ = [("gamm", C.c_double), ("Msq", C.c_double), ("g", C.c_double),
                 ("coriolis", C.c_double * 3), ("nonhydro", C.c_double),
                 ("is_compressible", C.c_int), ("compressibility", C.c_double),
-                ("is_arakawa_konor", C.c_int)]
+                ("is_arakawa_konor", C.c_int), ("wind", C.c_double * 3)]
 
 
 class SolveOpts(C.Structure):
@@ -78,6 +78,14 @@ SIGNATURES = {
     "pb_last_error": (C.c_char_p, [VP]),
     "pb_bc_cells": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), VP * 6, VP]),
     "pb_ghostnodes": (I, [VP, C.POINTER(Grid2D), VP]),
+    "pb_bc_cells_axis": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), VP * 6, VP, I]),
+    "pb_advective_fluxes": (I, [VP, C.POINTER(Grid2D), VP, VP, VP, VP, VP, VP]),
+    "pb_advect_split": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), I, D, I, VP * 6, VP * 6, VP]),
+    "pb_advect_rk_update": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), D, D, VP * 6, VP * 6, VP * 6, VP]),
+    "pb_expl_part": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), D, VP * 6, VP, VP]),
+    "pb_euler_forward": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), D, VP * 6, VP, VP, VP, VP, VP]),
+    "pb_cfl_work_words": (C.c_long, [C.POINTER(Grid2D)]),
+    "pb_cfl_maxima": (I, [VP, C.POINTER(Grid2D), D, D, VP, VP, VP, VP, VP, VP, C.POINTER(D)]),
     "pb_op2d_create": (I, [VP, C.POINTER(Grid2D), C.POINTER(VP)]),
     "pb_op2d_destroy": (I, [VP]),
     "pb_op2d_assemble": (I, [VP, C.POINTER(Phys), D, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),

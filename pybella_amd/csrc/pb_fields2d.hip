@@ -39,6 +39,8 @@ Dims2 dims_of(const pb_grid2d& g) {
 // ---------------------------------------------------------------------------
 struct BcParams {
   int periodic_x;
+  int gravity_y;    // ud.gravity_strength[1] != 0: hydrostatic images in y
+  int periodic_y;   // without gravity: wrap (periodic) or mirror (wall) in y
   int compressible;
   double Gamma_g;   // th.Gamma * g
   double half_dy;   // 0.5 * elem.dy
@@ -92,12 +94,57 @@ __device__ __forceinline__ void x_ghost_row(double* rho, double* rhou, double* r
   rhoX[a] = rhoX[b];
 }
 
-// Threads 0 .. cx-1: column ``col``'s gravity-axis images (rows 0, 1, cy-2,
-// cy-1), after the x-axis fill of the inner rows those images read (2, 3,
-// cy-4, cy-3) when col is an x-ghost column.  Threads cx .. cx + 4 (cy-4) - 1:
-// the x-axis fill of the other inner rows of the 4 ghost columns, one row
-// each.  No thread reads a value another thread writes: the fill reads inner
-// columns, the images read their own column.
+// y ghost row ``row`` (0, 1, cy-2 or cy-1) of column ``col`` on an axis
+// without gravity: copy of the wrapped (periodic) or mirrored (wall) inner row
+// (boundary.py:40-50 with 135-216 along y)
+__device__ __forceinline__ void y_ghost_row(double* rho, double* rhou, double* rhov, double* rhow,
+                                            double* rhoY, double* rhoX, int cx, int cy, int col, int row,
+                                            const BcParams& bp) {
+  int srcr;
+  double vsign = 1.0;
+  bool copy_w = true;
+  if (bp.periodic_y) {
+    srcr = row < 2 ? cy - 4 + row : row - (cy - 4);
+  } else {
+    srcr = row < 2 ? 3 - row : 2 * cy - 5 - row;
+    vsign = -1.0;
+    copy_w = false;
+  }
+  const long a = (long)row * cx + col, b = (long)srcr * cx + col;
+  rho[a] = rho[b];
+  rhou[a] = rhou[b];
+  rhov[a] = vsign * rhov[b];
+  if (copy_w) rhow[a] = rhow[b];
+  rhoY[a] = rhoY[b];
+  rhoX[a] = rhoX[b];
+}
+
+// the y-axis fill of one column: hydrostatic images on the gravity axis
+// (bottom rows 1 then 0, top rows cy-2 then cy-1), else wrap / mirror copies
+__device__ __forceinline__ void y_fill_column(double* rho, double* rhou, double* rhov, double* rhow,
+                                              double* rhoY, double* rhoX, const double* rhoY0, int cx,
+                                              int cy, int col, const BcParams& bp) {
+  if (!bp.gravity_y) {
+    y_ghost_row(rho, rhou, rhov, rhow, rhoY, rhoX, cx, cy, col, 1, bp);
+    y_ghost_row(rho, rhou, rhov, rhow, rhoY, rhoX, cx, cy, col, 0, bp);
+    y_ghost_row(rho, rhou, rhov, rhow, rhoY, rhoX, cx, cy, col, cy - 2, bp);
+    y_ghost_row(rho, rhou, rhov, rhow, rhoY, rhoX, cx, cy, col, cy - 1, bp);
+    return;
+  }
+  // bottom (direction +1): image 1 <- source 2, image 0 <- source 3
+  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, 1, 2, 2, 1.0, bp);
+  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, 0, 1, 3, 1.0, bp);
+  // top (direction -1)
+  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, cy - 2, cy - 3, cy - 3, -1.0, bp);
+  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, cy - 1, cy - 2, cy - 4, -1.0, bp);
+}
+
+// Threads 0 .. cx-1: column ``col``'s y-axis fill (rows 0, 1, cy-2, cy-1),
+// after the x-axis fill of the inner rows it reads (2, 3, cy-4, cy-3) when
+// col is an x-ghost column.  Threads cx .. cx + 4 (cy-4) - 1: the x-axis fill
+// of the other inner rows of the 4 ghost columns, one row each.  No thread
+// reads a value another thread writes: the x fill reads inner columns, the y
+// fill reads its own column.
 __global__ void k_bc_cells(Dims2 d, BcParams bp, double* rho, double* rhou, double* rhov,
                            double* rhow, double* rhoY, double* rhoX, const double* rhoY0) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -117,12 +164,24 @@ __global__ void k_bc_cells(Dims2 d, BcParams bp, double* rho, double* rhou, doub
     for (int k = 0; k < 4; ++k)
       if (rows[k] >= 2 && rows[k] < cy - 2) x_ghost_row(rho, rhou, rhov, rhow, rhoY, rhoX, cx, col, rows[k], bp);
   }
-  // bottom (direction +1): image 1 <- source 2, image 0 <- source 3
-  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, 1, 2, 2, 1.0, bp);
-  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, 0, 1, 3, 1.0, bp);
-  // top (direction -1)
-  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, cy - 2, cy - 3, cy - 3, -1.0, bp);
-  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, cy - 1, cy - 2, cy - 4, -1.0, bp);
+  y_fill_column(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, cy, col, bp);
+}
+
+// The fill along one axis only, as the advection's split steps call it
+// (boundary.py:30-39): axis 0 fills the 4 ghost columns over every row
+// (thread = row * 4 + ghost column), axis 1 the y ghost rows of every column.
+__global__ void k_bc_axis(Dims2 d, BcParams bp, int axis, double* rho, double* rhou, double* rhov,
+                          double* rhow, double* rhoY, double* rhoX, const double* rhoY0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int cx = d.cx, cy = d.cy;
+  if (axis == 0) {
+    if (t >= 4 * cy) return;
+    const int j = t >> 2, g = t & 3;
+    x_ghost_row(rho, rhou, rhov, rhow, rhoY, rhoX, cx, g < 2 ? g : cx - 4 + g, j, bp);
+    return;
+  }
+  if (t >= cx) return;
+  y_fill_column(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, cy, t, bp);
 }
 
 // ---------------------------------------------------------------------------
@@ -271,6 +330,12 @@ __global__ void k_correction(Dims2 d, CorrParams cp, const double* p, const doub
   rhoX[c] += -cp.updt_chi * cp.dt * dSdy[c] * rv_new;
 }
 
+BcParams bc_params(const pb_grid2d& g, const pb_phys& ph) {
+  const double gm1 = ph.gamm - 1.0;
+  return BcParams{g.bc_x == PB_BC_PERIODIC, ph.g != 0.0, g.bc_y == PB_BC_PERIODIC, ph.is_compressible == 1,
+                  (gm1 / ph.gamm) * ph.g, 0.5 * g.dy, gm1, 1.0 / gm1};
+}
+
 }  // namespace
 
 extern "C" int pb_bc_cells(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys,
@@ -279,14 +344,31 @@ extern "C" int pb_bc_cells(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* ph
   if (grid->nx < 3 || grid->ny < 5) return pb_fail(ctx, PB_ERR_ARG, "pb_bc_cells: grid too small");
   for (int k = 0; k < 6; ++k)
     if (!fields[k]) return pb_fail(ctx, PB_ERR_ARG, "pb_bc_cells: field %d is null", k);
-  if (!phys->is_compressible && !rhoY0_bg)
+  if (phys->g != 0.0 && !phys->is_compressible && !rhoY0_bg)
     return pb_fail(ctx, PB_ERR_ARG, "pb_bc_cells: background rhoY0 required when not compressible");
   const Dims2 d = dims_of(*grid);
-  const double gm1 = phys->gamm - 1.0;
-  BcParams bp{grid->bc_x == PB_BC_PERIODIC, phys->is_compressible == 1, (gm1 / phys->gamm) * phys->g,
-              0.5 * grid->dy, gm1, 1.0 / gm1};
+  const BcParams bp = bc_params(*grid, *phys);
   const int nthr = d.cx + 4 * (d.cy - 4);
   hipLaunchKernelGGL(k_bc_cells, dim3(ceil_div(nthr, 256)), dim3(256), 0, ctx->stream, d, bp,
+                     fields[0], fields[1], fields[2], fields[3], fields[4], fields[5], rhoY0_bg);
+  PB_LAUNCH_CHECK(ctx);
+  return PB_OK;
+}
+
+extern "C" int pb_bc_cells_axis(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys,
+                                double* const fields[6], const double* rhoY0_bg, int axis) {
+  if (!ctx || !grid || !phys || !fields) return pb_fail(ctx, PB_ERR_ARG, "pb_bc_cells_axis: null argument");
+  if (axis < 0) return pb_bc_cells(ctx, grid, phys, fields, rhoY0_bg);
+  if (axis > 1) return pb_fail(ctx, PB_ERR_ARG, "pb_bc_cells_axis: axis %d (0, 1 or -1)", axis);
+  if (grid->nx < 3 || grid->ny < 5) return pb_fail(ctx, PB_ERR_ARG, "pb_bc_cells_axis: grid too small");
+  for (int k = 0; k < 6; ++k)
+    if (!fields[k]) return pb_fail(ctx, PB_ERR_ARG, "pb_bc_cells_axis: field %d is null", k);
+  if (axis == 1 && phys->g != 0.0 && !phys->is_compressible && !rhoY0_bg)
+    return pb_fail(ctx, PB_ERR_ARG, "pb_bc_cells_axis: background rhoY0 required when not compressible");
+  const Dims2 d = dims_of(*grid);
+  const BcParams bp = bc_params(*grid, *phys);
+  const int nthr = axis == 0 ? 4 * d.cy : d.cx;
+  hipLaunchKernelGGL(k_bc_axis, dim3(ceil_div(nthr, 256)), dim3(256), 0, ctx->stream, d, bp, axis,
                      fields[0], fields[1], fields[2], fields[3], fields[4], fields[5], rhoY0_bg);
   PB_LAUNCH_CHECK(ctx);
   return PB_OK;

@@ -88,9 +88,10 @@ def grid_of(node, ud) -> _abi.Grid2D:
 
 def phys_of(ud, th) -> _abi.Phys:
     cor = (C.c_double * 3)(*[float(v) for v in ud.coriolis_strength])
+    wind = (C.c_double * 3)(float(ud.u_wind_speed), float(ud.v_wind_speed), float(ud.w_wind_speed))
     return _abi.Phys(float(th.gamm), float(ud.Msq), float(ud.gravity_strength[1]), cor,
                      float(ud.nonhydrostasy), int(ud.is_compressible), float(ud.compressibility),
-                     int(ud.is_ArakawaKonor))
+                     int(ud.is_ArakawaKonor), wind)
 
 
 def dSdy_cells(mpv, elem, node):
@@ -506,11 +507,7 @@ def _finish(pr, x, info, Sol, Sol0, mpv, early=None):
     """second_projection.py:235-277: counters, the device update, then the
     side effects are written back to the host objects (``early``: the batch
     _early_download started for this projection, if any)."""
-    global total_iter, total_calls
-    total_iter += info["iters"]
-    total_calls += 1
-    logging.info(info["iters"])
-    logging.info("Total calls to BiCGStab routine = %i, total iterations = %i" % (total_calls, total_iter))
+    _count(info)
     _project_update(pr, x)
     pairs = pr.sol.pairs(Sol) + (pr.sol0.pairs(Sol0) if pr.sol0 is not None else [])
     pairs += [(pr.p2n, mpv.p2_nodes), (pr.u, None), (pr.v, None), (pr.w, None)]
@@ -524,6 +521,29 @@ def _finish(pr, x, info, Sol, Sol0, mpv, early=None):
     mpv.wcenter, mpv.nu_c = res[-2:]
 
 
+def _solve(pr, ud):
+    """second_projection.py:205-230: the solve of the prepared system with
+    ``solver_options``; returns (x, info dict)."""
+    mode = solver_options.get("mode", "f64")
+    extra = {}
+    if mode == "nn":   # the learned phase has its own budget; ud.max_iterations caps the fallback
+        extra = dict(fallback_maxiter=int(ud.max_iterations), sweeps=int(solver_options.get("nn_sweeps", 2)))
+    return pr.op.solve(pr.b, tol=float(ud.tol),
+                       maxiter=int(solver_options.get("nn_budget", 0)) if mode == "nn" else int(ud.max_iterations),
+                       mode=mode, inner_tol=solver_options.get("inner_tol", 1e-3),
+                       max_outer=solver_options.get("max_outer", 12),
+                       check_every=solver_options.get("check_every", 32), **extra)
+
+
+def _count(info):
+    """The reference's global solver counters (second_projection.py:235-239)."""
+    global total_iter, total_calls
+    total_iter += info["iters"]
+    total_calls += 1
+    logging.info(info["iters"])
+    logging.info("Total calls to BiCGStab routine = %i, total iterations = %i" % (total_calls, total_iter))
+
+
 def euler_backward_non_advective_impl_part(Sol, mpv, elem, node, ud, th, t, dt, alpha_diff, Sol0=None,
                                            writer=None, label=None, debug=False):
     """GPU second projection with the reference's signature and side effects
@@ -531,15 +551,7 @@ def euler_backward_non_advective_impl_part(Sol, mpv, elem, node, ud, th, t, dt, 
     op = operator_for(node, ud)
     pr = _prepare(Sol, mpv, elem, node, ud, th, dt, Sol0, op)
     early = _early_download(pr, mpv)
-    mode = solver_options.get("mode", "f64")
-    extra = {}
-    if mode == "nn":   # the learned phase has its own budget; ud.max_iterations caps the fallback
-        extra = dict(fallback_maxiter=int(ud.max_iterations), sweeps=int(solver_options.get("nn_sweeps", 2)))
-    x, info = op.solve(pr.b, tol=float(ud.tol),
-                       maxiter=int(solver_options.get("nn_budget", 0)) if mode == "nn" else int(ud.max_iterations),
-                       mode=mode, inner_tol=solver_options.get("inner_tol", 1e-3),
-                       max_outer=solver_options.get("max_outer", 12),
-                       check_every=solver_options.get("check_every", 32), **extra)
+    x, info = _solve(pr, ud)
     _finish(pr, x, info, Sol, Sol0, mpv, early)
     return info
 

@@ -91,6 +91,18 @@ class UserData:
         self.w_wind_speed = 0.0
         self.tol = 1.0e-8
         self.max_iterations = 6000
+        # time stepping (user_data.py:57-64, 87-101, 105-118)
+        self.acoustic_timestep = 0
+        self.tout = np.arange(0.0, 1.01, 0.01)[10:]
+        self.stepmax = 10000
+        self.do_advection = True
+        self.continuous_blending = False
+        self.initial_blending = False
+        self.no_of_pi_initial = 1
+        self.no_of_pi_transition = 0
+        self.no_of_hy_initial = 0
+        self.no_of_hy_transition = 0
+        self.aux = ""
         for k, v in kw.items():
             setattr(self, k, v)
         self.derive()
@@ -243,6 +255,22 @@ class States:
         s = np.repeat(self.S0[None, :], elem.sc[0], axis=0)
         self.S0c = s
         return s
+
+
+class Flux:
+    """Face fluxes of one direction (variable.States as src/__main__.py:70-75
+    allocates them): rho, rhou, rhov, rhow, rhoY, rhoX of shape elem.sfx =
+    (icy, icx+1) for the x faces ([y, face]) or elem.sfy = (icx, icy+1) for
+    the y faces ([x, face]) (grid.py:164-166)."""
+
+    def __init__(self, shape):
+        for k in ("rho", "rhou", "rhov", "rhow", "rhoY", "rhoX"):
+            setattr(self, k, np.zeros(shape))
+
+
+def flux_init(elem):
+    """[x-face, y-face] flux containers of a 2D grid."""
+    return [Flux((elem.icy, elem.icx + 1)), Flux((elem.icx, elem.icy + 1))]
 
 
 class MPV:

@@ -31,7 +31,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .state import (MPV, BdryType, Thermo, UserData, Vars, grid_init,
+from .state import (MPV, BdryType, Thermo, UserData, Vars, flux_init, grid_init,
                     hydrostatic_state)
 
 
@@ -140,6 +140,157 @@ def rising_bubble_state(nx=161, ny=81, delth=2.0, xc=0.0, y0=0.2, r0=0.2, dt=0.0
     return Sol, Sol0, mpv, elem, node, ud, th
 
 
+def rising_bubble_ud(nx=161, ny=81, **kw):
+    """UserData of the rising-bubble time step: inputs/rising_bubble.py:5-39
+    (CFL 0.5, dtfixed0 = dtfixed = 100, 'CFLfixed' in aux, so dt = 21.69/t_ref
+    on the first two steps) composed with the UserDataInit defaults
+    (user_data.py:28-129) that the shipped file omits, pseudo-incompressible,
+    g = 9.81 m/s^2 and t_ref = 100 s."""
+    ud = UserData(inx=nx, iny=ny, inz=1, is_compressible=0, compressibility=0.0, CFL=0.5, dtfixed0=100.0,
+                  dtfixed=100.0, stepmax=10000, aux="debug_imbal_CFLfixed", **kw)
+    return ud
+
+
+def rising_bubble_init(ud=None, delth=2.0, xc=0.0, y0=0.2, r0=0.2):
+    """sol_init of inputs/rising_bubble.py:42-99: the isothermal background
+    (hydrostatics.py:79-132) with rho = rhoY / (1 + delth/300 cos^2(pi r/2))
+    inside r < 1 (r the distance to (xc, y0) in units of r0), zero wind, zero
+    rhoX and p2_nodes.  The reference's closing ghost fill is left to the
+    time step, whose first action is the same fill (time_update.py:122).
+    Returns (Sol, flux, mpv, elem, node, ud, th)."""
+    if ud is None:
+        ud = rising_bubble_ud()
+    th = Thermo(ud)
+    elem, node = grid_init(ud)
+    Sol = Vars(elem.sc, ud)
+    mpv = MPV(elem, node, ud)
+    hydrostatic_state(mpv, elem, node, th, ud)
+    x, y = np.meshgrid(elem.x, elem.y)
+    r = np.sqrt((x - xc) ** 2 + (y - y0) ** 2) / r0
+    rhoY = np.repeat(mpv.HydroState.rhoY0.reshape(1, -1), elem.icx, axis=0)
+    pert = (delth / 300.0) * (np.cos(0.5 * np.pi * r) ** 2)
+    pert[np.where(r > 1.0)] = 0.0
+    rho = rhoY / (ud.stratification(y) + pert.T)
+    Sol.rho[...] = rho
+    Sol.rhou[...] = rho * ud.u_wind_speed
+    Sol.rhov[...] = rho * ud.v_wind_speed
+    Sol.rhow[...] = rho * ud.w_wind_speed
+    Sol.rhoY[...] = rhoY
+    mpv.p2_nodes[...] = 0.0
+    return Sol, flux_init(elem), mpv, elem, node, ud, th
+
+
+def travelling_vortex_ud(n=65):
+    """UserData of src/tests/test_travelling_vortex.py:10-158: 64 x 64 cells
+    on [-1/2, 1/2]^2, doubly periodic, no gravity, background wind (1, 1),
+    compressible, CFL 0.45, dt at most 0.01, 101 steps."""
+    ud = UserData(inx=n, iny=n, inz=1, grav=0.0, omega=0.0, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5,
+                  zmin=-0.5, zmax=0.5, u_wind_speed=1.0, v_wind_speed=1.0, w_wind_speed=0.0, CFL=0.45,
+                  dtfixed=0.01, dtfixed0=0.01, is_compressible=1, compressibility=1.0, is_nonhydrostatic=1,
+                  tout=[1.0], stepmax=101, aux="")
+    ud.bdry_type[1] = BdryType.PERIODIC
+    ud.initial_projection = True
+    return ud
+
+
+_TV_COE = (1.0 / 24.0, -6.0 / 13.0, 15.0 / 7.0, -74.0 / 15.0, 57.0 / 16.0, 174.0 / 17.0, -269.0 / 9.0,
+           450.0 / 19.0, 1071.0 / 40.0, -1564.0 / 21.0, 510.0 / 11.0, 1020.0 / 23.0, -1105.0 / 12.0,
+           204.0 / 5.0, 510.0 / 13.0, -1564.0 / 27.0, 153.0 / 8.0, 450.0 / 29.0, -269.0 / 15.0, 174.0 / 31.0,
+           57.0 / 32.0, -74.0 / 33.0, 15.0 / 17.0, -6.0 / 35.0, 1.0 / 72.0)
+_TV_CONST = (1.0 / 24, -6.0 / 13, 33.0 / 14, -22.0 / 3, 495.0 / 32, -396.0 / 17, +77.0 / 3, -396.0 / 19,
+             99.0 / 8, -110.0 / 21, +3.0 / 2, -6.0 / 23, +1.0 / 48)
+
+
+def travelling_vortex_init(project, ud=None):
+    """sol_init of src/tests/test_travelling_vortex.py:163-355 (no seed): a
+    vortex of radius 0.4 with a density dip and its balancing pressure in the
+    background wind, then the initial pseudo-incompressible projection with
+    dt = dtfixed (the wind removed around it, p2_nodes restored after it).
+    ``project`` is the projection to use (the signature of
+    euler_backward_non_advective_impl_part).  Returns (Sol, flux, mpv, elem,
+    node, ud, th)."""
+    if ud is None:
+        ud = travelling_vortex_ud()
+    th = Thermo(ud)
+    elem, node = grid_init(ud)
+    Sol = Vars(elem.sc, ud)
+    mpv = MPV(elem, node, ud)
+    u0, v0, w0 = ud.u_wind_speed, ud.v_wind_speed, 0.0
+    rotdir, p0, alpha, alpha_const = 1.0, 1.0, -1.0, 3.0
+    rho0, del_rho, R0, fac = 1.0, -0.5, 0.4, 1.0 * 1024.0
+    xc = yc = 0.0
+    xcm = xc - np.sign(xc) * (ud.xmax - ud.xmin)
+    ycm = yc - np.sign(yc) * (ud.ymax - ud.ymin)
+    igy = elem.igs[1]
+    igxn, igyn = node.igx, node.igy
+    hydrostatic_state(mpv, elem, node, th, ud)
+    xs = elem.x.reshape(-1, 1)
+    ys = elem.y[igy:-igy].reshape(1, -1)
+    xccs = np.zeros_like(xs)
+    yccs = np.zeros_like(ys)
+    xccs[...] = xc * (np.abs(xs - xc) < np.abs(xs - xcm))
+    xccs[...] += xcm * (np.abs(xs - xc) > np.abs(xs - xcm))
+    yccs[...] = yc * (np.abs(ys - yc) < np.abs(ys - ycm))
+    yccs[...] += ycm * (np.abs(ys - yc) > np.abs(ys - ycm))
+    r = np.sqrt((xs - xccs) ** 2 + (ys - yccs) ** 2)
+    uth = (rotdir * fac * (1.0 - r / R0) ** 6 * (r / R0) ** 6) * (r < R0)
+    u = u0 + uth * (-(ys - yccs) / r)
+    v = v0 + uth * (+(xs - xccs) / r)
+    w = w0
+    rho = np.zeros_like(r)
+    rho[...] += (rho0 + del_rho * (1.0 - (r / R0) ** 2) ** 6) * (r < R0)
+    rho[...] += rho0 * (r >= R0)
+    dp2c = np.zeros_like(r)
+    dp2c_const = np.zeros_like(r)
+    for ip in range(25):
+        dp2c += (_TV_COE[ip] * ((r / R0) ** (12 + ip) - 1.0) * rotdir ** 2) * (r / R0 < 1.0)
+    for ip in range(13):
+        dp2c_const += (_TV_CONST[ip] * ((r / R0) ** (12 + ip) - 1.0) * rotdir ** 2) * (r / R0 < 1.0)
+    dp2c = alpha * dp2c + alpha_const * dp2c_const
+    Sol.rho[:, igy:-igy] = rho
+    Sol.rhou[:, igy:-igy] = rho * u
+    Sol.rhov[:, igy:-igy] = rho * v
+    Sol.rhow[:, igy:-igy] = rho * w
+    if ud.is_compressible:
+        p = p0 + ud.Msq * fac ** 2 * dp2c
+        Sol.rhoY[:, igy:-igy] = p ** th.gamminv
+    else:
+        Sol.rhoY[:, igy:-igy] = mpv.HydroState.rhoY0[igy:-igy]
+    xs = node.x[igxn:-igxn].reshape(-1, 1)
+    ys = node.y[igyn:-igyn].reshape(1, -1)
+    xccs = np.zeros_like(xs)
+    yccs = np.zeros_like(ys)
+    yccs[np.where(np.abs(ys - yc) < np.abs(ys - ycm))] = yc
+    yccs[np.where(np.abs(ys - yc) > np.abs(ys - ycm))] = ycm
+    xccs[np.where(np.abs(xs - xc) < np.abs(xs - xcm))] = xc
+    xccs[np.where(np.abs(xs - xc) > np.abs(xs - xcm))] = xcm
+    r = np.sqrt((xs - xccs) ** 2 + (ys - yccs) ** 2)
+    inner = mpv.p2_nodes[igxn:-igxn, igyn:-igyn]
+    for ip in range(25):
+        inner += alpha * _TV_COE[ip] * ((r / R0) ** (12 + ip) - 1.0) * rotdir ** 2
+    for ip in range(13):
+        inner += alpha_const * _TV_CONST[ip] * ((r / R0) ** (12 + ip) - 1.0) * rotdir ** 2
+    inner *= r / R0 < 1.0
+    inner[...] = th.Gamma * fac ** 2 * np.divide(inner, mpv.HydroState.rhoY0[igyn:-igyn + 1])
+    ud.nonhydrostasy = float(ud.is_nonhydrostatic)
+    ud.compressibility = float(ud.is_compressible)
+    if getattr(ud, "initial_projection", False):
+        is_comp, comp = ud.is_compressible, ud.compressibility
+        ud.is_compressible = 0
+        ud.compressibility = 0.0
+        p2aux = np.copy(mpv.p2_nodes)
+        Sol.rhou -= u0 * Sol.rho
+        Sol.rhov -= v0 * Sol.rho
+        project(Sol, mpv, elem, node, ud, th, 0.0, ud.dtfixed, 0.5)
+        mpv.p2_nodes[...] = p2aux
+        mpv.dp2_nodes[...] = 0.0
+        Sol.rhou += u0 * Sol.rho
+        Sol.rhov += v0 * Sol.rho
+        ud.is_compressible = is_comp
+        ud.compressibility = comp
+    return Sol, flux_init(elem), mpv, elem, node, ud, th
+
+
 def random_momentum_state(nx, ny, seed, dt=0.05, compressible=False):
     """Stratified state with seeded smooth momenta and rhoX, used to exercise
     every term of the projection (divergence, both corrections, rhoX update)."""
@@ -236,4 +387,5 @@ def fourier_rhs_3d(nx, ny, nz, seed, nmodes=64, kmax=12):
 
 __all__ = ["stratified_state", "fourier_rhs", "left_null_vectors", "project_range",
            "stratified_rhs", "rising_bubble_state", "random_momentum_state",
-           "batch_member_bubble", "BdryType", "State3D", "stratified_state_3d", "fourier_rhs_3d"]
+           "batch_member_bubble", "rising_bubble_ud", "rising_bubble_init", "travelling_vortex_ud",
+           "travelling_vortex_init", "BdryType", "State3D", "stratified_state_3d", "fourier_rhs_3d"]

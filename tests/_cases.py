@@ -60,3 +60,67 @@ def random_op(nx, ny, seed, x_wall=False, y_wall=True, coriolis=True, center=Tru
 
 def checkerboard(ny, nx):
     return np.where((np.arange(ny)[:, None] + np.arange(nx)[None, :]) % 2 == 0, 1.0, -1.0)
+
+
+def timestep_state(kind, nx=33, ny=17, seed=3):
+    """A generic state for the time-step kernels: (Sol, flux, mpv, elem, node,
+    ud, th), ghost cells filled by the oracle.
+      "bubble"   the rising bubble (x periodic, y walls with gravity, psinc)
+                 with smooth random momenta and rhoX;
+      "xwall"    the same with walls in x, compressible;
+      "periodic" doubly periodic, no gravity, compressible, wind (0.3, -0.2),
+                 smooth random fields."""
+    from oracle import boundary as obd
+    from pybella_amd.state import MPV, Thermo, Vars, flux_init, grid_init, hydrostatic_state
+    rng = np.random.default_rng(seed)
+    if kind in ("bubble", "xwall"):
+        ud = W.rising_bubble_ud(nx, ny)
+        if kind == "xwall":
+            ud.bdry_type[0] = W.BdryType.WALL
+            ud.is_compressible, ud.compressibility = 1, 1.0
+        Sol, flux, mpv, elem, node, ud, th = W.rising_bubble_init(ud)
+        X, Y = np.meshgrid(elem.x, elem.y, indexing="ij")
+        Lx = 2.0
+    else:
+        ud = W.travelling_vortex_ud(nx)
+        ud.u_wind_speed, ud.v_wind_speed = 0.3, -0.2
+        th = Thermo(ud)
+        elem, node = grid_init(ud)
+        Sol = Vars(elem.sc, ud)
+        mpv = MPV(elem, node, ud)
+        hydrostatic_state(mpv, elem, node, th, ud)
+        flux = flux_init(elem)
+        X, Y = np.meshgrid(elem.x, elem.y, indexing="ij")
+        Lx = 1.0
+        Sol.rho[...] = 1.0
+        Sol.rhoY[...] = 1.0
+
+    def smooth(amp):
+        f = np.zeros_like(X)
+        for _ in range(4):
+            kx, ky = rng.integers(1, 3, 2)
+            f += amp * rng.standard_normal() * np.cos(2 * np.pi * kx * X / Lx + rng.uniform(0, 6.3)) * \
+                np.cos(np.pi * ky * Y + rng.uniform(0, 6.3))
+        return f
+    Sol.rho[...] *= 1.0 + smooth(0.02)
+    Sol.rhoY[...] *= 1.0 + smooth(0.01)
+    Sol.rhou[...] = Sol.rho * (ud.u_wind_speed + smooth(0.1))
+    Sol.rhov[...] = Sol.rho * (ud.v_wind_speed + smooth(0.1))
+    Sol.rhoX[...] = Sol.rho * smooth(0.01)
+    mpv.p2_nodes[...] = 0.01 * rng.standard_normal(mpv.p2_nodes.shape)
+    obd.set_explicit_boundary_data(Sol, elem, ud, th, mpv)
+    obd.set_ghostnodes_p2(mpv.p2_nodes, node, ud)
+    return Sol, flux, mpv, elem, node, ud, th
+
+
+def strip_node_null(p, q):
+    """Max-norm relative difference of node arrays p and q (reference layout,
+    two ghost layers) on the inner nodes after removing the difference's
+    components along the constant and the checkerboard."""
+    a, b = p[2:-2, 2:-2], q[2:-2, 2:-2]
+    nx, ny = a.shape
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    B = np.stack([np.ones(a.size), np.where((i + j) % 2 == 0, 1.0, -1.0).ravel()], axis=1)
+    d = (a - b).ravel()
+    c, *_ = np.linalg.lstsq(B, d, rcond=None)
+    return float(np.max(np.abs(d - B @ c)) / max(np.max(np.abs(b)), 1e-300))

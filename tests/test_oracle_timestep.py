@@ -1,0 +1,155 @@
+"""CPU pins of the time-step oracle (oracle/timestep.py).
+
+* The reference's own regression target: the travelling vortex of
+  src/tests/test_travelling_vortex.py (64 x 64 cells, doubly periodic,
+  compressible, wind (1, 1), initial pseudo-incompressible projection) run
+  for its 101 steps reproduces the float32 sums of rho, rhou, rhov, rhow,
+  rhoY, rhoX and p2_nodes in src/tests/test_targets.yml, checked with
+  numpy.isclose as src/tests/diagnostics.py:42-58 checks them
+  (tests/golden/reference_targets.json).
+* Exact invariants of the advection, which a dropped term, a wrong sign or a
+  wrong index breaks: the Strang sweep is symmetric under the exchange of x
+  and y (odd step of a state == transposed even step of the transposed
+  state, bit for bit); the flux form conserves every field's sum on a doubly
+  periodic grid; a smooth profile carried by a uniform wind converges at
+  second order to the translated profile.
+"""
+import copy
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import projection as opj
+from oracle import timestep as ots
+from pybella_amd import workloads as W
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_targets.json")
+FIELDS = ots.FIELDS
+
+
+def reference_targets(name):
+    with open(GOLD) as f:
+        return json.load(f)["targets"][name]
+
+
+def check_targets(Sol, p2_nodes, name):
+    """src/tests/diagnostics.py:42-58: float32 sums vs the targets, isclose."""
+    bad = {}
+    for key, ref in reference_targets(name).items():
+        arr = p2_nodes if key == "p2_nodes" else getattr(Sol, key)
+        test = arr.astype("float32").sum()
+        if not np.isclose(ref, test):
+            bad[key] = (ref, float(test))
+    return bad
+
+
+def test_travelling_vortex_reproduces_reference_targets():
+    Sol, flux, mpv, elem, node, ud, th = W.travelling_vortex_init(opj.euler_backward_non_advective_impl_part)
+    rec = []
+    Sol, flux, mpv, steps = ots.do(Sol, flux, mpv, 0.0, ud.tout[0], ud, elem, node, [0, 0], th, record=rec)
+    assert steps == [101, 101]
+    assert all(s.info == 0 for r in rec for s in r.solves)
+    assert check_targets(Sol, mpv.p2_nodes, "test_travelling_vortex") == {}
+
+
+def test_targets_catch_a_planted_mistake(monkeypatch):
+    """The same run with the slopes doubled (a dropped 1/2, planted in the
+    oracle) misses the targets, so the check above can fail."""
+    orig = ots._slopes
+    monkeypatch.setattr(ots, "_slopes", lambda d: 2.0 * orig(d))
+    Sol, flux, mpv, elem, node, ud, th = W.travelling_vortex_init(opj.euler_backward_non_advective_impl_part)
+    with np.errstate(all="ignore"):   # the broken scheme goes unstable
+        Sol, flux, mpv, _ = ots.do(Sol, flux, mpv, 0.0, ud.tout[0], ud, elem, node, [0, 0], th)
+    assert check_targets(Sol, mpv.p2_nodes, "test_travelling_vortex") != {}
+
+
+def _periodic_state(n=33, seed=3, wind=(0.3, -0.2)):
+    """Doubly periodic, no gravity, smooth random fields (cells with ghosts)."""
+    ud = W.travelling_vortex_ud(n)
+    ud.u_wind_speed, ud.v_wind_speed = wind
+    from pybella_amd.state import MPV, Thermo, Vars, flux_init, grid_init, hydrostatic_state
+    th = Thermo(ud)
+    elem, node = grid_init(ud)
+    Sol = Vars(elem.sc, ud)
+    mpv = MPV(elem, node, ud)
+    hydrostatic_state(mpv, elem, node, th, ud)
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(elem.x, elem.y, indexing="ij")
+
+    def smooth(amp):
+        f = np.zeros_like(X)
+        for _ in range(4):
+            kx, ky = rng.integers(1, 3, 2)
+            f += amp * rng.standard_normal() * np.cos(2 * np.pi * kx * X + rng.uniform(0, 6.3)) * \
+                np.cos(2 * np.pi * ky * Y + rng.uniform(0, 6.3))
+        return f
+    Sol.rho[...] = 1.0 + smooth(0.05)
+    Sol.rhoY[...] = 1.0 + smooth(0.02)
+    Sol.rhou[...] = Sol.rho * (wind[0] + smooth(0.1))
+    Sol.rhov[...] = Sol.rho * (wind[1] + smooth(0.1))
+    Sol.rhoX[...] = Sol.rho * smooth(0.1)
+    from oracle import boundary as obd
+    obd.set_explicit_boundary_data(Sol, elem, ud, th, mpv)
+    return Sol, flux_init(elem), mpv, elem, node, ud, th
+
+
+def _transpose(Sol):
+    T = copy.deepcopy(Sol)
+    for k in ("rho", "rhow", "rhoY", "rhoX"):
+        setattr(T, k, np.ascontiguousarray(getattr(Sol, k).T))
+    T.rhou = np.ascontiguousarray(Sol.rhov.T)
+    T.rhov = np.ascontiguousarray(Sol.rhou.T)
+    return T
+
+
+def test_strang_sweep_is_symmetric_in_x_and_y():
+    Sol, flux, mpv, elem, node, ud, th = _periodic_state()
+    T = _transpose(Sol)
+    fT = copy.deepcopy(flux)
+    ots.recompute_advective_fluxes(flux, Sol)
+    ots.recompute_advective_fluxes(fT, T)
+    ots.advect(Sol, flux, 0.01, elem, 1, ud, th, mpv)
+    ots.advect(T, fT, 0.01, elem, 0, ud, th, mpv)
+    back = _transpose(T)
+    for k in FIELDS:
+        assert np.array_equal(getattr(back, k), getattr(Sol, k)), k
+
+
+@pytest.mark.parametrize("rk", [False, True])
+def test_advection_conserves_every_field(rk):
+    Sol, flux, mpv, elem, node, ud, th = _periodic_state(seed=5)
+    inner = (slice(2, -2), slice(2, -2))
+    before = {k: getattr(Sol, k)[inner].sum() for k in FIELDS}
+    ots.recompute_advective_fluxes(flux, Sol)
+    (ots.advect_rk if rk else ots.advect)(Sol, flux, 0.01, elem, 1, ud, th, mpv)
+    for k in FIELDS:
+        a, b = getattr(Sol, k)[inner].sum(), before[k]
+        assert abs(a - b) <= 1e-12 * max(1.0, abs(b)), (k, a, b)
+
+
+def _translation_error(n):
+    """Carry rho = 1 + 0.2 sin(2 pi x) (rhoY = 1, wind (1, 0)) for t = 0.1
+    with the Strang sweep and return the max error against the exact shift."""
+    Sol, flux, mpv, elem, node, ud, th = _periodic_state(n=n, wind=(1.0, 0.0))
+    X = np.meshgrid(elem.x, elem.y, indexing="ij")[0]
+    prof = lambda x: 1.0 + 0.2 * np.sin(2 * np.pi * x)
+    Sol.rhoY[...] = 1.0
+    Sol.rho[...] = prof(X)
+    Sol.rhou[...] = Sol.rho * 1.0
+    Sol.rhov[...] = 0.0
+    Sol.rhoX[...] = 0.0
+    dt = 0.25 * elem.dx
+    steps = int(round(0.1 / dt))
+    for s in range(steps):
+        ots.recompute_advective_fluxes(flux, Sol)
+        ots.advect(Sol, flux, dt, elem, s % 2, ud, th, mpv)
+    exact = prof(X - steps * dt)
+    return np.max(np.abs(Sol.rho - exact)[2:-2, 2:-2])
+
+
+def test_translation_converges_at_second_order():
+    e1, e2 = _translation_error(33), _translation_error(65)
+    order = np.log2(e1 / e2)
+    assert order > 1.8, (e1, e2, order)
