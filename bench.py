@@ -160,7 +160,8 @@ def extra_workloads(args):
     from its own bench.py process (--workload 3d / batch) run here."""
     import subprocess
     out = {}
-    for wl, key, extra in (("3d", "workload_3d", []), ("batch", "batch", ["--members-per-size", "16"])):
+    for wl, key, extra in (("timestep", "time_steps", []), ("3d", "workload_3d", []),
+                           ("batch", "batch", ["--members-per-size", "16"])):
         cmd = [sys.executable, os.path.abspath(__file__), "--workload", wl, "--steps", "2", *extra]
         log("running", " ".join(cmd[1:]))
         t0 = time.perf_counter()
@@ -173,6 +174,105 @@ def extra_workloads(args):
         rec["wall_s"] = round(time.perf_counter() - t0, 1)
         out[key] = rec
     return out
+
+
+def _node_null_rel(p, q):
+    """Max-norm relative difference of node arrays (reference layout) on the
+    inner nodes, modulo the constant and the checkerboard."""
+    a, b = p[2:-2, 2:-2], q[2:-2, 2:-2]
+    i, j = np.meshgrid(np.arange(a.shape[0]), np.arange(a.shape[1]), indexing="ij")
+    B = np.stack([np.ones(a.size), np.where((i + j) % 2 == 0, 1.0, -1.0).ravel()], axis=1)
+    d = (a - b).ravel()
+    c, *_ = np.linalg.lstsq(B, d, rcond=None)
+    return float(np.max(np.abs(d - B @ c)) / np.max(np.abs(b)))
+
+
+class _SkipSmall(Exception):
+    pass
+
+
+def bench_timestep(args):
+    """Time steps per second of the rising bubble (time_update.do's loop
+    body: advection, explicit parts, both projections; the state resident on
+    the GPU, dt from the device CFL maxima): 161x81 nodes with the reference's
+    solver semantics (the CPU-runnable configuration), and 1025x513 nodes
+    with the plain FP64 solve and with the learned mode.  The first 161x81
+    steps are checked against the oracle's do() and timed on the CPU."""
+    import copy
+    import torch
+    from pybella_amd import second_projection as sp
+    from pybella_amd import time_update as tu
+    from pybella_amd import workloads as W
+    out = {"workload": "rising bubble (inputs/rising_bubble.py sol_init), psinc, dt = 21.69/t_ref on steps 0-1 "
+                       "then CFL 0.5; one step = both projections + RK and Strang advection + explicit parts"}
+
+    def run(nx, ny, nsteps, opts):
+        sp.solver_options.update(opts)
+        ud = W.rising_bubble_ud(nx, ny)
+        host = W.rising_bubble_init(ud)
+        Sol, flux, mpv, elem, node, ud, th = copy.deepcopy(host)
+        M = tu.model_for(elem, node, ud, th)
+        M.upload(Sol, flux, mpv)
+        tu.step(M, 0.0, 1e9, 0, 0)          # warm-up (workspaces, graphs)
+        M.upload(Sol, flux, mpv)
+        rate, t, recs = tu.steps_per_second(M, 0.0, 1e9, nsteps)
+        S2, f2, m2 = copy.deepcopy((Sol, flux, mpv))
+        M.download(S2, f2, m2, halves=False)
+        return host, rate, t, recs, (S2, m2)
+
+    base = dict(sp.solver_options)
+    want = (lambda k: args.ts_only in (None, k))
+    try:
+        # 161 x 81, reference semantics
+        n1 = max(args.steps, 10)
+        if not want("161x81"):
+            raise _SkipSmall
+        host, rate, t, recs, _ = run(161, 81, n1, {"mode": "f64"})
+        out["161x81"] = {"steps_per_s": round(rate, 2), "ms_per_step": round(1e3 / rate, 3), "steps": n1,
+                         "t_end": round(t, 4), "solver": SOLVER_NAMES["f64"],
+                         "iterations_per_step": [r["iters"] for r in recs]}
+        log("time steps 161x81:", out["161x81"])
+        # parity and the CPU leg on the first two steps
+        _, _, _, _, (S2, m2) = run(161, 81, 2, {"mode": "f64"})
+        if not args.no_cpu_baseline:
+            from oracle import timestep as ots
+            Sol, flux, mpv, elem, node, ud, th = copy.deepcopy(host)
+            ud.stepmax = 2
+            t0 = time.perf_counter()
+            Sol, flux, mpv, _ = ots.do(Sol, flux, mpv, 0.0, 1e9, ud, elem, node, [0, 0], th)
+            el = time.perf_counter() - t0
+            errs = {k: float(np.max(np.abs(getattr(S2, k) - getattr(Sol, k))) / np.max(np.abs(getattr(Sol, k))))
+                    for k in ("rho", "rhou", "rhov", "rhoY", "rhoX")}
+            errs["p2_nodes_mod_null"] = _node_null_rel(m2.p2_nodes, mpv.p2_nodes)
+            out["161x81"]["parity_vs_oracle_after_2_steps"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+            out["161x81"]["cpu_baseline"] = {
+                "value": round(2 / el, 4), "unit": "time steps/s", "cores": 1, "kind": "port",
+                "sample": f"the first 2 steps of the same run by the NumPy oracle (oracle/timestep.py do(), "
+                          f"vectorised restatement, FP64 BiCGStab to 1e-8), {el:.2f} s; CPU: {_cpu_model()}"}
+            out["161x81"]["gpu_over_cpu"] = round(rate / (2 / el), 1)
+    except _SkipSmall:
+        pass
+    try:
+        # 1025 x 513: plain FP64 solve and the learned mode
+        for name, opts in (("f64fast", {"mode": "f64fast"}), ("nn", {"mode": "nn", "nn_budget": 300})):
+            if not want(f"1025x513_{name}"):
+                continue
+            if name == "nn":
+                from pybella_amd.nn import Network, default_weights_path
+                net = Network(seed=0)
+                net.load(default_weights_path())
+                ud = W.rising_bubble_ud(1025, 513)
+                sp.operator_for(W.rising_bubble_init(ud)[4], ud).attach_nn(net)
+            _, rate, t, recs, _ = run(1025, 513, 4, opts)
+            out[f"1025x513_{name}"] = {"steps_per_s": round(rate, 3), "ms_per_step": round(1e3 / rate, 2),
+                                       "steps": 4, "t_end": round(t, 4), "solver": SOLVER_NAMES[name],
+                                       "iterations_per_step": [r["iters"] for r in recs]}
+            log(f"time steps 1025x513 {name}:", out[f"1025x513_{name}"])
+    finally:
+        sp.solver_options.clear()
+        sp.solver_options.update(base)
+    torch.cuda.synchronize()
+    print(json.dumps(out), flush=True)
 
 
 # Algorithmic words per node of the 3D kernels (cell weights ~ one word per
@@ -514,10 +614,13 @@ def main():
                          "residual) on the workload, for profiler runs")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
     ap.add_argument("--tile", type=int, default=4, help="rows per thread for --kernel-only")
-    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "batch", "step"],
+    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "batch", "step", "timestep"],
                     help="3d: the isolated 257x129x129 27-point solve; batch: 64 uneven rising-bubble "
                          "solves in one graph per chunk; step: whole projection steps (the drop-in "
-                         "impl_part) at 1025x513 (each its own JSON line, not the metric)")
+                         "impl_part) at 1025x513; timestep: rising-bubble time steps/s at 161x81 and "
+                         "1025x513 (each its own JSON line, not the metric)")
+    ap.add_argument("--ts-only", default=None, choices=["161x81", "1025x513_f64fast", "1025x513_nn"],
+                    help="--workload timestep: run only this configuration (for profiler runs)")
     ap.add_argument("--members-per-size", type=int, default=16)
     ap.add_argument("--batch-sizes", default="129x65,257x129,513x257,1025x513",
                     help="grid sizes of the batch workload, NXxNY comma-separated")
@@ -548,6 +651,9 @@ def main():
         return
     if args.workload == "step":
         bench_step(args)
+        return
+    if args.workload == "timestep":
+        bench_timestep(args)
         return
     if args.workload == "batch":
         bench_batch(args, tuple(tuple(int(v) for v in z.split("x")) for z in args.batch_sizes.split(",")))

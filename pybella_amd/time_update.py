@@ -22,9 +22,11 @@ one, each a thin wrapper over one C-ABI entry point (include/pybella_amd.h):
   the implicit projection      second_projection.py:97-277 (second_projection)
 
 Scope: the reference's path for one member as src/__main__.py runs it with
-one member, i.e. blending off (blending.py reduces to the eos regime
-functions), no Rayleigh damping or forcing (time_update.py:296-348,
-478-524) and no writer.  The stratification function must be 1 when y has
+one member, i.e. blending off (the Blend object it passes is inert without
+continuous or initial blending, and blending.py reduces to the eos regime
+functions), the non-hydrostatic regime, no Rayleigh damping or forcing
+(time_update.py:296-348, 478-524) and no writer; do() raises
+NotImplementedError outside that.  The stratification function must be 1 when y has
 gravity (inputs/rising_bubble.py, user_data.py:180-182).  There is no CPU
 path: without the built library or a GPU every call raises.
 """
@@ -316,8 +318,13 @@ def do(Sol, flux, mpv, t, tout, ud, elem, node, steps, th, bld=None, writer=None
     """time_update.py:59-708 for one member on the GPU: steps from t until
     tout or ud.stepmax, updating Sol, flux and mpv as the reference does.
     Returns [Sol, flux, mpv, [window_step, step]]."""
-    if bld is not None:
-        raise NotImplementedError("blending (data_assimilation.blending) is not part of this path")
+    blending = getattr(ud, "continuous_blending", False) or getattr(ud, "initial_blending", False)
+    if bld is not None:   # blending.Blend: active only with continuous blending (blending.py:16-32)
+        blending = blending or getattr(bld, "cb", False) or getattr(bld, "bb", False)
+    if blending:
+        raise NotImplementedError("regime blending (data_assimilation/blending.py) is not part of this path")
+    if getattr(ud, "is_nonhydrostatic", 1) != 1:
+        raise NotImplementedError("the hydrostatic regime (time_update.py:137-152, 232-255) is not part of this path")
     if writer is not None or debug:
         raise NotImplementedError("the HDF5 writer is not part of this path")
     if ud.bdry_type[1].name == "RAYLEIGH" or getattr(ud, "rayleigh_forcing", False):
