@@ -18,6 +18,7 @@
  * Reference interfaces replaced (pyBELLA tree, file:line):
  *   pb_bc_cells            set_explicit_boundary_data   src/dycore/utils/boundary.py:7
  *   pb_ghostnodes          set_ghostnodes_p2            src/dycore/utils/boundary.py:296
+ *   pb_ghostcells_p2       set_ghostcells_p2            src/dycore/utils/boundary.py:284
  *   pb_op2d_assemble       operator_coefficients_nodes  second_projection.py:308
  *                          + multiply_inverse_coriolis(get_coeffs=True) :604
  *                          + precon_diag_prepare        laplacian.py:1284
@@ -81,6 +82,10 @@ typedef struct pb_phys {
   double compressibility;  /* ud.compressibility                         */
   int is_arakawa_konor;    /* ud.is_ArakawaKonor                         */
   double wind[3];          /* ud.u/v/w_wind_speed (time-step kernels)     */
+  int stratified;          /* 1: strat_inv below holds 1/stratification(y)
+                              at the gravity-axis ghost rows; 0: 1 there   */
+  double strat_inv[4];     /* cell ghost rows 0, 1, ny+1, ny+2 (bottom to
+                              top) of the y images (boundary.py:83)        */
 } pb_phys;
 
 /* 3D node grid: nx * ny * nz inner nodes (y the gravity axis), spacings,
@@ -142,6 +147,11 @@ int pb_bc_cells(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys,
  * Without gravity (phys->g == 0) y is filled like x: wrap or mirror. */
 int pb_bc_cells_axis(pb_ctx* ctx, const pb_grid2d* grid, const pb_phys* phys,
                      double* const fields[6], const double* rhoY0_bg, int axis);
+
+/* cell ghost fill of a scalar (ny+3)x(nx+3) cell array such as mpv.p2_cells:
+ * wrap (periodic) or mirror (wall), x then y.  Replaces set_ghostcells_p2,
+ * src/dycore/utils/boundary.py:284-293. */
+int pb_ghostcells_p2(pb_ctx* ctx, const pb_grid2d* grid, double* p);
 
 /* node ghost fill of a (ny+4)x(nx+4) node array (periodic_plus_one / reflect) */
 int pb_ghostnodes(pb_ctx* ctx, const pb_grid2d* grid, double* p);

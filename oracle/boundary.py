@@ -107,6 +107,18 @@ def set_explicit_boundary_data(Sol, elem, ud, th, mpv, step=None):
             offset += 1
 
 
+def set_ghostcells_p2(p, elem, ud):
+    """Cell ghost fill of a scalar cell array (boundary.py:284-293): 'wrap'
+    on periodic axes, 'symmetric' otherwise, axis by axis."""
+    igs = elem.igs
+    for dim in range(elem.ndim):
+        pads = [(0, 0)] * elem.ndim
+        pads[dim] = (igs[dim], igs[dim])
+        idx = [slice(None)] * elem.ndim
+        idx[dim] = slice(igs[dim], -igs[dim])
+        p[...] = np.pad(p[tuple(idx)], pads, "wrap" if _periodic(ud.bdry_type[dim]) else "symmetric")
+
+
 def _periodic_plus_one_1d(vec, lo, hi):
     """boundary.py:348-369 applied to one padded vector (in place)."""
     if lo > 0 and hi > 0:

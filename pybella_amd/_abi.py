@@ -35,7 +35,8 @@ class Phys(C.Structure):
     _fields_ = [("gamm", C.c_double), ("Msq", C.c_double), ("g", C.c_double),
                 ("coriolis", C.c_double * 3), ("nonhydro", C.c_double),
                 ("is_compressible", C.c_int), ("compressibility", C.c_double),
-                ("is_arakawa_konor", C.c_int), ("wind", C.c_double * 3)]
+                ("is_arakawa_konor", C.c_int), ("wind", C.c_double * 3), ("stratified", C.c_int),
+                ("strat_inv", C.c_double * 4)]
 
 
 class SolveOpts(C.Structure):
@@ -78,6 +79,7 @@ SIGNATURES = {
     "pb_last_error": (C.c_char_p, [VP]),
     "pb_bc_cells": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), VP * 6, VP]),
     "pb_ghostnodes": (I, [VP, C.POINTER(Grid2D), VP]),
+    "pb_ghostcells_p2": (I, [VP, C.POINTER(Grid2D), VP]),
     "pb_bc_cells_axis": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), VP * 6, VP, I]),
     "pb_advective_fluxes": (I, [VP, C.POINTER(Grid2D), VP, VP, VP, VP, VP, VP]),
     "pb_advect_split": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), I, D, I, VP * 6, VP * 6, VP]),

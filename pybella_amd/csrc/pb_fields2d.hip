@@ -7,6 +7,7 @@
 // Reference counterparts (pyBELLA tree):
 //   k_bc_cells            set_explicit_boundary_data  src/dycore/utils/boundary.py:7-132
 //   k_ghost_x / k_ghost_y set_ghostnodes_p2           src/dycore/utils/boundary.py:296-369
+//   k_ghostcells_x / _y   set_ghostcells_p2           src/dycore/utils/boundary.py:284-293
 //   k_zero_wall_momenta + k_divergence
 //                         divergence_nodes            second_projection.py:525-575
 //   k_correction          correction_nodes + grad_nodes + multiply_inverse_coriolis
@@ -45,16 +46,16 @@ struct BcParams {
   double Gamma_g;   // th.Gamma * g
   double half_dy;   // 0.5 * elem.dy
   double gm1, gm1inv;
+  double Sinv[4];   // 1 / stratification(y) of image rows 0, 1, cy-2, cy-1
 };
 
 __device__ __forceinline__ void y_image(double* rho, double* rhou, double* rhov, double* rhow,
                                         double* rhoY, double* rhoX, const double* rhoY0,
                                         int cx, int col, int img, int last, int src, double dir,
-                                        const BcParams& bp) {
+                                        double S, const BcParams& bp) {
   const long il = (long)last * cx + col, is = (long)src * cx + col, ii = (long)img * cx + col;
   const double Y_last = rhoY[il] / rho[il];
   const double rhoYv_image = -rhov[is] * rhoY[is] / rho[is];
-  const double S = 1.0;  // 1 / stratification(y); the stratification function is 1
   const double dpi = dir * bp.Gamma_g * bp.half_dy * (1.0 / Y_last + S);
   const double rY = bp.compressible ? pow(pow(rhoY[il], bp.gm1) + dpi, bp.gm1inv) : rhoY0[img];
   const double r = rY * S;
@@ -132,11 +133,11 @@ __device__ __forceinline__ void y_fill_column(double* rho, double* rhou, double*
     return;
   }
   // bottom (direction +1): image 1 <- source 2, image 0 <- source 3
-  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, 1, 2, 2, 1.0, bp);
-  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, 0, 1, 3, 1.0, bp);
+  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, 1, 2, 2, 1.0, bp.Sinv[1], bp);
+  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, 0, 1, 3, 1.0, bp.Sinv[0], bp);
   // top (direction -1)
-  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, cy - 2, cy - 3, cy - 3, -1.0, bp);
-  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, cy - 1, cy - 2, cy - 4, -1.0, bp);
+  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, cy - 2, cy - 3, cy - 3, -1.0, bp.Sinv[2], bp);
+  y_image(rho, rhou, rhov, rhow, rhoY, rhoX, rhoY0, cx, col, cy - 1, cy - 2, cy - 4, -1.0, bp.Sinv[3], bp);
 }
 
 // Threads 0 .. cx-1: column ``col``'s y-axis fill (rows 0, 1, cy-2, cy-1),
@@ -221,6 +222,46 @@ __global__ void k_ghost_y(Dims2 d, double* p, int periodic) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= d.nxn) return;
   fill_line(p + col, d.nxn, d.ny, periodic);
+}
+
+// set_ghostcells_p2 on a cell array (boundary.py:284-293): np.pad 'wrap'
+// (periodic) or 'symmetric' (wall) of the two ghost layers, x over every row,
+// then y over every column.
+__global__ void k_ghostcells_x(Dims2 d, double* p, int periodic) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= d.cy) return;
+  const int cx = d.cx;
+  double* v = p + (long)row * cx;
+  if (periodic) {
+    v[0] = v[cx - 4];
+    v[1] = v[cx - 3];
+    v[cx - 2] = v[2];
+    v[cx - 1] = v[3];
+  } else {
+    v[1] = v[2];
+    v[0] = v[3];
+    v[cx - 2] = v[cx - 3];
+    v[cx - 1] = v[cx - 4];
+  }
+}
+
+__global__ void k_ghostcells_y(Dims2 d, double* p, int periodic) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= d.cx) return;
+  const long s = d.cx;
+  double* v = p + col;
+  const int cy = d.cy;
+  if (periodic) {
+    v[0] = v[(cy - 4) * s];
+    v[s] = v[(cy - 3) * s];
+    v[(cy - 2) * s] = v[2 * s];
+    v[(cy - 1) * s] = v[3 * s];
+  } else {
+    v[s] = v[2 * s];
+    v[0] = v[3 * s];
+    v[(cy - 2) * s] = v[(cy - 3) * s];
+    v[(cy - 1) * s] = v[(cy - 4) * s];
+  }
 }
 
 __global__ void k_scatter_p2(Dims2 d, const double* x, double* pf) {
@@ -332,8 +373,11 @@ __global__ void k_correction(Dims2 d, CorrParams cp, const double* p, const doub
 
 BcParams bc_params(const pb_grid2d& g, const pb_phys& ph) {
   const double gm1 = ph.gamm - 1.0;
-  return BcParams{g.bc_x == PB_BC_PERIODIC, ph.g != 0.0, g.bc_y == PB_BC_PERIODIC, ph.is_compressible == 1,
-                  (gm1 / ph.gamm) * ph.g, 0.5 * g.dy, gm1, 1.0 / gm1};
+  BcParams bp{g.bc_x == PB_BC_PERIODIC, ph.g != 0.0, g.bc_y == PB_BC_PERIODIC, ph.is_compressible == 1,
+              (gm1 / ph.gamm) * ph.g, 0.5 * g.dy, gm1, 1.0 / gm1, {1.0, 1.0, 1.0, 1.0}};
+  if (ph.stratified)
+    for (int k = 0; k < 4; ++k) bp.Sinv[k] = ph.strat_inv[k];
+  return bp;
 }
 
 }  // namespace
@@ -381,6 +425,18 @@ extern "C" int pb_ghostnodes(pb_ctx* ctx, const pb_grid2d* grid, double* p) {
   hipLaunchKernelGGL(k_ghost_x, dim3(ceil_div(d.nyn, 64)), dim3(64), 0, ctx->stream, d, p,
                      grid->bc_x == PB_BC_PERIODIC);
   hipLaunchKernelGGL(k_ghost_y, dim3(ceil_div(d.nxn, 64)), dim3(64), 0, ctx->stream, d, p,
+                     grid->bc_y == PB_BC_PERIODIC);
+  PB_LAUNCH_CHECK(ctx);
+  return PB_OK;
+}
+
+extern "C" int pb_ghostcells_p2(pb_ctx* ctx, const pb_grid2d* grid, double* p) {
+  if (!ctx || !grid || !p) return pb_fail(ctx, PB_ERR_ARG, "pb_ghostcells_p2: null argument");
+  if (grid->nx < 3 || grid->ny < 3) return pb_fail(ctx, PB_ERR_ARG, "pb_ghostcells_p2: grid too small");
+  const Dims2 d = dims_of(*grid);
+  hipLaunchKernelGGL(k_ghostcells_x, dim3(ceil_div(d.cy, 64)), dim3(64), 0, ctx->stream, d, p,
+                     grid->bc_x == PB_BC_PERIODIC);
+  hipLaunchKernelGGL(k_ghostcells_y, dim3(ceil_div(d.cx, 64)), dim3(64), 0, ctx->stream, d, p,
                      grid->bc_y == PB_BC_PERIODIC);
   PB_LAUNCH_CHECK(ctx);
   return PB_OK;

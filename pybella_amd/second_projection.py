@@ -89,9 +89,21 @@ def grid_of(node, ud) -> _abi.Grid2D:
 def phys_of(ud, th) -> _abi.Phys:
     cor = (C.c_double * 3)(*[float(v) for v in ud.coriolis_strength])
     wind = (C.c_double * 3)(float(ud.u_wind_speed), float(ud.v_wind_speed), float(ud.w_wind_speed))
+    sinv = _ghost_strat_inv(ud)
     return _abi.Phys(float(th.gamm), float(ud.Msq), float(ud.gravity_strength[1]), cor,
                      float(ud.nonhydrostasy), int(ud.is_compressible), float(ud.compressibility),
-                     int(ud.is_ArakawaKonor), wind)
+                     int(ud.is_ArakawaKonor), wind, int(any(v != 1.0 for v in sinv)), (C.c_double * 4)(*sinv))
+
+
+def _ghost_strat_inv(ud):
+    """1 / ud.stratification(y) at the four cell ghost rows of the gravity
+    axis (rows 0, 1, ny+1, ny+2), with y as grid.py computes the cell centres
+    (y0 - 2 dy + dy/2 + j dy)."""
+    dy = (ud.ymax - ud.ymin) / (ud.iny - 1.0)
+    y0 = ud.ymin - 2 * dy + 0.5 * dy
+    rows = np.array([0, 1, ud.iny + 1, ud.iny + 2])
+    ys = y0 + dy * rows
+    return [float(1.0 / np.asarray(ud.stratification(np.array([yv])), dtype=np.float64).ravel()[0]) for yv in ys]
 
 
 def dSdy_cells(mpv, elem, node):

@@ -26,8 +26,7 @@ one member, i.e. blending off (the Blend object it passes is inert without
 continuous or initial blending, and blending.py reduces to the eos regime
 functions), the non-hydrostatic regime, no Rayleigh damping or forcing
 (time_update.py:296-348, 478-524) and no writer; do() raises
-NotImplementedError outside that.  The stratification function must be 1 when y has
-gravity (inputs/rising_bubble.py, user_data.py:180-182).  There is no CPU
+NotImplementedError outside that.  There is no CPU
 path: without the built library or a GPU every call raises.
 """
 from __future__ import annotations
@@ -329,11 +328,6 @@ def do(Sol, flux, mpv, t, tout, ud, elem, node, steps, th, bld=None, writer=None
         raise NotImplementedError("the HDF5 writer is not part of this path")
     if ud.bdry_type[1].name == "RAYLEIGH" or getattr(ud, "rayleigh_forcing", False):
         raise NotImplementedError("Rayleigh damping / forcing is not part of this path")
-    if ud.gravity_strength[1] != 0.0:
-        yc = elem.y
-        s = ud.stratification(yc)
-        if np.any(np.asarray(s) != 1.0):
-            raise NotImplementedError("the GPU ghost fill assumes stratification(y) == 1 on the gravity axis")
     window_step, step_no = steps
     M = model_for(elem, node, ud, th)
     M.upload(Sol, flux, mpv)

@@ -193,6 +193,35 @@ def travelling_vortex_ud(n=65):
     return ud
 
 
+def internal_long_wave_ud():
+    """UserData of src/tests/test_internal_long_wave.py:8-180: 301 x 10 cells
+    on x in [-300, 300] (periodic) and y in [0, 1] (walls), g = 9.81 m/s^2,
+    Coriolis omega t_ref on the first axis only, stratification
+    exp(N^2 y / g) with N^2 = 1e-4 s^-2, compressible, dt = 1 (CFL 0.9), solves
+    to 1e-12, 31 steps.  ``molly`` is the test's window function."""
+    ud = UserData(inx=302, iny=11, inz=1, grav=9.81, omega=7.292e-5, xmin=-300.0, xmax=300.0, ymin=0.0,
+                  ymax=1.0, zmin=-1.0, zmax=1.0, u_wind_speed=0.0, v_wind_speed=0.0, w_wind_speed=0.0, CFL=0.9,
+                  dtfixed0=1.0, dtfixed=1.0, is_compressible=1, compressibility=0.0, is_nonhydrostatic=1,
+                  tol=1.0e-12, max_iterations=6000, tout=[20.0 * 3000.0 / 100.0], stepmax=31,
+                  no_of_pi_initial=0, no_of_hy_initial=1, aux="")
+    ud.bdry_type[2] = BdryType.PERIODIC
+    ud.coriolis_strength[2] = 0.0     # the test sets only the first component
+    ud.scale_factor = 20.0
+    ud.Nsq_ref = 1.0e-4
+    Nsq = ud.Nsq_ref * ud.t_ref * ud.t_ref
+    g = ud.gravity_strength[1] / ud.Msq
+    ud.stratification = lambda y: np.exp(Nsq * y / g)
+
+    def molly(x):
+        del0 = 0.25
+        L = ud.xmax - ud.xmin
+        xi_l = np.minimum(1.0, (x - ud.xmin) / (del0 * L))
+        xi_r = np.minimum(1.0, (ud.xmax - x) / (del0 * L))
+        return 0.5 * np.minimum(1.0 - np.cos(np.pi * xi_l), 1.0 - np.cos(np.pi * xi_r))
+    ud.molly = molly
+    return ud
+
+
 _TV_COE = (1.0 / 24.0, -6.0 / 13.0, 15.0 / 7.0, -74.0 / 15.0, 57.0 / 16.0, 174.0 / 17.0, -269.0 / 9.0,
            450.0 / 19.0, 1071.0 / 40.0, -1564.0 / 21.0, 510.0 / 11.0, 1020.0 / 23.0, -1105.0 / 12.0,
            204.0 / 5.0, 510.0 / 13.0, -1564.0 / 27.0, 153.0 / 8.0, 450.0 / 29.0, -269.0 / 15.0, 174.0 / 31.0,
@@ -388,4 +417,4 @@ def fourier_rhs_3d(nx, ny, nz, seed, nmodes=64, kmax=12):
 __all__ = ["stratified_state", "fourier_rhs", "left_null_vectors", "project_range",
            "stratified_rhs", "rising_bubble_state", "random_momentum_state",
            "batch_member_bubble", "rising_bubble_ud", "rising_bubble_init", "travelling_vortex_ud",
-           "travelling_vortex_init", "BdryType", "State3D", "stratified_state_3d", "fourier_rhs_3d"]
+           "travelling_vortex_init", "internal_long_wave_ud", "BdryType", "State3D", "stratified_state_3d", "fourier_rhs_3d"]

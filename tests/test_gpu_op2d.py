@@ -88,7 +88,8 @@ def test_matvec_large_1025x513():
 @pytest.mark.parametrize("shape", [(41, 21), (1025, 513), (3, 5)])
 @pytest.mark.parametrize("periodic_x", [True, False])
 def test_bc_cells_and_ghostnodes(periodic_x, shape):
-    """Ghost cells and ghost nodes against the oracle: one block of threads
+    """Ghost cells (fields and the scalar p2_cells fill) and ghost nodes
+    against the oracle: one block of threads
     (41x21), many blocks (1025x513: 1028 column threads and 2048 ghost-row
     threads) and the smallest grid the entry point takes (3x5, whose image
     source rows coincide)."""
@@ -123,6 +124,12 @@ def test_bc_cells_and_ghostnodes(periodic_x, shape):
         pd = sp.to_dev(p)
         op.ctx.check(op.ctx.lib.pb_ghostnodes(op.ctx.h, C.byref(sp.grid_of(node, ud)), _abi.ptr(pd)))
         assert np.array_equal(sp.from_dev(pd), pref)
+        c = rng.standard_normal(Sol.rho.shape)
+        cref = c.copy()
+        obd.set_ghostcells_p2(cref, elem, ud)
+        cd = sp.to_dev(c)
+        op.ctx.check(op.ctx.lib.pb_ghostcells_p2(op.ctx.h, C.byref(sp.grid_of(node, ud)), _abi.ptr(cd)))
+        assert np.array_equal(sp.from_dev(cd), cref)
 
 
 def test_divergence_and_correction():
