@@ -160,7 +160,7 @@ def extra_workloads(args):
     from its own bench.py process (--workload 3d / batch) run here."""
     import subprocess
     out = {}
-    for wl, key, extra in (("timestep", "time_steps", []), ("3d", "workload_3d", []),
+    for wl, key, extra in (("timestep", "time_steps", []), ("ensemble", "ensemble_steps", []), ("3d", "workload_3d", []),
                            ("batch", "batch", ["--members-per-size", "16"])):
         cmd = [sys.executable, os.path.abspath(__file__), "--workload", wl, "--steps", "2", *extra]
         log("running", " ".join(cmd[1:]))
@@ -185,6 +185,59 @@ def _node_null_rel(p, q):
     d = (a - b).ravel()
     c, *_ = np.linalg.lstsq(B, d, rcond=None)
     return float(np.max(np.abs(d - B @ c)) / np.max(np.abs(b)))
+
+
+def bench_ensemble(args, count=16, nx=161, ny=81, nsteps=6):
+    """Member time steps per second of a rising-bubble ensemble on one grid:
+    the members stepped in lockstep with their solves batched (one graph per
+    chunk of iterations over all members), beside the same members stepped
+    one after the other as the reference's member loop does
+    (src/__main__.py:223-254), both with the FP64 fast solver."""
+    import copy
+    import torch
+    from pybella_amd import ensemble as en
+    from pybella_amd import second_projection as sp
+    from pybella_amd import time_update as tu
+    members, elem, node, ud, th = en.rising_bubble_members(count, nx, ny)
+    E = en.Ensemble(copy.deepcopy(members), elem, node, ud, th, mode="f64fast")
+    E.run(0.0, 1e9, [0, 0], 1)                      # warm-up (graphs, workspaces)
+    for M, (Sol, flux, mpv) in zip(E.models, members):
+        M.upload(Sol, flux, mpv)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, recs = E.run(0.0, 1e9, [0, 0], nsteps)
+    torch.cuda.synchronize()
+    el_b = time.perf_counter() - t0
+    base = dict(sp.solver_options)
+    sp.solver_options["mode"] = "f64fast"
+    try:
+        models = [tu.DeviceModel(elem, node, copy.deepcopy(ud), th, op=sp.Operator2D(sp.grid_of(node, ud)))
+                  for _ in members]
+        for M, (Sol, flux, mpv) in zip(models, members):
+            M.upload(Sol, flux, mpv)
+            tu.step(M, 0.0, 1e9, 0, 0)
+            M.upload(Sol, flux, mpv)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for M in models:
+            t = 0.0
+            for k in range(nsteps):
+                dt, _, _, _ = tu.step(M, t, 1e9, k, k)
+                t += dt
+        torch.cuda.synchronize()
+        el_s = time.perf_counter() - t0
+    finally:
+        sp.solver_options.clear()
+        sp.solver_options.update(base)
+    its = [sum(i["iters"] for i in r[1]) for rr in recs for r in rr]
+    out = {"workload": f"rising-bubble ensemble, {count} members on {nx}x{ny} nodes (delth, xc, r0 per member from "
+                       f"default_rng(20261204 + m)), {nsteps} steps, FP64 fast solves",
+           "member_steps_per_s_batched": round(count * nsteps / el_b, 2),
+           "member_steps_per_s_sequential": round(count * nsteps / el_s, 2),
+           "batched_over_sequential": round(el_s / el_b, 2),
+           "mean_iterations_per_member_step": round(float(np.mean(its)), 1)}
+    log("ensemble:", out)
+    print(json.dumps(out), flush=True)
 
 
 class _SkipSmall(Exception):
@@ -614,11 +667,12 @@ def main():
                          "residual) on the workload, for profiler runs")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
     ap.add_argument("--tile", type=int, default=4, help="rows per thread for --kernel-only")
-    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "batch", "step", "timestep"],
+    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "batch", "step", "timestep", "ensemble"],
                     help="3d: the isolated 257x129x129 27-point solve; batch: 64 uneven rising-bubble "
                          "solves in one graph per chunk; step: whole projection steps (the drop-in "
                          "impl_part) at 1025x513; timestep: rising-bubble time steps/s at 161x81 and "
-                         "1025x513 (each its own JSON line, not the metric)")
+                         "1025x513; ensemble: 16 rising-bubble members stepped with batched solves (each its "
+                         "own JSON line, not the metric)")
     ap.add_argument("--ts-only", default=None, choices=["161x81", "1025x513_f64fast", "1025x513_nn"],
                     help="--workload timestep: run only this configuration (for profiler runs)")
     ap.add_argument("--members-per-size", type=int, default=16)
@@ -654,6 +708,9 @@ def main():
         return
     if args.workload == "timestep":
         bench_timestep(args)
+        return
+    if args.workload == "ensemble":
+        bench_ensemble(args)
         return
     if args.workload == "batch":
         bench_batch(args, tuple(tuple(int(v) for v in z.split("x")) for z in args.batch_sizes.split(",")))

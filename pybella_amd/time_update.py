@@ -57,11 +57,11 @@ class DeviceModel:
     [y][face], y faces [face][x], nodes [y][x]) and the work arrays of a
     step."""
 
-    def __init__(self, elem, node, ud, th):
+    def __init__(self, elem, node, ud, th, op=None):
         torch = _torch()
         self.elem, self.node, self.ud, self.th = elem, node, ud, th
         self.grid = sp.grid_of(node, ud)
-        self.op = sp.operator_for(node, ud)
+        self.op = op if op is not None else sp.operator_for(node, ud)
         self.ctx = self.op.ctx
         nx, ny = self.grid.nx, self.grid.ny
         cx, cy = nx + 3, ny + 3
@@ -221,10 +221,10 @@ def dynamic_timestep(M: DeviceModel, time, time_output, step):
     return dt, cfl, cfl_ac
 
 
-def project(M: DeviceModel, dt, with_sol0):
-    """euler_backward_non_advective_impl_part (second_projection.py:97-277)
-    on the device state, Sol0 = the state at the start of the step when
-    ``with_sol0``.  Returns the solver info dict."""
+def project_prepare(M: DeviceModel, dt, with_sol0):
+    """The part of the projection before the solve (second_projection.py:
+    113-214) on the device state: returns the prepared _Projection (its
+    right-hand side in M.b)."""
     pr = sp._Projection()
     pr.grid, pr.phys, pr.op, pr.ctx, pr.dt = M.grid, M.phys(), M.op, M.ctx, dt
     pr.compressible = int(M.ud.is_compressible)
@@ -236,6 +236,14 @@ def project(M: DeviceModel, dt, with_sol0):
     pr.wplus0, pr.wplus1, pr.nu_c = M.wplus0, M.wplus1, M.nu_c
     pr.wcenter, pr.diag_inv, pr.b = M.wcenter, M.diag_inv, M.b
     sp._project_prepare(pr)
+    return pr
+
+
+def project(M: DeviceModel, dt, with_sol0):
+    """euler_backward_non_advective_impl_part (second_projection.py:97-277)
+    on the device state, Sol0 = the state at the start of the step when
+    ``with_sol0``.  Returns the solver info dict."""
+    pr = project_prepare(M, dt, with_sol0)
     x, info = sp._solve(pr, M.ud)
     sp._count(info)
     sp._project_update(pr, x)
@@ -253,9 +261,11 @@ def _nonhydrostasy(ud, step):
     return float(ud.is_nonhydrostatic)
 
 
-def step(M: DeviceModel, t, tout, step_no, window_step):
-    """One pass of the loop body of time_update.py:121-690.  Returns
-    (dt, cfl, cfl_ac, [info of both solves])."""
+def step_begin(M: DeviceModel, t, tout, step_no, window_step):
+    """time_update.py:121-250 up to the first projection's solve: ghost
+    cells, dt, the regime functions, Sol0, the advective fluxes, the RK half
+    step and the explicit part.  Returns (dt, cfl, cfl_ac, prepared
+    projection)."""
     ud = M.ud
     F = M.sol.f
     M.sync()
@@ -274,7 +284,17 @@ def step(M: DeviceModel, t, tout, step_no, window_step):
         advect_rk(M, 0.5 * dt, step_no % 2)
     M.p2n0.copy_(M.p2n)
     euler_backward_non_advective_expl_part(M, 0.5 * dt)
-    infos = [project(M, 0.5 * dt, with_sol0=(ud.is_compressible == 0))]
+    return dt, cfl, cfl_ac, project_prepare(M, 0.5 * dt, with_sol0=(ud.is_compressible == 0))
+
+
+def step_middle(M: DeviceModel, dt, step_no, pr, x):
+    """time_update.py:251-462 after the first solve: the projection's update,
+    the half-step fluxes and copies, the restart from Sol0, the explicit
+    forward step, the Strang advection and the explicit part.  Returns the
+    second projection, prepared."""
+    ud = M.ud
+    F = M.sol.f
+    sp._project_update(pr, x)
     recompute_advective_fluxes(M)
     for d, s in zip(M.half, F.values()):
         d.copy_(s)
@@ -294,8 +314,20 @@ def step(M: DeviceModel, t, tout, step_no, window_step):
     if getattr(ud, "do_advection", True):
         advect(M, dt, step_no % 2)
     euler_backward_non_advective_expl_part(M, 0.5 * dt)
-    infos.append(project(M, 0.5 * dt, with_sol0=False))
-    return dt, cfl, cfl_ac, infos
+    return project_prepare(M, 0.5 * dt, with_sol0=False)
+
+
+def step(M: DeviceModel, t, tout, step_no, window_step):
+    """One pass of the loop body of time_update.py:121-690.  Returns
+    (dt, cfl, cfl_ac, [info of both solves])."""
+    dt, cfl, cfl_ac, pr = step_begin(M, t, tout, step_no, window_step)
+    x, info1 = sp._solve(pr, M.ud)
+    sp._count(info1)
+    pr2 = step_middle(M, dt, step_no, pr, x)
+    x2, info2 = sp._solve(pr2, M.ud)
+    sp._count(info2)
+    sp._project_update(pr2, x2)
+    return dt, cfl, cfl_ac, [info1, info2]
 
 
 _models: dict = {}
