@@ -236,8 +236,69 @@ def bench_ensemble(args, count=16, nx=161, ny=81, nsteps=6):
            "member_steps_per_s_sequential": round(count * nsteps / el_s, 2),
            "batched_over_sequential": round(el_s / el_b, 2),
            "mean_iterations_per_member_step": round(float(np.mean(its)), 1)}
+    out["letkf"] = bench_letkf(args, E, count, nx, ny)
+    log("letkf:", out["letkf"])
     log("ensemble:", out)
     print(json.dumps(out), flush=True)
+
+
+def bench_letkf(args, E, count, nx, ny, obs_X=11):
+    """One LETKF analysis (r-localisation, the reference's defaults: rho,
+    rhou, rhov, rhoY on cells and p2_nodes on nodes, 11 x 11 windows,
+    gaussian weights, 10 % of the locations observed, VarCov noise of 5 %)
+    of the ensemble's members, observed from the ensemble mean plus noise;
+    timed on the GPU, and on one host core by the oracle on a bounded
+    sample of grid points."""
+    import copy
+    import torch
+    from pybella_amd import letkf as lk
+    from pybella_amd import second_projection as sp
+    cell = ("rho", "rhou", "rhov", "rhoY")
+    rng = np.random.default_rng(20261206)
+    obs, mask, covar = {}, {}, {}
+    for a in cell + ("p2_nodes",):
+        f = np.mean([sp.from_dev(M.p2n if a == "p2_nodes" else M.sol.f[a]) for M in E.models], axis=0)[2:-2, 2:-2]
+        var = 0.05 * ((f - f.mean()) ** 2).mean() + 1e-12
+        obs[a] = f + np.sqrt(var) * rng.standard_normal(f.shape)
+        mask[a] = (rng.random(f.shape) >= 0.10).astype(float)
+        covar[a] = var
+    L = lk.LETKF(E.ud, attributes=cell + ("p2_nodes",), obs_X=obs_X, obs_Y=obs_X)
+    saved = [[t.clone() for t in list(M.sol.f.values()) + [M.p2n]] for M in E.models]
+    L.analyse(E.models, obs, mask, covar)   # warm-up
+    reps = 3
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for M, sv in zip(E.models, saved):
+            for t, v in zip(list(M.sol.f.values()) + [M.p2n], sv):
+                t.copy_(v)
+        L.analyse(E.models, obs, mask, covar)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    out = {"members": count, "grid": f"{nx - 1}x{ny - 1} cells + {nx}x{ny} nodes", "window": f"{obs_X}x{obs_X}",
+           "analysis_ms": round(el * 1e3, 3), "grid_points": (nx - 1) * (ny - 1) + nx * ny}
+    if not args.no_cpu_baseline:
+        from oracle import letkf as ol
+        names = list(E.models[0].sol.f)
+        cols = obs_X + 1                            # a bounded sample: obs_X + 1 columns of each grid
+        ms, parts = 0.0, []
+        for attrs, pick, gx, gy in ((cell, lambda sv, a: sv[names.index(a)], nx - 1, ny - 1),
+                                    (("p2_nodes",), lambda sv, a: sv[6], nx, ny)):
+            state = np.array([[sp.from_dev(pick(sv, a))[2:-2, 2:-2][:cols] for a in attrs] for sv in saved])
+            t0 = time.perf_counter()
+            ol.analyse_grid(state, np.array([obs[a][:cols] for a in attrs]), np.array([mask[a][:cols] for a in attrs]),
+                            np.array([covar[a] for a in attrs]), ol.loc_matrix(obs_X, obs_X),
+                            ol.boundary_mask((cols, gy), (True, False), (obs_X - 1) // 2, (obs_X - 1) // 2),
+                            obs_X, obs_X, 1.0)
+            per_point = (time.perf_counter() - t0) / (cols * gy)
+            ms += 1e3 * per_point * gx * gy
+            parts.append(f"{per_point * 1e3:.3f} ms per {len(attrs)}-attribute point")
+        out["cpu_baseline"] = {"value": round(ms, 1), "unit": "ms per analysis", "cores": 1, "kind": "port",
+                               "sample": f"the oracle's per-point analysis (oracle/letkf.py, NumPy + SciPy eigh, "
+                                         f"the same window) on {cols} columns of the cell and of the node grid "
+                                         f"({', '.join(parts)}), scaled to every grid point"}
+        out["gpu_over_cpu"] = round(ms / out["analysis_ms"], 1)
+    return out
 
 
 class _SkipSmall(Exception):

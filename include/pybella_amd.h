@@ -302,6 +302,25 @@ int pb_cfl_maxima(pb_ctx* ctx, const pb_grid2d* grid, double gamm, double Minv, 
                   const double* rhou, const double* rhov, const double* rhow, const double* rhoY,
                   double* work, double* maxima);
 
+/* ---- LETKF local analysis (data assimilation of an ensemble) ----
+ * One grid type (cells or nodes) of nx * ny inner points; every point is
+ * analysed over the obs_x * obs_y window around it (odd sizes, periodic
+ * wrap over the inner grid, zero localisation weight outside a wall).
+ * state, analysis: (members, attrs, ny, nx); obs, mask: (attrs, ny, nx),
+ * mask != 0 where a location is not observed; covar: (attrs) observation
+ * variances or NULL (1); loc: (obs_x, obs_y) localisation weights.  Members
+ * 2..32.  Replaces prepare_rloc.analyse_by_grid_type with analysis.analyse,
+ * src/data_assimilation/letkf.py:288-426, 120-166. */
+typedef struct pb_letkf_cfg {
+  int members, attrs, nx, ny;
+  int obs_x, obs_y;
+  int periodic_x, periodic_y;
+  double inflation;    /* dap.inflation_factor (rho)                      */
+} pb_letkf_cfg;
+
+int pb_letkf_analyse(pb_ctx* ctx, const pb_letkf_cfg* cfg, const double* state, const double* obs,
+                     const double* mask, const double* covar, const double* loc, double* analysis);
+
 /* ---- learned preconditioner (no reference counterpart: pyBELLA has none) ----
  * Fixed architecture: 4-level convolutional encoder-decoder, 3x3 convolutions,
  * 16-32-64-64 channels, GELU, 15 layers, BF16 MFMA kernels with FP32

@@ -39,6 +39,11 @@ class Phys(C.Structure):
                 ("strat_inv", C.c_double * 4)]
 
 
+class LetkfCfg(C.Structure):
+    _fields_ = [("members", C.c_int), ("attrs", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("obs_x", C.c_int),
+                ("obs_y", C.c_int), ("periodic_x", C.c_int), ("periodic_y", C.c_int), ("inflation", C.c_double)]
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("tol", C.c_double), ("maxiter", C.c_int),
                 ("inner_tol", C.c_double), ("max_outer", C.c_int), ("check_every", C.c_int),
@@ -87,6 +92,7 @@ SIGNATURES = {
     "pb_expl_part": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), D, VP * 6, VP, VP]),
     "pb_euler_forward": (I, [VP, C.POINTER(Grid2D), C.POINTER(Phys), D, VP * 6, VP, VP, VP, VP, VP]),
     "pb_cfl_work_words": (C.c_long, [C.POINTER(Grid2D)]),
+    "pb_letkf_analyse": (I, [VP, C.POINTER(LetkfCfg), VP, VP, VP, VP, VP, VP]),
     "pb_cfl_maxima": (I, [VP, C.POINTER(Grid2D), D, D, VP, VP, VP, VP, VP, VP, C.POINTER(D)]),
     "pb_op2d_create": (I, [VP, C.POINTER(Grid2D), C.POINTER(VP)]),
     "pb_op2d_destroy": (I, [VP]),
