@@ -94,3 +94,46 @@ class LETKF:
             fields = [[get(M, a) for a in attrs] for M in models]
             cv = None if covar is None else [covar[a] for a in attrs]
             self._analyse_type(fields, [obs[a] for a in attrs], [mask[a] for a in attrs], cv)
+
+
+class RLoc:
+    """Drop-in for the reference's prepare_rloc (src/data_assimilation/
+    letkf.py:190-260) on host containers: ``RLoc(ud, elem, node, dap, N)``
+    and ``analyse(results, obs, covar, mask, N, tout)`` with the reference's
+    arguments and return value (letkf.py:288-297): ``results`` the members'
+    [Sol, flux, mpv, steps] lists, ``obs`` / ``mask`` per DA time a dict of
+    attribute -> full array (ghosts included), ``covar`` the pair (cell,
+    node) of per-time variance arrays or None.  The analysed attributes are
+    set on the members as new arrays with zero ghost layers, as the
+    reference sets them."""
+
+    def __init__(self, ud, elem, node, dap, N):
+        self.N = N
+        self.da_times = list(dap.da_times)
+        self.attrs = list(dap.obs_attributes)
+        self.loc_of = dict(dap.loc)
+        self.obs_X, self.obs_Y = int(dap.obs_X), int(dap.obs_Y)
+        assert self.obs_X < elem.iicx and self.obs_Y < elem.iicy, "window larger than the inner grid"
+        assert self.obs_X % 2 == 1 and self.obs_Y % 2 == 1, "window sizes must be odd"
+        loc = np.asarray(dap.localisation_matrix, dtype=np.float64).reshape(self.obs_Y, self.obs_X).T
+        self.engine = LETKF(ud, attributes=self.attrs, obs_X=self.obs_X, obs_Y=self.obs_Y, loc=loc,
+                            inflation=float(dap.inflation_factor))
+
+    def analyse(self, results, obs, covar, mask, N, tout):
+        from . import second_projection as sp
+        ti = self.da_times.index(tout)
+        i2 = (slice(2, -2), slice(2, -2))
+        eng = self.engine
+        for grid, attrs, cov in (("cell", eng.cell_attrs, None if covar is None else covar[0]),
+                                 ("node", eng.node_attrs, None if covar is None else covar[1])):
+            if not attrs:
+                continue
+            holder = (lambda mem: mem[0]) if grid == "cell" else (lambda mem: mem[2])
+            fields = [[sp.to_dev(getattr(holder(mem), a)) for a in attrs] for mem in results]
+            cv = None if cov is None else [float(v) for v in np.asarray(cov)[ti]]
+            eng._analyse_type(fields, [np.asarray(obs[ti][a])[i2] for a in attrs],
+                              [np.asarray(mask[ti][a])[i2] for a in attrs], cv)
+            for mem, fs in zip(results, fields):
+                for a, f in zip(attrs, fs):
+                    setattr(holder(mem), a, sp.from_dev(f))
+        return results
