@@ -106,17 +106,35 @@ __device__ __forceinline__ double eps2() { return 4.930380657631324e-32; }
 //     indices, so no load sits behind a branch: faster for the matvec and
 //     the five-dot launch, slower for launch 1 (measured).
 //   ZC: z planes per block (the launch's grid z is ceil(nz / ZC)).
+//   RAW: ``in`` is a struct with a Raw type, load(g) -> Raw and
+//     combine(Raw) -> operand.  The plane fetch loads the raw values, the
+//     combined operand goes to the node plane, and the raw values of the
+//     tile's own nodes stay in LDS (one slot per node-plane slot), where
+//     ``out(g, y, pc, e, raw, acc)`` finds them: a launch that needs a node's
+//     inputs both combined (for the stencil) and raw (for its updates) reads
+//     them from memory once.
+template <class In, bool RAW, typename T>
+struct FetchOf {
+  using type = T;
+};
+template <class In, typename T>
+struct FetchOf<In, true, T> {
+  using type = typename In::Raw;
+};
+
 template <typename T, int NV, int PF = (sizeof(T) == 4 ? 2 : 1), bool UNR = false, bool GUARD = false,
-          int ZC = Z3, class In, class PreFn, class Out>
+          int ZC = Z3, bool RAW = false, class In, class PreFn, class Out>
 __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out out,
                                        double (&acc)[NV > 0 ? NV : 1]) {
   using PreT = decltype(pre(0L));
+  using FT = typename FetchOf<In, RAW, T>::type;
   constexpr bool AHEAD = PF == 2;   // node operands one step ahead
   constexpr bool UNCOND = !GUARD && (!AHEAD || UNR);   // branch-free plane and weight loads
   // three node-plane slots: plane k+1 is stored while threads that are still
   // in step k-1's output stage read plane k-1, so it needs a slot of its own
   __shared__ T P[3][Y3 + 2][X3 + 2];
   __shared__ T F[3][Y3 + 1][X3 + 1];
+  __shared__ FT RS[RAW ? 3 : 1][RAW ? Y3 : 1][RAW ? X3 : 1];   // own nodes' raw inputs (RAW)
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tid3();
   const MarchTile mt = march_tile(bid3(), gridDim.x, gridDim.y, gridDim.z);
   const int bzz = mt.bz;
@@ -141,7 +159,7 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
   // stored), so no load sits behind a branch and the wait counts stay
   // static; otherwise only the threads with a value load (fewer loads, which
   // the one-step-ahead FP32 marches prefer).
-  auto fetch = [&](int k, T (&r)[QPT]) {
+  auto fetch = [&](int k, FT (&r)[QPT]) {
     const long kk = wrap3(k, o.nz, o.pz);
 #pragma unroll
     for (int e = 0; e < QPT; ++e) {
@@ -150,19 +168,28 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
         const int q = min(q0, NQ - 1);
         const int jj = q / (X3 + 2), ii = q - jj * (X3 + 2);
         const int gj = wrap3(y0 - 1 + jj, ny, o.py), gi = wrap3(x0 - 1 + ii, nx, o.px);
-        r[e] = in((kk * ny + gj) * nx + gi);
+        if constexpr (RAW) r[e] = in.load((kk * ny + gj) * nx + gi);
+        else r[e] = in((kk * ny + gj) * nx + gi);
       }
     }
   };
-  auto store = [&](int slot, const T (&r)[QPT]) {
+  auto store = [&](int slot, const FT (&r)[QPT]) {
 #pragma unroll
     for (int e = 0; e < QPT; ++e) {
       const int q = tid + e * NT3;
-      if (q < NQ) P[slot][q / (X3 + 2)][q % (X3 + 2)] = r[e];
+      if (q < NQ) {
+        const int jj = q / (X3 + 2), ii = q % (X3 + 2);
+        if constexpr (RAW) {
+          P[slot][jj][ii] = in.combine(r[e]);
+          if (jj >= 1 && jj <= Y3 && ii >= 1 && ii <= X3) RS[slot][jj - 1][ii - 1] = r[e];
+        } else {
+          P[slot][jj][ii] = r[e];
+        }
+      }
     }
   };
   auto load_plane = [&](int k, int slot) {
-    T r[QPT];
+    FT r[QPT];
     fetch(k, r);
     store(slot, r);
   };
@@ -223,7 +250,7 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
   };
   load_plane(kb - 1, 0);
   load_plane(kb, 1);
-  T nxt[QPT];
+  FT nxt[QPT];
   fetch(kb + 1, nxt);       // plane kb+1 in flight during the first cell plane
   // two register sets for a step's cell weights and node operands, used in
   // turn (the march is unrolled by two), so a set loaded one step ahead is
@@ -270,7 +297,12 @@ __device__ __forceinline__ void march3(const Op3<T>& o, In in, PreFn pre, Out ou
       const long g = ((long)k * ny + j) * nx + i;
       const T pc = P[sk][ty + 1][tx + 1];
       T y = (Dp + Dn) + (Zn - Zp);
-      if (PF == 0) {
+      if constexpr (RAW) {
+        static_assert(!RAW || PF == 2, "RAW marches load their node operands one step ahead");
+        if (o.center) y += nd.hc * pc;
+        y *= nd.dinv;
+        out(g, y, pc, nd.e, RS[sk][ty][tx], acc);
+      } else if (PF == 0) {
         if (o.center) y += o.hc[g] * pc;
         y *= o.dinv[g];
         out(g, y, pc, pre(g), acc);
@@ -531,6 +563,22 @@ __device__ __forceinline__ void sum_rec3(const Rec3<NV>& e, const double* P, int
 template <typename T>
 constexpr int kZ1 = sizeof(T) == 4 ? Z3 : 24;
 
+// launch 1's node operand: r, p, v, t raw, p_k = (r - alpha v - omega t) +
+// beta (p - omega v) combined
+template <typename T>
+struct Iter1In {
+  struct Raw {
+    T v, r, p, t;
+  };
+  const T *va, *ra, *pa, *t;
+  T al, om, be;
+  __device__ __forceinline__ Raw load(long g) const { return Raw{va[g], ra[g], pa[g], t[g]}; }
+  __device__ __forceinline__ T combine(const Raw& q) const {
+    const T rn = (q.r - al * q.v) - om * q.t;
+    return rn + be * (q.p - om * q.v);
+  }
+};
+
 // launch 1 of iteration k: first the scalar step of iteration k-1's launch 2
 // (its five dots, when pending), then r_{k-1} = r - alpha v - omega t,
 // x += alpha p + omega (r - alpha v), p_k = r_{k-1} + beta (p - omega v),
@@ -553,23 +601,19 @@ __global__ __launch_bounds__(NT3) void k3_iter1(Op3<T> o, const KS3* Sin, KS3* S
   if (k.status != S3_RUN) return;
   const T al = (T)k.alpha, om = (T)k.omega, be = (T)k.beta;
   double acc[1] = {0.0};
-  struct Ops {   // this node's v, r, p, t, r~, x, loaded a plane step ahead
-    T v, r, p, t, rt, x;
+  struct Ops {   // this node's r~ and x, loaded a plane step ahead
+    T rt, x;
   };
-  // FP64: its node operands one step ahead too (112 VGPRs, two blocks per
-  // CU), over 24-plane chunks so the grid fits one round of blocks
-  march3<T, 1, 2, false, true, kZ1<T>>(
-      o,
-      [&](long g) {
-        const T v = va[g];
-        const T rn = (ra[g] - al * v) - om * t[g];
-        return rn + be * (pa[g] - om * v);
-      },
-      [&](long g) { return Ops{va[g], ra[g], pa[g], t[g], rt[g], x[g]}; },
-      [&](long g, T y, T pn, const Ops& e, double(&a)[1]) {
-        const T s = e.r - al * e.v;
-        x[g] = e.x + (al * e.p + om * s);
-        rb[g] = s - om * e.t;
+  // The node plane's r, p, v, t are read once: the operand p_k is combined
+  // from them for the stencil, and the tile's own nodes keep them in LDS for
+  // the updates.  Node operands one step ahead, over 24-plane chunks (FP64)
+  // so the grid fits one round of blocks.
+  march3<T, 1, 2, false, true, kZ1<T>, true>(
+      o, Iter1In<T>{va, ra, pa, t, al, om, be}, [&](long g) { return Ops{rt[g], x[g]}; },
+      [&](long g, T y, T pn, const Ops& e, const typename Iter1In<T>::Raw& w, double(&a)[1]) {
+        const T s = w.r - al * w.v;
+        x[g] = e.x + (al * w.p + om * s);
+        rb[g] = s - om * w.t;
         pb[g] = pn;
         vb[g] = y;
         a[0] += (double)e.rt * (double)y;
