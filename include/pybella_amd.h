@@ -57,6 +57,13 @@ extern "C" {
                                the network as its coarse solver                 */
 #define PB_SOLVE_F64_FAST 3 /* FP64 BiCGStab, rho and ||r|| from s/t dots
                                (2 launches per iteration)                      */
+#define PB_SOLVE_F64_FUSED 4 /* FP64 BiCGStab in one launch per iteration:
+                               rho and ||r|| as PB_SOLVE_F64_FAST, r~.v_k as
+                               u.p_k with u = A^T r~ formed once per solve.
+                               Needs the two copies of each duplicated periodic
+                               end node equal (initial residual, diag_inv,
+                               hcenter, seam cell weights); otherwise the
+                               solve runs PB_SOLVE_F64_FAST (info.fallback 1) */
 
 typedef struct pb_ctx pb_ctx;
 typedef struct pb_op2d pb_op2d;
@@ -124,7 +131,9 @@ typedef struct pb_solve_info {
   double resid;        /* last recursive residual norm                             */
   double bnrm2;        /* ||b||                                                    */
   double true_rel_resid; /* ||b - A x|| / ||b|| recomputed in FP64                 */
-  int fallback;        /* PB_SOLVE_NN: 1 if plain BiCGStab had to finish the solve  */
+  int fallback;        /* PB_SOLVE_NN: 1 if plain BiCGStab had to finish the solve;
+                          PB_SOLVE_F64_FUSED: 1 if the seams sent it to the
+                          two-launch iteration                               */
 } pb_solve_info;
 
 int pb_version(void);

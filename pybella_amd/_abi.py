@@ -19,6 +19,7 @@ PB_SOLVE_F64 = 0
 PB_SOLVE_MIXED = 1
 PB_SOLVE_NN = 2
 PB_SOLVE_F64_FAST = 3
+PB_SOLVE_F64_FUSED = 4
 
 
 class Grid2D(C.Structure):

@@ -602,6 +602,447 @@ __global__ __launch_bounds__(NTHR) void k_ex_update(int nx, int ny, T* r, const 
 }
 
 // ---------------------------------------------------------------------------
+// one-launch variant (PB_SOLVE_F64_FUSED)
+//
+// alpha = rho / (rt.v_k) needs a grid-wide sum between v_k = A p_k and
+// s_k = r - alpha v_k.  With u = A^T rt, formed once per solve, rt.v_k =
+// u.p_k, and u.p_k follows from the sums of the launch before:
+//   u.p_k = (u.s - omega u.t) + beta (u.p_{k-1} - omega u.v_{k-1}).
+// So iteration k is one launch F(k):
+//   omega, ||r_{k-1}||, rho and beta from F(k-1)'s sums of s.s, t.s, t.t,
+//   rt.s, rt.t (as the B launch's five dots give them), alpha_k = rho / u.p_k;
+//   r_{k-1} = s - omega t and p_k = r_{k-1} + beta (p_{k-1} - omega v_{k-1})
+//   on the tile with a two-node halo; x += alpha_{k-1} p_{k-1} + omega s;
+//   v_k = A p_k on the tile and the one-node ring around it; s_k = r -
+//   alpha_k v_k there; t_k = A s_k on the tile; sums s.s, t.s, t.t, rt.s,
+//   rt.t, u.s, u.t, u.p, u.v of iteration k.
+// Positions outside the grid take the values of the nodes the wrap maps them
+// to (period n - 1 on both axes, as the one-node halo of the operator: -1 ->
+// n-2, n -> 1).  On a wall axis those positions only meet cells of zero
+// weight.  On a periodic axis the ring node at -1 stands for node n-2 and
+// takes node 0 as its neighbour where node n-2 has node n-1: exact while the
+// two copies of the duplicated end node are equal in every vector, which
+// holds when they are equal in the initial residual and the operator's
+// diag_inv, hcenter and seam cells (every later vector is formed by the same
+// operations on equal values).  The launch that forms u counts mismatches of
+// those, and F(1) ends a solve that has any with ST_SEAM.
+// ---------------------------------------------------------------------------
+// records: s.s, t.s, t.t, rt.s, rt.t, u.s, u.t, u.p, u.v; the adjoint
+// launch's records carry the seam mismatches in the t.s slot
+constexpr int NFREC = 9;
+
+// Positions reach at most two nodes before the grid and one tile past it.
+// The wrap is exact for -2 <= i <= 2n - 2, which covers every position whose
+// value reaches a node of the grid (at most two past the last node); farther
+// positions (tiles wider than the grid) are clamped to the last node: any
+// finite value serves there.
+__device__ __forceinline__ int wrap_node(int i, int n) {
+  const int m = n - 1;
+  return i < 0 ? i + m : (i >= n ? min(i - m, m) : i);
+}
+__device__ __forceinline__ int wrap_cell(int c, int n) {   // op-cells 0..n, same rule
+  const int m = n - 1;
+  return c < 0 ? c + m : (c > n ? min(c - m, n) : c);
+}
+
+// (A^T rt)_j for node (j, i).  Node i's row of A gathers op-cells (J, I),
+// J in {j, j+1}, I in {i, i+1}, which have node i as a corner in their own
+// (unwrapped) coordinates; a cell's flux reads its corners through the wrap.
+// So node j receives from every cell one of whose corners wraps onto j: per
+// axis, cell i (j as its right / lower corner), cell i + 1 (left / upper),
+// and at the seams cell 0 (left corner -1 -> n-2) and cell n (right corner
+// n -> 1).  Each such cell passes on the weighted values z = diag_inv rt of
+// its unwrapped corners that are nodes.
+template <bool CROSS, bool CENTER>
+__device__ __forceinline__ double adjoint_at(const OpDev<double>& op, const double* rt, int j, int i) {
+  const int nx = op.nx, ny = op.ny;
+  const long g = (long)j * nx + i;
+  double u = CENTER ? op.hc[g] * (op.dinv[g] * rt[g]) : 0.0;
+  int cxs[4], kxs[4], ncx = 0, cys[4], kys[4], ncy = 0;
+  cxs[ncx] = i, kxs[ncx++] = 1;
+  cxs[ncx] = i + 1, kxs[ncx++] = 0;
+  if (i == nx - 2) cxs[ncx] = 0, kxs[ncx++] = 0;
+  if (i == 1) cxs[ncx] = nx, kxs[ncx++] = 1;
+  cys[ncy] = j, kys[ncy++] = 1;
+  cys[ncy] = j + 1, kys[ncy++] = 0;
+  if (j == ny - 2) cys[ncy] = 0, kys[ncy++] = 0;
+  if (j == 1) cys[ncy] = ny, kys[ncy++] = 1;
+  for (int a = 0; a < ncy; ++a) {
+    for (int b = 0; b < ncx; ++b) {
+      const int cy = cys[a], cx = cxs[b];
+      const long cc = (long)cy * (nx + 1) + cx;
+      const double gxk = kxs[b] ? 1.0 : -1.0, gyk = kys[a] ? 1.0 : -1.0;
+      double cfx = op.wxx[cc] * gxk, cfy = op.wyy[cc] * gyk;
+      if (CROSS) {
+        cfx += op.wyx[cc] * gyk;
+        cfy += op.wxy[cc] * gxk;
+      }
+#pragma unroll
+      for (int iy = 0; iy < 2; ++iy) {
+        const int nj = cy - 1 + iy;
+        if (nj < 0 || nj >= ny) continue;
+#pragma unroll
+        for (int ix = 0; ix < 2; ++ix) {
+          const int ni = cx - 1 + ix;
+          if (ni < 0 || ni >= nx) continue;
+          const long gn = (long)nj * nx + ni;
+          const double coef = (ix == 0 ? cfx : -cfx) + (iy == 0 ? cfy : -cfy);
+          u += coef * (op.dinv[gn] * rt[gn]);
+        }
+      }
+    }
+  }
+  return u;
+}
+
+// mismatches between the two copies of the duplicated end nodes that node
+// (j, i) is responsible for: its own pair when it lies on the first column
+// (row) of a periodic axis, and the pairs of seam cells of its cell row
+// (column)
+template <bool CROSS, bool CENTER>
+__device__ __forceinline__ double seam_mismatch(const OpDev<double>& op, const double* rt, int j, int i, int per_x,
+                                                int per_y) {
+  const int nx = op.nx, ny = op.ny;
+  double m = 0.0;
+  auto ne = [&](double a, double b) { m += (a == b) ? 0.0 : 1.0; };
+  auto nodes = [&](long g0, long g1) {
+    ne(rt[g0], rt[g1]);
+    ne(op.dinv[g0], op.dinv[g1]);
+    if (CENTER) ne(op.hc[g0], op.hc[g1]);
+  };
+  auto cells = [&](long c0, long c1) {
+    ne(op.wxx[c0], op.wxx[c1]);
+    ne(op.wyy[c0], op.wyy[c1]);
+    if (CROSS) {
+      ne(op.wyx[c0], op.wyx[c1]);
+      ne(op.wxy[c0], op.wxy[c1]);
+    }
+  };
+  const long W = nx + 1;
+  if (per_x && i == 0) {
+    nodes((long)j * nx, (long)j * nx + nx - 1);
+    for (int J = j; J <= (j == ny - 1 ? ny : j); ++J) {
+      cells(J * W + 0, J * W + nx - 1);
+      cells(J * W + 1, J * W + nx);
+    }
+  }
+  if (per_y && j == 0) {
+    nodes(i, (long)(ny - 1) * nx + i);
+    for (int I = i; I <= (i == nx - 1 ? nx : i); ++I) {
+      cells(I, (long)(ny - 1) * W + I);
+      cells(W + I, (long)ny * W + I);
+    }
+  }
+  return m;
+}
+
+// u = A^T rt on the tiles of the one-launch iteration, after the init launch:
+// the scalars after init from its records, records (rt.rt, seam mismatches,
+// 0, rt.rt, 0, u.rt, 0, 0, 0) for F(1)
+template <bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_adj_f(OpDev<double> op, const double* rt, double* u, int per_x, int per_y,
+                                                const KState* Sin, KState* Sout, const double* PB, double* Pout) {
+  __shared__ double sred[NFREC * (NTHR / kWave)];
+  const int tid = thread_id(), nb = nblocks(), bid = xcd_tile(block_id(), nb);
+  const int bx = bid % gridDim.x, by = bid / gridDim.x;
+  constexpr int TY = TYT * RY;
+  double tot[5];
+  sum_partials<5>(PB, nb, tot, sred);
+  const KState st = scalars_after_init(*Sin, tot[0], tot[1]);
+  double acc[NFREC];
+#pragma unroll
+  for (int k = 0; k < NFREC; ++k) acc[k] = 0.0;
+#pragma unroll
+  for (int q = 0; q < RY; ++q) {
+    const int j = by * TY + threadIdx.y + TYT * q, i = bx * TX + threadIdx.x;
+    if (j < op.ny && i < op.nx) {
+      const long g = (long)j * op.nx + i;
+      const double uj = adjoint_at<CROSS, CENTER>(op, rt, j, i), r = rt[g];
+      u[g] = uj;
+      acc[0] += r * r;
+      acc[3] += r * r;
+      acc[5] += uj * r;
+      acc[1] += seam_mismatch<CROSS, CENTER>(op, rt, j, i, per_x, per_y);
+    }
+  }
+  __syncthreads();   // every wave has read the init sums out of sred
+  write_partials<NFREC>(acc, sred, Pout, nb, bid);
+  if (bid == 0 && tid == 0) *Sout = st;
+}
+
+template <int RY>
+struct FusedLds {
+  static constexpr int TY = TYT * RY;
+  static constexpr int H2R = TY + 4, H2C = TX + 4;   // nodes with a two-node halo
+  static constexpr int H1R = TY + 2, H1C = TX + 2;   // nodes with a one-node ring
+  static constexpr int C1R = TY + 3, C1C = TX + 3;   // cells between the H2 nodes
+  double P[H2R * H2C];   // p_k
+  double R[H1R * H1C];   // r_{k-1}
+  double S[H1R * H1C];   // s_k
+  double FX[C1R * C1C], FY[C1R * C1C];
+};
+
+struct FPtrs {
+  const double *s0, *t0, *p0, *v0;   // iteration k-1
+  double *s1, *t1, *p1, *v1;         // iteration k
+  const double *rt, *u;
+  double* x;
+};
+
+template <bool CROSS, bool CENTER, int RY>
+__global__ __launch_bounds__(NTHR) void k_iter_f(OpDev<double> op, FPtrs q, const KState* Sin, KState* Sout,
+                                                 const double* Pin, double* Pout) {
+  using Lt = FusedLds<RY>;
+  static_assert(TYT == 4, "the halo-row pass gives each of the four thread rows one of the four halo rows");
+  constexpr int TY = Lt::TY, H2C = Lt::H2C, H1C = Lt::H1C, C1C = Lt::C1C;
+  constexpr int NC1 = Lt::C1R * C1C, NCP = (NC1 + NTHR - 1) / NTHR;
+  constexpr int NRING = 2 * H1C + 2 * TY, NEDGE = 4 * Lt::H2R;
+  static_assert(NRING <= NTHR && NEDGE <= NTHR, "one ring node and one edge value per thread");
+  __shared__ Lt L;
+  __shared__ double sred[NFREC * (NTHR / kWave)];
+  const int tid = thread_id(), nb = nblocks(), bid = xcd_tile(block_id(), nb);
+  const int bx = bid % gridDim.x, by = bid / gridDim.x;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int nx = op.nx, ny = op.ny, i0 = bx * TX, j0 = by * TY;
+  const bool writer = bid == 0 && tid == 0;
+  KState st = *Sin;
+  EarlyPartials<NFREC> early;
+  early.load(Pin, nb);
+  auto gw = [&](int j, int i) { return (unsigned)(wrap_node(j, ny) * nx + wrap_node(i, nx)); };
+  // iteration k-1's s, t, p, v on the two-node halo tile: the own rows
+  // (q < RY), one of the four halo rows (q = RY: rows 0, 1, TY+2, TY+3 by
+  // thread row) and one value of the four edge columns (q = RY + 1)
+  constexpr int NQ = RY + 2;
+  double rs[NQ], rt_[NQ], rp[NQ], rv[NQ];
+  int hr[NQ], hc[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    if (k < RY) {
+      hr[k] = 2 + ty + TYT * k;
+      hc[k] = 2 + tx;
+    } else if (k == RY) {
+      hr[k] = ty < 2 ? ty : TY + ty;
+      hc[k] = 2 + tx;
+    } else {
+      const int e = tid < NEDGE ? tid : 0, side = e & 3;
+      hr[k] = e >> 2;
+      hc[k] = side < 2 ? side : TX + side;
+    }
+    const unsigned g = gw(j0 - 2 + hr[k], i0 - 2 + hc[k]);
+    rs[k] = q.s0[g];
+    rt_[k] = q.t0[g];
+    rp[k] = q.p0[g];
+    rv[k] = q.v0[g];
+  }
+  // own positions: diag_inv, hcenter (every position: positions past the
+  // grid feed the ring); x, rt, u are loaded after the scalar step
+  double dv[RY], hv[RY];
+  bool ok[RY];
+  unsigned og[RY];
+#pragma unroll
+  for (int k = 0; k < RY; ++k) {
+    const int j = j0 + ty + TYT * k, i = i0 + tx;
+    ok[k] = j < ny && i < nx;
+    og[k] = ok[k] ? (unsigned)(j * nx + i) : 0u;
+    const unsigned g = gw(j, i);
+    dv[k] = op.dinv[g];
+    hv[k] = CENTER ? op.hc[g] : 0.0;
+  }
+  // this thread's ring node (H1 coordinates)
+  int ra = 0, rb = 0;
+  if (tid < H1C) {
+    ra = 0, rb = tid;
+  } else if (tid < 2 * H1C) {
+    ra = TY + 1, rb = tid - H1C;
+  } else if (tid < 2 * H1C + TY) {
+    ra = 1 + tid - 2 * H1C, rb = 0;
+  } else {
+    ra = 1 + min(tid - 2 * H1C - TY, TY - 1), rb = H1C - 1;
+  }
+  const bool ring = tid < NRING;
+  const unsigned rg = gw(j0 - 1 + ra, i0 - 1 + rb);
+  const double rdv = op.dinv[rg], rhc = CENTER ? op.hc[rg] : 0.0;
+  // cell weights of the C1 cells this thread forms
+  double wa[NCP], wb[NCP], wc[NCP], wd[NCP];
+#pragma unroll
+  for (int c = 0; c < NCP; ++c) {
+    const int k = min(tid + c * NTHR, NC1 - 1);
+    const int cy = k / C1C, cx = k - cy * C1C;
+    const unsigned cc = (unsigned)(wrap_cell(j0 - 1 + cy, ny) * (nx + 1) + wrap_cell(i0 - 1 + cx, nx));
+    wa[c] = op.wxx[cc];
+    wb[c] = op.wyy[cc];
+    wc[c] = CROSS ? op.wyx[cc] : 0.0;
+    wd[c] = CROSS ? op.wxy[cc] : 0.0;
+  }
+  // scalars (every thread the same)
+  double tot[NFREC];
+  sum_partials<NFREC>(early, Pin, nb, tot, sred);
+  const double alpha_old = st.alpha;
+  bool run = false, xup = false, half = false;
+  double al = 0.0, om = 0.0, be = 0.0;
+  if (!st_final(st.status)) {
+    if (st.phase == PH_INIT) {
+      if (tot[1] != 0.0) {
+        st.status = ST_SEAM;
+      } else {
+        const double a = st.rho / tot[5];
+        if (tot[5] == 0.0 || !isfinite(a)) st.status = ST_BRK_OMEGA;
+        else al = a, run = true;
+      }
+    } else {
+      const double t5[5] = {tot[0], tot[1], tot[2], tot[3], tot[4]};
+      st = scalars_after_b(st, t5);
+      if (st.status == ST_RUN) {
+        om = st.omega;
+        be = st.beta;
+        xup = true;
+        const double up = (tot[5] - om * tot[6]) + be * (tot[7] - om * tot[8]);
+        const double a = st.rho / up;
+        if (up == 0.0 || !isfinite(a)) st.status = ST_BRK_OMEGA;   // after iteration k-1's x update
+        else al = a, run = true;
+      } else if (st.status >= ST_P_CONV) {   // final x update of a finished run
+        xup = true;
+        half = st.status == ST_P_HALF;
+        om = st.omega;
+        if (half) st.half = 1;
+        st.status = finalize(st.status);
+      }
+    }
+  }
+  st.phase = PH_F;
+  if (!run) {
+    if (xup) {
+#pragma unroll
+      for (int k = 0; k < RY; ++k) {
+        if (ok[k]) {
+          const double xg = q.x[og[k]] + alpha_old * rp[k];
+          q.x[og[k]] = half ? xg : xg + om * rs[k];
+        }
+      }
+    }
+    if (writer) *Sout = st;
+    return;
+  }
+  st.alpha = al;
+  // r_{k-1} and p_k on the halo tile
+  double pr[RY], rr[RY];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    if (k <= RY || tid < NEDGE) {
+      const double r = rs[k] - om * rt_[k];
+      const double p = r + be * (rp[k] - om * rv[k]);
+      L.P[hr[k] * H2C + hc[k]] = p;
+      if (hr[k] >= 1 && hr[k] <= TY + 2 && hc[k] >= 1 && hc[k] <= TX + 2) L.R[(hr[k] - 1) * H1C + hc[k] - 1] = r;
+      if (k < RY) {
+        pr[k] = p;
+        rr[k] = r;
+      }
+    }
+  }
+  // iteration k-1's x update is applied at the end of the launch; x, rt and
+  // u arrive during the two products
+  double ox[RY], ort[RY], ou[RY];
+#pragma unroll
+  for (int k = 0; k < RY; ++k) {
+    ox[k] = q.x[og[k]];
+    ort[k] = q.rt[og[k]];
+    ou[k] = q.u[og[k]];
+  }
+  __syncthreads();
+  // v_k = A p_k: cell fluxes, then the own nodes and the ring
+#pragma unroll
+  for (int c = 0; c < NCP; ++c) {
+    const int k = tid + c * NTHR;
+    if (k < NC1) {
+      const int cy = k / C1C, cx = k - cy * C1C;
+      const double* sp = L.P + cy * H2C + cx;
+      const double tl = sp[0], tr = sp[1], bl = sp[H2C], br = sp[H2C + 1];
+      const double gx = (tr - tl) + (br - bl), gy = (bl - tl) + (br - tr);
+      double fx = wa[c] * gx, fy = wb[c] * gy;
+      if (CROSS) {
+        fx += wc[c] * gy;
+        fy += wd[c] * gx;
+      }
+      L.FX[k] = fx;
+      L.FY[k] = fy;
+    }
+  }
+  __syncthreads();
+  auto gather = [&](int a, int b) {   // flux sum of H1 node (a, b)
+    const int c0 = a * C1C + b, c1 = c0 + C1C;
+    return ((L.FX[c0 + 1] + L.FX[c1 + 1]) - (L.FX[c0] + L.FX[c1])) +
+           ((L.FY[c1] + L.FY[c1 + 1]) - (L.FY[c0] + L.FY[c0 + 1]));
+  };
+  double vv[RY], ss[RY];
+#pragma unroll
+  for (int k = 0; k < RY; ++k) {
+    const int a = 1 + ty + TYT * k, b = 1 + tx;
+    double acc = gather(a, b);
+    if (CENTER) acc += hv[k] * pr[k];
+    vv[k] = dv[k] * acc;
+    ss[k] = rr[k] - al * vv[k];
+    L.S[a * H1C + b] = ss[k];
+  }
+  if (ring) {
+    double acc = gather(ra, rb);
+    if (CENTER) acc += rhc * L.P[(ra + 1) * H2C + rb + 1];
+    L.S[ra * H1C + rb] = L.R[ra * H1C + rb] - al * (rdv * acc);
+  }
+  __syncthreads();
+  // t_k = A s_k: fluxes of the cells inside the ring, then the own nodes
+#pragma unroll
+  for (int c = 0; c < NCP; ++c) {
+    const int k = tid + c * NTHR;
+    if (k < NC1) {
+      const int cy = k / C1C, cx = k - cy * C1C;
+      if (cy >= 1 && cy <= TY + 1 && cx >= 1 && cx <= TX + 1) {
+        const double* sp = L.S + (cy - 1) * H1C + (cx - 1);
+        const double tl = sp[0], tr = sp[1], bl = sp[H1C], br = sp[H1C + 1];
+        const double gx = (tr - tl) + (br - bl), gy = (bl - tl) + (br - tr);
+        double fx = wa[c] * gx, fy = wb[c] * gy;
+        if (CROSS) {
+          fx += wc[c] * gy;
+          fy += wd[c] * gx;
+        }
+        L.FX[k] = fx;
+        L.FY[k] = fy;
+      }
+    }
+  }
+  __syncthreads();
+  double acc[NFREC];
+#pragma unroll
+  for (int k = 0; k < NFREC; ++k) acc[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < RY; ++k) {
+    const int a = 1 + ty + TYT * k, b = 1 + tx;
+    double y = gather(a, b);
+    if (CENTER) y += hv[k] * ss[k];
+    const double t = dv[k] * y;
+    if (ok[k]) {
+      const unsigned g = og[k];
+      if (xup) __builtin_nontemporal_store((ox[k] + alpha_old * rp[k]) + om * rs[k], q.x + g);
+      __builtin_nontemporal_store(pr[k], q.p1 + g);
+      __builtin_nontemporal_store(vv[k], q.v1 + g);
+      __builtin_nontemporal_store(ss[k], q.s1 + g);
+      __builtin_nontemporal_store(t, q.t1 + g);
+      const double s = ss[k];
+      acc[0] += s * s;
+      acc[1] += t * s;
+      acc[2] += t * t;
+      acc[3] += ort[k] * s;
+      acc[4] += ort[k] * t;
+      acc[5] += ou[k] * s;
+      acc[6] += ou[k] * t;
+      acc[7] += ou[k] * pr[k];
+      acc[8] += ou[k] * vv[k];
+    }
+  }
+  write_partials<NFREC>(acc, sred, Pout, nb, bid);
+  if (writer) *Sout = st;
+}
+
+// ---------------------------------------------------------------------------
 // FP64 true residual r = b - A x with partials of r.r, and a one-block sum
 // ---------------------------------------------------------------------------
 struct TrIn {
@@ -691,6 +1132,12 @@ int alloc_vecs(pb_op2d* op, Work<T>& w, int nb) {
   rc |= dmalloc(op, &w.PA, nb);
   rc |= dmalloc(op, &w.PB, 5L * nb);
   rc |= dmalloc(op, &w.PC, 2L * nb);
+  if (std::is_same<T, double>::value) {   // one-launch iteration (FP64 only)
+    rc |= dmalloc(op, &w.t2, n);
+    rc |= dmalloc(op, &w.u, n);
+    rc |= dmalloc(op, &w.PF[0], (long)NFREC * nb);
+    rc |= dmalloc(op, &w.PF[1], (long)NFREC * nb);
+  }
   rc |= dmalloc(op, &w.S, 3);
   return rc;
 }
@@ -796,16 +1243,62 @@ void launch_l3(pb_op2d* op, Work<T>& w, int k, hipStream_t s) {
 #undef PB_L3_AT
 }
 
+// iteration variants of a Krylov run
+enum { VAR_FAST = 0, VAR_EXACT = 1, VAR_FUSED = 2 };
+
+// tile height of the one-launch iteration: 5 rows per thread (20-row tiles;
+// 209-249 VGPRs, two waves per SIMD, no spills)
+constexpr int kFusedRY = 5;
+inline int fused_ry(const pb_op2d*) { return kFusedRY; }
+
+#define PB_FUSED_RY(KERNEL, RYV, grd, s, ...)                                                     \
+  do {                                                                                          \
+    const dim3 _blk(TX, TYT);                                                                   \
+    if (_c && _h) hipLaunchKernelGGL((KERNEL<true, true, RYV>), grd, _blk, 0, s, __VA_ARGS__);  \
+    else if (_c) hipLaunchKernelGGL((KERNEL<true, false, RYV>), grd, _blk, 0, s, __VA_ARGS__);  \
+    else if (_h) hipLaunchKernelGGL((KERNEL<false, true, RYV>), grd, _blk, 0, s, __VA_ARGS__);  \
+    else hipLaunchKernelGGL((KERNEL<false, false, RYV>), grd, _blk, 0, s, __VA_ARGS__);         \
+  } while (0)
+#define PB_FUSED_DISPATCH(KERNEL, op, grd, s, ...)                         \
+  do {                                                                    \
+    const bool _c = (op)->cross, _h = (op)->center;                       \
+    PB_FUSED_RY(KERNEL, kFusedRY, grd, s, __VA_ARGS__);                   \
+  } while (0)
+
+// u = A^T r~ after the init launch (reads S0 and the init records in PB,
+// writes S1 and the records PF[0])
+void launch_adj(pb_op2d* op, Work<double>& w, hipStream_t s) {
+  const dim3 grd = stencil_grid(op, fused_ry(op));
+  const OpDev<double> d = opdev<double>(op);
+  const int px = op->g.bc_x == PB_BC_PERIODIC, py = op->g.bc_y == PB_BC_PERIODIC;
+  PB_FUSED_DISPATCH(k_adj_f, op, grd, s, d, (const double*)w.rt, w.u, px, py, (const KState*)w.S, w.S + 1,
+                    (const double*)w.PB, w.PF[0]);
+}
+
+// F(k): reads the vectors of set (k-1)&1, state S[k&1] and records PF[(k-1)&1];
+// writes set k&1, S[(k+1)&1] and PF[k&1] (s lives in r, t in t / t2)
+void launch_f(pb_op2d* op, Work<double>& w, int k, hipStream_t s) {
+  const dim3 grd = stencil_grid(op, fused_ry(op));
+  const OpDev<double> d = opdev<double>(op);
+  const int a = (k - 1) & 1, b = k & 1;
+  FPtrs q{w.r[a], a ? w.t2 : w.t, w.p[a], w.v[a], w.r[b], b ? w.t2 : w.t, w.p[b], w.v[b], w.rt, w.u, w.x};
+  PB_FUSED_DISPATCH(k_iter_f, op, grd, s, d, q, (const KState*)(w.S + (k & 1)), w.S + ((k + 1) & 1),
+                    (const double*)w.PF[a], w.PF[b]);
+}
+
 // chunk of c iterations starting at an odd k (the buffer parity pattern
-// repeats every two iterations).  fast: B(k), A(k+1); exact: L2(k), L3(k), L1(k+1)
+// repeats every two iterations).  fast: B(k), A(k+1); exact: L2(k), L3(k),
+// L1(k+1); fused: F(k+1)
 template <typename T>
-void launch_chunk(pb_op2d* op, Work<T>& w, int c, bool exact, hipStream_t s) {
+void launch_chunk(pb_op2d* op, Work<T>& w, int c, int variant, hipStream_t s) {
   for (int i = 0; i < c; ++i) {
     const int k = 1 + i;
-    if (exact) {
+    if (variant == VAR_EXACT) {
       launch_b<T>(op, w, k, s, 0, 1, true);
       launch_l3<T>(op, w, k, s);
       launch_l1<T>(op, w, k + 1, s);
+    } else if (variant == VAR_FUSED) {
+      if constexpr (std::is_same<T, double>::value) launch_f(op, w, k + 1, s);
     } else {
       launch_b<T>(op, w, k, s);
       launch_a<T>(op, w, k + 1, s);
@@ -819,17 +1312,17 @@ bool graphs_enabled() {
 }
 
 template <typename T>
-int run_chunk(pb_op2d* op, Work<T>& w, int c, bool exact, hipStream_t s) {
+int run_chunk(pb_op2d* op, Work<T>& w, int c, int variant, hipStream_t s) {
   if (graphs_enabled() && s != nullptr) {
     // captured launches are specialised on the tile height and on the
     // operator's (cross, center) flags, which a re-assembly may change
-    const int key = (((c * 2 + (exact ? 1 : 0)) * 16 + op->ry) * 2 + (op->cross ? 1 : 0)) * 2 + (op->center ? 1 : 0);
+    const int key = (((c * 3 + variant) * 16 + op->ry) * 2 + (op->cross ? 1 : 0)) * 2 + (op->center ? 1 : 0);
     auto it = w.graphs.find(key);
     if (it == w.graphs.end()) {
       hipGraph_t gr = nullptr;
       hipGraphExec_t ex = nullptr;
       PB_HIP(op->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      launch_chunk<T>(op, w, c, exact, s);
+      launch_chunk<T>(op, w, c, variant, s);
       PB_HIP(op->ctx, hipStreamEndCapture(s, &gr));
       PB_HIP(op->ctx, hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
       (void)hipGraphDestroy(gr);
@@ -837,7 +1330,7 @@ int run_chunk(pb_op2d* op, Work<T>& w, int c, bool exact, hipStream_t s) {
     }
     PB_HIP(op->ctx, hipGraphLaunch(it->second, s));
   } else {
-    launch_chunk<T>(op, w, c, exact, s);
+    launch_chunk<T>(op, w, c, variant, s);
   }
   PB_LAUNCH_CHECK(op->ctx);
   return PB_OK;
@@ -852,8 +1345,11 @@ __global__ void k_put_state(KState* dst, KState v) { *dst = v; }
 // buffer the chunks are launched directly, so the run allocates nothing.
 template <typename T>
 int krylov_run(pb_op2d* op, Work<T>& w, const T* b, const double* x0, double tol, int maxiter, int legacy,
-               int check_every, hipStream_t s, KState* out, bool exact = false, KState* host = nullptr) {
+               int check_every, hipStream_t s, KState* out, int variant = VAR_FAST, KState* host = nullptr) {
   pb_ctx* ctx = op->ctx;
+  if (variant == VAR_FUSED && !std::is_same<T, double>::value)
+    return pb_fail(ctx, PB_ERR_ARG, "bicgstab: the one-launch iteration is FP64 only");
+  const bool exact = variant == VAR_EXACT, fused = variant == VAR_FUSED;
   KState* hs = host ? host : op->hstate;
   KState init{};
   init.tol = tol;
@@ -865,17 +1361,29 @@ int krylov_run(pb_op2d* op, Work<T>& w, const T* b, const double* x0, double tol
   // record travels as a kernel argument
   hipLaunchKernelGGL(k_put_state, dim3(1), dim3(1), 0, s, w.S + (exact ? 2 : 0), init);
   {
-    const dim3 grd = stencil_grid(op, op->ry);
+    // the one-launch iteration's adjoint launch sums the init records over
+    // its own tiling
+    const int iry = fused ? fused_ry(op) : op->ry;
+    const dim3 grd = stencil_grid(op, iry);
     const OpDev<T> d = opdev<T>(op);
     InitIn<T> in{x0, x0 != nullptr};
     InitOut<T> o{b, w.r[0], w.rt, w.p[0], w.v[0], w.t, w.x, x0, x0 != nullptr, 0.0, 0.0};
-    if (exact) PB_DISPATCH(k_init2, T, op, grd, s, d, in, o, w.PC);
-    else PB_DISPATCH(k_init5, T, op, grd, s, d, in, o, w.PB);
+    const bool _c = op->cross, _h = op->center;
+    if (exact) PB_RY_SWITCH(iry, PB_DISPATCH_AT, k_init2, T, grd, s, d, in, o, w.PC);
+    else PB_RY_SWITCH(iry, PB_DISPATCH_AT, k_init5, T, grd, s, d, in, o, w.PB);
   }
-  if (exact) launch_l1<T>(op, w, 1, s);
-  else launch_a<T>(op, w, 1, s);
+  if (exact) {
+    launch_l1<T>(op, w, 1, s);
+  } else if (fused) {
+    if constexpr (std::is_same<T, double>::value) {
+      launch_adj(op, w, s);
+      launch_f(op, w, 1, s);
+    }
+  } else {
+    launch_a<T>(op, w, 1, s);
+  }
   PB_LAUNCH_CHECK(ctx);
-  const int chk = exact ? 0 : 1;   // slot written by the iteration's first launch
+  const int chk = exact ? 0 : (fused ? 0 : 1);   // slot written by the iteration's first launch
   const int chunk = std::max(2, (check_every > 0 ? check_every : 32) & ~1);
   long issued = 1;
   for (;;) {
@@ -890,10 +1398,10 @@ int krylov_run(pb_op2d* op, Work<T>& w, const T* b, const double* x0, double tol
       return pb_fail(ctx, PB_ERR_STATE, "bicgstab: %ld iterations issued without a final state (status %d)",
                      issued, h.status);
     if (host) {
-      launch_chunk<T>(op, w, chunk, exact, s);
+      launch_chunk<T>(op, w, chunk, variant, s);
       PB_LAUNCH_CHECK(ctx);
     } else {
-      int rc = run_chunk<T>(op, w, chunk, exact, s);
+      int rc = run_chunk<T>(op, w, chunk, variant, s);
       if (rc) return rc;
     }
     issued += chunk;
@@ -1026,11 +1534,18 @@ extern "C" int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x, const 
   const long n = op->n;
   const int nb = elem_blocks(n);
   pb_solve_info out{};
-  if (opts->mode == PB_SOLVE_F64 || opts->mode == PB_SOLVE_F64_FAST) {
+  if (opts->mode == PB_SOLVE_F64 || opts->mode == PB_SOLVE_F64_FAST || opts->mode == PB_SOLVE_F64_FUSED) {
     KState h{};
+    const int var = opts->mode == PB_SOLVE_F64 ? VAR_EXACT : (opts->mode == PB_SOLVE_F64_FUSED ? VAR_FUSED : VAR_FAST);
     rc = krylov_run<double>(op, op->w64, b, opts->x0_given ? x : nullptr, opts->tol, opts->maxiter,
-                            opts->x0_given ? 0 : 1, opts->check_every, s, &h, opts->mode == PB_SOLVE_F64);
+                            opts->x0_given ? 0 : 1, opts->check_every, s, &h, var);
     if (rc) return rc;
+    if (h.status == ST_SEAM) {   // duplicated end nodes differ: the two-launch iteration
+      out.fallback = 1;
+      rc = krylov_run<double>(op, op->w64, b, opts->x0_given ? x : nullptr, opts->tol, opts->maxiter,
+                              opts->x0_given ? 0 : 1, opts->check_every, s, &h, VAR_FAST);
+      if (rc) return rc;
+    }
     PB_HIP(ctx, hipMemcpyAsync(x, op->w64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     out.info = info_of(h);
     out.iters = h.iter;
@@ -1095,7 +1610,7 @@ extern "C" int pb_bicgstab_solve(pb_op2d* op, const double* b, double* x, const 
       int left = fmax;
       for (int restart = 0; restart < 8 && left > 0; ++restart) {
         KState h{};
-        rc = krylov_run<double>(op, op->w64, b, x, opts->tol, left, 0, opts->check_every, s, &h, false);
+        rc = krylov_run<double>(op, op->w64, b, x, opts->tol, left, 0, opts->check_every, s, &h, VAR_FAST);
         if (rc) return rc;
         PB_HIP(ctx, hipMemcpyAsync(x, op->w64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         out.iters += h.iter;
@@ -1187,7 +1702,7 @@ extern "C" int pb_bicgstab_solve_noalloc(pb_op2d* op, const double* b, double* x
   KState h{};
   const bool exact = opts->mode == PB_SOLVE_F64;
   int rc = krylov_run<double>(op, w, b, opts->x0_given ? x : nullptr, opts->tol, opts->maxiter,
-                              opts->x0_given ? 0 : 1, opts->check_every, s, &h, exact, hbuf);
+                              opts->x0_given ? 0 : 1, opts->check_every, s, &h, exact ? VAR_EXACT : VAR_FAST, hbuf);
   if (rc) return rc;
   pb_solve_info out{};
   out.info = info_of(h);
@@ -1217,10 +1732,10 @@ int time_kernel_t(pb_op2d* op, int kernel, int reps, hipStream_t s, double* avg_
   for (int k = 0; k < 9; ++k) hipLaunchKernelGGL(k_fill_pattern<T>, dim3(nb), dim3(256), 0, s, n, vecs[k], 0.7 * k);
   hipLaunchKernelGGL(k_fill_pattern<double>, dim3(nb), dim3(256), 0, s, n, w.x, 0.1);
   // partials that keep the state running: B totals (1, .5, 1, .3, .1), A total 1
-  std::vector<double> pb(5L * nbs), pa(nbs, 1.0 / nbs);   // record layout [block][value]
+  std::vector<double> pb(5L * nbs), pa(nbs, 1.0 / nbs);   // record layout [value][block]
   const double tb[5] = {1.0, 0.5, 1.0, 0.3, 0.1};
   for (int k = 0; k < 5; ++k)
-    for (int b = 0; b < nbs; ++b) pb[(long)b * 5 + k] = tb[k] / nbs;
+    for (int b = 0; b < nbs; ++b) pb[(long)k * nbs + b] = tb[k] / nbs;
   PB_HIP(ctx, hipMemcpyAsync(w.PB, pb.data(), sizeof(double) * pb.size(), hipMemcpyHostToDevice, s));
   PB_HIP(ctx, hipMemcpyAsync(w.PA, pa.data(), sizeof(double) * pa.size(), hipMemcpyHostToDevice, s));
   KState st{};
@@ -1670,7 +2185,7 @@ int batch_graph(pb_batch* bt, int chunk, hipStream_t s, hipGraphExec_t* out) {
     PB_HIP(bt->ctx, hipEventRecord(bt->fork, s));
     for (size_t m = 0; m < bt->ops.size(); ++m) {
       PB_HIP(bt->ctx, hipStreamWaitEvent(bt->streams[m], bt->fork, 0));
-      launch_chunk<T>(bt->ops[m], work_of<T>(bt->ops[m]), chunk, false, bt->streams[m]);
+      launch_chunk<T>(bt->ops[m], work_of<T>(bt->ops[m]), chunk, VAR_FAST, bt->streams[m]);
       PB_HIP(bt->ctx, hipEventRecord(bt->joins[m], bt->streams[m]));
       PB_HIP(bt->ctx, hipStreamWaitEvent(s, bt->joins[m], 0));
     }
@@ -1737,7 +2252,7 @@ int krylov_batch(pb_batch* bt, const std::vector<char>& active, const std::vecto
     if (graphs) {
       PB_HIP(bt->ctx, hipGraphLaunch(ex, s));
     } else {
-      for (size_t m = 0; m < M; ++m) launch_chunk<T>(bt->ops[m], work_of<T>(bt->ops[m]), chunk, false, s);
+      for (size_t m = 0; m < M; ++m) launch_chunk<T>(bt->ops[m], work_of<T>(bt->ops[m]), chunk, VAR_FAST, s);
     }
     PB_LAUNCH_CHECK(bt->ctx);
     issued += chunk;

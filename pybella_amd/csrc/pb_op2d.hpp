@@ -197,13 +197,16 @@ struct EarlyPartials {
     for (int h = 0; h < 2; ++h) {
       const int b = min(tid + h * NTHR, nb - 1);
 #pragma unroll
-      for (int k = 0; k < NV; ++k) v[h][k] = P[(long)b * NV + k];
+      for (int k = 0; k < NV; ++k) v[h][k] = P[(long)k * nb + b];
     }
   }
 };
 
-// Sum ``nb`` block partials of NV values (record layout [b * NV + k]) in a
-// fixed order; the result is valid in every thread.  sred: NV * (NTHR/64).
+// Sum ``nb`` block partials of NV values in a fixed order; the result is
+// valid in every thread.  sred: NV * (NTHR/64).  Records are stored value
+// by value, [k * nb + b] (written by write_partials with the same nb), so
+// the 64 lanes of a load read 64 consecutive words: with [b * NV + k] a
+// five- or nine-value record set spreads one load over 20-36 cache lines.
 // ``early`` holds records tid and tid + NTHR (EarlyPartials::load on the same
 // P and nb); thread tid adds its records b = tid, tid + NTHR, ... in order.
 template <int NV>
@@ -221,7 +224,7 @@ __device__ __forceinline__ void sum_partials(const EarlyPartials<NV>& early, con
   }
   for (int b = tid + 2 * NTHR; b < nb; b += NTHR) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) tot[k] += P[(long)b * NV + k];
+    for (int k = 0; k < NV; ++k) tot[k] += P[(long)k * nb + b];
   }
   // wave sums into sred, then every thread adds them in wave order (the same
   // order in every thread and block); sred is next written after the tile's
@@ -250,15 +253,15 @@ __device__ __forceinline__ void sum_partials(const double* P, int nb, double (&t
   sum_partials<NV>(early, P, nb, tot, sred);
 }
 
-// Block sum of NV thread values, written as this block's record (plain stores).
+// Block sum of NV thread values, written as this block's record (plain
+// stores, value k at P[k * nb + bid]).
 template <int NV>
 __device__ __forceinline__ void write_partials(double (&v)[NV], double* sred, double* P, int nb, int bid) {
   block_sum_w<NV, NTHR / kWave>(v, sred);
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  (void)nb;
   if (tid == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) P[(long)bid * NV + k] = v[k];
+    for (int k = 0; k < NV; ++k) P[(long)k * nb + bid] = v[k];
   }
 }
 
@@ -307,11 +310,15 @@ enum {
   ST_P_MAXIT = 12,
   ST_P_BRK_RHO = 13,
   ST_P_BRK_OMEGA = 14,
+  // final, one-launch iteration only: the two copies of a duplicated periodic
+  // end node differ in the initial residual or the operator, so the solve
+  // runs the two-launch iteration instead
+  ST_SEAM = -20,
 };
-enum { PH_INIT = 0, PH_A = 1, PH_B = 2, PH_L3 = 3 };
+enum { PH_INIT = 0, PH_A = 1, PH_B = 2, PH_L3 = 3, PH_F = 4 };
 
 __host__ __device__ inline bool st_final(int s) {
-  return s == ST_CONV || s == ST_MAXIT || s == ST_BRK_RHO || s == ST_BRK_OMEGA;
+  return s == ST_CONV || s == ST_MAXIT || s == ST_BRK_RHO || s == ST_BRK_OMEGA || s == ST_SEAM;
 }
 
 template <typename T>
@@ -326,6 +333,11 @@ struct Work {
   double* PA = nullptr;    // partials of the A launch (1 value)
   double* PB = nullptr;    // partials of the init / B launch (<= 5 values)
   double* PC = nullptr;    // partials of the exact-mode update launch (2 values)
+  // one-launch iteration (FP64 only): the second t buffer (s, p, v use r, p,
+  // v), u = A^T r~, and its partials (10 values), double-buffered
+  T* t2 = nullptr;
+  T* u = nullptr;
+  double* PF[2] = {nullptr, nullptr};
   KState* S = nullptr;     // [3]
   std::map<int, hipGraphExec_t> graphs;   // (variant, chunk) key -> instantiated graph
 };

@@ -342,13 +342,14 @@ __device__ __forceinline__ void write_partials3(double (&acc)[NV], double* part)
   block_sum_w<NV, NW>(acc, sh);
   if (tid3() == 0)
 #pragma unroll
-    for (int v = 0; v < NV; ++v) part[(long)bid3() * NV + v] = acc[v];
+    for (int v = 0; v < NV; ++v) part[(long)v * nblk3() + bid3()] = acc[v];
 }
 
 // threads of the one-block scalar kernels: one record per thread up to 1024 blocks
 constexpr int NRT = 1024;
 
-// One-block reduction of the partial records [b][NV] (NRT threads): thread t
+// One-block reduction of the partial records (value v of block b at
+// part[v * nb + b], so a wave's loads are contiguous) (NRT threads): thread t
 // adds records t, t + NRT, ... in order, then the block sum takes the wave
 // sums and the per-wave values in wave order.  The first record is loaded
 // unconditionally (clamped to the last one) and zeroed after, so its load is
@@ -358,7 +359,7 @@ template <int NV>
 __device__ __forceinline__ void first_record(const double* part, int nb, double (&out)[NV]) {
   const int b0 = min((int)threadIdx.x, nb - 1);
 #pragma unroll
-  for (int v = 0; v < NV; ++v) out[v] = part[(long)b0 * NV + v];
+  for (int v = 0; v < NV; ++v) out[v] = part[(long)v * nb + b0];
 }
 
 template <int NV>
@@ -370,7 +371,7 @@ __device__ __forceinline__ void sum_records(const double* part, int nb, double (
     for (int v = 0; v < NV; ++v) out[v] = 0.0;
   for (int b = t + NRT; b < nb; b += NRT)
 #pragma unroll
-    for (int v = 0; v < NV; ++v) out[v] += part[(long)b * NV + v];
+    for (int v = 0; v < NV; ++v) out[v] += part[(long)v * nb + b];
   block_sum_w<NV, NRT / kWave>(out, sh);
 }
 
@@ -511,10 +512,11 @@ __device__ __forceinline__ void fold2(KS3& k, const double (&s)[5]) {
 }
 
 // Every block of a marching launch reduces the previous launch's partial
-// records [b][NV] itself, all in the same fixed order (thread t adds records
-// t, t + NT3, ...; wave sums; the NT3/64 wave values in order), so every
-// block holds bitwise the same sums.  A thread's first two records are
-// loaded at kernel entry (indices clamped, loads unconditional).
+// records (value q of block b at P[q * nb + b]) itself, all in the same
+// fixed order (thread t adds records t, t + NT3, ...; wave sums; the NT3/64
+// wave values in order), so every block holds bitwise the same sums.  A
+// thread's first two records are loaded at kernel entry (indices clamped,
+// loads unconditional).
 template <int NV>
 struct Rec3 {
   double v[2][NV];
@@ -524,7 +526,7 @@ struct Rec3 {
     for (int h = 0; h < 2; ++h) {
       const int b = min(t + h * NT3, nb - 1);
 #pragma unroll
-      for (int q = 0; q < NV; ++q) v[h][q] = P[(long)b * NV + q];
+      for (int q = 0; q < NV; ++q) v[h][q] = P[(long)q * nb + b];
     }
   }
 };
@@ -544,7 +546,7 @@ __device__ __forceinline__ void sum_rec3(const Rec3<NV>& e, const double* P, int
       for (int q = 0; q < NV; ++q) s[q] += e.v[h][q];
   for (int b = t + 2 * NT3; b < nb; b += NT3)
 #pragma unroll
-    for (int q = 0; q < NV; ++q) s[q] += P[(long)b * NV + q];
+    for (int q = 0; q < NV; ++q) s[q] += P[(long)q * nb + b];
   wave_sum_n<NV>(s);
   if ((t & (kWave - 1)) == 0)
 #pragma unroll
@@ -559,9 +561,11 @@ __device__ __forceinline__ void sum_rec3(const Rec3<NV>& e, const double* P, int
   }
 }
 
-// z planes per block of launch 1 (FP64 marches 24: see k3_iter1)
+// z planes per block of launch 1: 24 in both precisions (FP64: the grid
+// fits one round of blocks; FP32: 48.7 against 51.3 us with 16-plane chunks
+// at 257x129x129, the extra plane per chunk being the smaller share)
 template <typename T>
-constexpr int kZ1 = sizeof(T) == 4 ? Z3 : 24;
+constexpr int kZ1 = 24;
 
 // launch 1's node operand: r, p, v, t raw, p_k = (r - alpha v - omega t) +
 // beta (p - omega v) combined
@@ -606,8 +610,7 @@ __global__ __launch_bounds__(NT3) void k3_iter1(Op3<T> o, const KS3* Sin, KS3* S
   };
   // The node plane's r, p, v, t are read once: the operand p_k is combined
   // from them for the stencil, and the tile's own nodes keep them in LDS for
-  // the updates.  Node operands one step ahead, over 24-plane chunks (FP64)
-  // so the grid fits one round of blocks.
+  // the updates.  Node operands one step ahead, over 24-plane chunks.
   march3<T, 1, 2, false, true, kZ1<T>, true>(
       o, Iter1In<T>{va, ra, pa, t, al, om, be}, [&](long g) { return Ops{rt[g], x[g]}; },
       [&](long g, T y, T pn, const Ops& e, const typename Iter1In<T>::Raw& w, double(&a)[1]) {
@@ -1202,9 +1205,9 @@ extern "C" int pb_op3d_time_kernel(pb_op3d* op, int kernel, int precision, int r
   *op->hst = k;
   PB_HIP(op->ctx, hipMemcpyAsync(op->st, op->hst, sizeof(KS3), hipMemcpyHostToDevice, s));
   std::vector<double> rec5((size_t)nbm * kNV), rec1((size_t)nbm, 1.0);
-  for (int b = 0; b < nbm; ++b) {
+  for (int b = 0; b < nbm; ++b) {   // value-major records, as write_partials3 stores them
     const double pat[kNV] = {1.0, 0.5, 1.0, 1.0, 0.5};
-    for (int q = 0; q < kNV; ++q) rec5[(size_t)b * kNV + q] = pat[q];
+    for (int q = 0; q < kNV; ++q) rec5[(size_t)q * nbm + b] = pat[q];
   }
   PB_HIP(op->ctx, hipMemcpyAsync(op->part, rec5.data(), sizeof(double) * rec5.size(), hipMemcpyHostToDevice, s));
   PB_HIP(op->ctx, hipMemcpyAsync(op->part1, rec1.data(), sizeof(double) * rec1.size(), hipMemcpyHostToDevice, s));

@@ -30,8 +30,10 @@ total_calls = 0
 
 # Solver settings used by the drop-in.  mode: "f64" (reference semantics:
 # FP64 BiCGStab, x0 = 0, legacy tolerance), "f64fast" (two launches per
-# iteration), "mixed" (FP32 Krylov in FP64 iterative refinement, stop on the
-# FP64 true residual), "nn" (the learned preconditioner: the network attached
+# iteration), "f64fused" (one launch per iteration; "f64fast" when the
+# duplicated periodic end nodes differ), "mixed" (FP32 Krylov in FP64
+# iterative refinement, stop on the FP64 true residual), "nn" (the learned
+# preconditioner: the network attached
 # to the grid's operator with operator_for(node, ud).attach_nn(net) solves the
 # coarse level of the multigrid cycle that preconditions flexible GMRES, for
 # at most nn_budget iterations with nn_sweeps Jacobi sweeps each way per
@@ -40,7 +42,7 @@ solver_options = {"mode": "f64", "inner_tol": 1e-3, "max_outer": 12, "check_ever
                   "nn_budget": 300, "nn_sweeps": 2}
 
 _MODES = {"f64": _abi.PB_SOLVE_F64, "f64fast": _abi.PB_SOLVE_F64_FAST, "mixed": _abi.PB_SOLVE_MIXED,
-          "nn": _abi.PB_SOLVE_NN}
+          "nn": _abi.PB_SOLVE_NN, "f64fused": _abi.PB_SOLVE_F64_FUSED}
 
 
 def _torch():
@@ -155,7 +157,7 @@ class Operator2D:
     def solve(self, b, x=None, tol=1e-8, maxiter=6000, mode="f64", inner_tol=1e-3, max_outer=12,
               check_every=32, x0=None, fallback_maxiter=6000, krylov_m=30, sweeps=2, omega=0.8,
               use_network=True):
-        """mode: "f64" (reference semantics), "f64fast", "mixed" or "nn"
+        """mode: "f64" (reference semantics), "f64fast", "f64fused", "mixed" or "nn"
         (flexible GMRES preconditioned by the multigrid cycle whose coarse
         level the attached network solves; ``maxiter`` is then its iteration
         budget before plain BiCGStab takes over for up to ``fallback_maxiter``

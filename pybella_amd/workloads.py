@@ -70,7 +70,8 @@ def stratified_state(nx=1025, ny=513, theta0=300.0, theta1=330.0, compressible=F
 
 
 def fourier_rhs(node, seed, nmodes=64, kmax=16):
-    """Node field of shape (NX, NY) in the reference's [x, y] layout."""
+    """Node field of shape (NX, NY) in the reference's [x, y] layout.  The
+    last column duplicates the first (the periodic end node), bit for bit."""
     rng = np.random.default_rng(seed)
     ks = [(kx, ky) for kx in range(-kmax, kmax + 1) for ky in range(0, kmax + 1)
           if (kx, ky) != (0, 0) and kx * kx + ky * ky <= kmax * kmax]
@@ -86,6 +87,7 @@ def fourier_rhs(node, seed, nmodes=64, kmax=16):
         kx, ky = ks[k]
         f += amp[m] * np.cos(2 * np.pi * kx * (x - x[0]) / Lx + phi[m])[:, None] * \
             np.cos(np.pi * ky * (y - y[0]) / H)[None, :]
+    f[-1, :] = f[0, :]
     return f
 
 

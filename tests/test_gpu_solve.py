@@ -88,10 +88,11 @@ def test_converged_f64(nx, ny):
     assert np.linalg.norm(a - r) / np.linalg.norm(r) <= 2e-6
 
 
-@pytest.mark.parametrize("mode", ["f64", "nn"])
+@pytest.mark.parametrize("mode", ["f64", "nn", "f64fused"])
 def test_converged_parity_1025x513(mode):
-    """The metric's grid: the GPU solution (FP64 BiCGStab, or the learned
-    mode with its FP32 / BF16 preconditioner) at tolerance 1e-10 against the
+    """The metric's grid: the GPU solution (FP64 BiCGStab, its one-launch
+    iteration, or the learned mode with its FP32 / BF16 preconditioner) at
+    tolerance 1e-10 against the
     C restatement's BiCGStab (oracle/c/pb_oracle.c, serial, FP64) at 1e-10,
     modulo the null space: relative L2 <= 1e-8."""
     import torch
@@ -114,6 +115,8 @@ def test_converged_parity_1025x513(mode):
         if net is not None:
             op.attach_nn(None)
     assert info["info"] == 0 and info["true_rel_resid"] <= 1e-10, info
+    if mode == "f64fused":
+        assert info["fallback"] == 0, info   # the metric's RHS duplicates its end column
     a, r = strip_null(x), strip_null(xo.reshape(513, 1025))
     assert np.linalg.norm(a - r) / np.linalg.norm(r) <= 1e-8
 
@@ -304,7 +307,7 @@ def test_reassembly_with_other_operator_kind():
         return op.scale_rhs(sp.to_dev(W.stratified_rhs(node, seed=3)))
 
     shared = sp.Operator2D(sp.grid_of(A[3], A[4]))
-    for mode in ("f64", "f64fast", "mixed"):
+    for mode in ("f64", "f64fast", "f64fused", "mixed"):
         b = assemble(shared, A)
         shared.solve(b, tol=1e-10, mode=mode)
         b = assemble(shared, B)
