@@ -47,6 +47,7 @@ SOLVER_NAMES = {
     "mixed": "plain BiCGStab, FP32 Krylov (FP64 iterate) in FP64 iterative refinement",
     "f64": "plain BiCGStab FP64 (reference semantics, 3 launches/iteration)",
     "f64fast": "plain BiCGStab FP64 (rho, ||r|| from s/t dots, 2 launches/iteration)",
+    "f64fused": "plain BiCGStab FP64 (as f64fast; r~.v as u.p with u = A^T r~, 1 launch/iteration)",
     "nn": "learned preconditioner: FP64 flexible GMRES preconditioned by the FP32 two-branch multigrid cycle "
           "whose coarse level (65x33 nodes) the BF16 network solves; plain BiCGStab FP64 only past the budget",
 }
@@ -680,7 +681,7 @@ def only_run(args, op, phys, dt, rho, rhoY, dS, rhs_d, b, x):
     """Warm-up and K timed solves of one mode (assembly + scaling + solve per
     step, as in the default line), for profiler runs."""
     import torch
-    cfg = dict(mode="f64fast", maxiter=6000)
+    cfg = dict(mode=args.only if args.only in ("f64fast", "f64fused") else "f64fast", maxiter=6000)
     if args.only in ("nn", "mg"):
         from pybella_amd.nn import Network, default_weights_path
         net = Network(seed=0)
@@ -713,14 +714,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="auto", choices=["auto", "nn", "f64", "f64fast", "mixed"],
+    ap.add_argument("--mode", default="auto", choices=["auto", "nn", "f64", "f64fast", "f64fused", "mixed"],
                     help="auto = the learned-preconditioned solve timed beside the fastest plain mode")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle checks and the CPU baseline")
     ap.add_argument("--no-extra-workloads", action="store_true",
                     help="skip the 3D and batch summaries of the default line")
-    ap.add_argument("--only", default=None, choices=["nn", "f64fast", "mg"],
+    ap.add_argument("--only", default=None, choices=["nn", "f64fast", "f64fused", "mg"],
                     help="time only this solve of the 2D workload (no calibration; for profiler runs): the "
-                         "learned mode, plain BiCGStab, or the cycle continued to the coarsest level")
+                         "learned mode, plain BiCGStab (two launches or one per iteration), or the cycle "
+                         "continued to the coarsest level")
     ap.add_argument("--cpu-sample-iters", type=int, default=300)
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--kernel-only", type=int, default=None,
@@ -839,7 +841,7 @@ def main():
     info64 = step("f64")
     log("f64 (exact) solve:", info64)
     rates, tiles, infos_cal = {}, {}, {}
-    plain_modes = ["f64", "f64fast", "mixed"] if args.mode in ("auto", "nn") else [args.mode]
+    plain_modes = ["f64", "f64fast", "f64fused", "mixed"] if args.mode in ("auto", "nn") else [args.mode]
     for m in plain_modes:
         best, table = tune_tile(op, lambda: step(m), key=(grid.nx, grid.ny, m))
         infos_cal[m] = step(m)
