@@ -690,7 +690,7 @@ def only_run(args, op, phys, dt, rho, rhoY, dS, rhs_d, b, x):
         cfg = dict(mode="nn", maxiter=2000, sweeps=2, krylov_m=30, fallback_maxiter=6000,
                    use_network=True if args.only == "nn" else "mg")
     else:
-        op.set_tile(5)
+        op.set_tile(args.only_tile)
 
     def step():
         op.assemble(phys, dt, rho, rhoY, rho, rhoY, dS)
@@ -723,6 +723,7 @@ def main():
                     help="time only this solve of the 2D workload (no calibration; for profiler runs): the "
                          "learned mode, plain BiCGStab (two launches or one per iteration), or the cycle "
                          "continued to the coarsest level")
+    ap.add_argument("--only-tile", type=int, default=5, help="rows per thread of the plain solves under --only")
     ap.add_argument("--cpu-sample-iters", type=int, default=300)
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--kernel-only", type=int, default=None,

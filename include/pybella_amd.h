@@ -226,10 +226,13 @@ int pb_op2d_time_kernel(pb_op2d* op, int kernel, int precision, int reps, double
 
 /* Average device time (microseconds) of the A (kernel 1) or B (kernel 2)
  * launch of the PB_SOLVE_F64_FAST iteration inside a real solve of b to
- * tolerance tol (x0 = 0): every launch of that kernel in the live iterations
- * is bracketed by HIP events on the solve stream (iterations launched
- * directly, no graph); *launches = the number timed; x receives the
- * solution.  Synchronous.  Measurement aid; no reference counterpart. */
+ * tolerance tol (x0 = 0; iterations launched directly, no graph): at the
+ * start of each chunk of 32 iterations whose residual is above 100x the
+ * target, that launch is repeated 8 times back to back on the live state,
+ * in its place in the iteration, between HIP events on the solve stream
+ * (the repeats write identical outputs; A's in-place x update is saved and
+ * restored); *launches = the number timed; x receives the solution.
+ * Synchronous.  Measurement aid; no reference counterpart. */
 int pb_op2d_time_in_solve(pb_op2d* op, const double* b, double* x, double tol, int kernel,
                           double* avg_us, int* launches);
 

@@ -319,3 +319,24 @@ def test_reassembly_with_other_operator_kind():
         assert i1["info"] == 0 and i2["info"] == 0, (mode, i1, i2)
         assert i1["iters"] == i2["iters"] and torch.equal(x1, x2), mode
         fresh.close()
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_time_in_solve_leaves_the_solution(kernel):
+    """The in-solve timer repeats one launch of the two-launch iteration back
+    to back on the live state (A's in-place x update saved and restored): the
+    solution it returns is bitwise the fast mode's, and it timed launches."""
+    import torch
+    sp = _sp()
+    print(f"case time_in_solve kernel {kernel} 257x129", flush=True)
+    (Sol, mpv, elem, node, ud, th, dt), op_o, rhs, b = system(257, 129)
+    op = sp.operator_for(node, ud)
+    rho, rhoY = sp.to_dev(Sol.rho), sp.to_dev(Sol.rhoY)
+    op.assemble(sp.phys_of(ud, th), dt, rho, rhoY, rho, rhoY, sp.to_dev(sp.dSdy_cells(mpv, elem, node)))
+    bd = op.scale_rhs(sp.to_dev(rhs))
+    us, n, xt = op.time_in_solve(bd, kernel)
+    x, info = op.solve(bd, tol=1e-8, maxiter=6000, mode="f64fast", check_every=32)
+    torch.cuda.synchronize()
+    assert info["info"] == 0
+    assert n > 0 and n % 8 == 0 and 0.0 < us < 1e4, (us, n)
+    assert torch.equal(xt, x)
