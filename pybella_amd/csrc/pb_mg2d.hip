@@ -231,8 +231,10 @@ struct ProlongIn {
   int nxd;
   float dgain;      // 1: add the checkerboard branch, 0: drop it
   __device__ Raw load(long g) const {
-    const int j = (int)(g / nxd), i = (int)(g - (long)j * nxd);
-    const int ju = j % u.ny, iu = i % u.nx;
+    // 32-bit division (the grid holds < 2^30 nodes); the operator grid has
+    // at most one duplicated end node past the unique grid on each axis
+    const int j = (int)g / nxd, i = (int)g - j * nxd;
+    const int ju = j >= u.ny ? j - u.ny : j, iu = i >= u.nx ? i - u.nx : i;
     const int J0 = ju >> 1, I0 = iu >> 1;
     const int J1 = (ju & 1) ? (J0 + 1 == c.ny && c.py ? 0 : J0 + 1) : J0;
     const int I1 = (iu & 1) ? (I0 + 1 == c.nx && c.px ? 0 : I0 + 1) : I0;
@@ -310,7 +312,7 @@ struct Nb {
 };
 
 __device__ __forceinline__ void load_row(const Dims& d, const float* st, long g, Nb& nb) {
-  const int j = (int)(g / d.nx), i = (int)(g - (long)j * d.nx);
+  const int j = (int)g / d.nx, i = (int)g - j * d.nx;   // levels hold < 2^31 nodes: 32-bit division
   nbr9(d, j, i, nb.idx);
   const long p = (long)d.ny * d.nx;
 #pragma unroll
@@ -454,7 +456,8 @@ __device__ __forceinline__ float interp(const Dims& c, const float* e, int j, in
                   (e[(long)J1 * c.nx + I0] + e[(long)J1 * c.nx + I1]));
 }
 
-// z' = z + P e, then out = z' + omega (v - A z') / d
+// z' = z + P e, then out = z' + omega (v - A z') / d.  The neighbours' rows
+// and columns come from the same wrap as their flat indices (no division).
 __global__ void kc_prolong_sweep(Dims d, Dims c, const float* st, const float* idg, const float* v, const float* z,
                                  const float* e, float omega, float* out) {
   const long n = (long)d.ny * d.nx, pc = (long)c.ny * c.nx;
@@ -463,6 +466,14 @@ __global__ void kc_prolong_sweep(Dims d, Dims c, const float* st, const float* i
   Nb nb;
   load_row(d, st, g, nb);
   const float ig = idg[g];
+  const int j = (int)g / d.nx, i = (int)g - j * d.nx;
+  int jn[3], in3[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    bool ok;
+    jn[a] = axis_idx(j + a - 1, d.ny, d.py, ok);
+    in3[a] = axis_idx(i + a - 1, d.nx, d.px, ok);
+  }
 #pragma unroll
   for (int ch = 0; ch < 2; ++ch) {
     const float* zc = z + ch * n;
@@ -471,7 +482,7 @@ __global__ void kc_prolong_sweep(Dims d, Dims c, const float* st, const float* i
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const int q = nb.idx[k];
-      const int jq = q / d.nx, iq = q - jq * d.nx;
+      const int jq = jn[k / 3], iq = in3[k % 3];
       const float zq = zc[q] + interp(c, ec, jq, iq);
       acc += nb.s[k] * zq;
       if (k == 4) zg = zq;
