@@ -12,8 +12,8 @@
 
 using namespace pb2d;
 
-int pb_nn_forward_batch(pb_nn* nn, int nimg, int h, int w, const float* r, const float* f1, const float* f2,
-                        float* z, hipStream_t s);
+int pb_nn_forward_unique(pb_nn* nn, int nimg, int uny, int unx, int upy, int upx, int nh, int nw, const float* v,
+                         const float* f1, const float* f2, float* z, hipStream_t s);
 
 namespace pbmg {
 namespace {
@@ -530,41 +530,6 @@ __global__ __launch_bounds__(kAlphaThreads) void kc_net_alpha(Dims d, const floa
   }
 }
 
-// the network level's right-hand sides onto the duplicated grid, and back
-__global__ void k_to_net(Dims u, int nh, int nw, const float* v, float* nr) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= nw || j >= nh) return;
-  const long pu = (long)u.ny * u.nx, pd = (long)nh * nw;
-  const long s = (long)(j % u.ny) * u.nx + (i % u.nx), o = (long)j * nw + i;
-  nr[o] = v[s];
-  nr[pd + o] = v[pu + s];
-}
-__global__ void k_from_net(Dims u, int nh, int nw, const float* nz, float* z) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= u.nx || j >= u.ny) return;
-  const long pu = (long)u.ny * u.nx, pd = (long)nh * nw;
-  // a duplicated node takes the mean of its copies
-  const bool dx = u.px && i == 0, dy = u.py && j == 0;
-  for (int ch = 0; ch < 2; ++ch) {
-    const float* q = nz + ch * pd;
-    float a = q[(long)j * nw + i];
-    float cnt = 1.f;
-    if (dx) {
-      a += q[(long)j * nw + (nw - 1)];
-      cnt += 1.f;
-    }
-    if (dy) {
-      a += q[(long)(nh - 1) * nw + i];
-      cnt += 1.f;
-    }
-    if (dx && dy) {
-      a += q[(long)(nh - 1) * nw + (nw - 1)];
-      cnt += 1.f;
-    }
-    z[ch * pu + (long)j * u.nx + i] = a / cnt;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -635,8 +600,6 @@ int layout(pb_op2d* op, Hier* H) {
   rc |= hmalloc(op, H, &H->r0, op->n);
   if (H->nn_level >= 0) {
     const long nd = (long)H->nh * H->nw;
-    rc |= hmalloc(op, H, &H->nr, 2 * nd);
-    rc |= hmalloc(op, H, &H->nz, 2 * nd);
     rc |= hmalloc(op, H, &H->f1, nd);
     rc |= hmalloc(op, H, &H->f2, nd);
   }
@@ -665,11 +628,9 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
   auto swap = [&] { std::swap(cur, alt); };
   if (l == last) {
     if (o.coarse == COARSE_NET && l == H->nn_level && nn) {
-      const dim3 gn(ceil_div(H->nw, 64), H->nh), gu(ceil_div(d.nx, 64), d.ny);
-      hipLaunchKernelGGL(k_to_net, gn, dim3(64), 0, s, d, H->nh, H->nw, (const float*)L.v, H->nr);
-      *rc = pb_nn_forward_batch(nn, 2, H->nh, H->nw, H->nr, H->f1, H->f2, H->nz, s);
+      *rc = pb_nn_forward_unique(nn, 2, d.ny, d.nx, d.py, d.px, H->nh, H->nw, (const float*)L.v, H->f1, H->f2, cur,
+                                 s);
       if (*rc) return nullptr;
-      hipLaunchKernelGGL(k_from_net, gu, dim3(64), 0, s, d, H->nh, H->nw, (const float*)H->nz, cur);
       hipLaunchKernelGGL(kc_net_alpha, dim3(1), dim3(kAlphaThreads), 0, s, d, (const float*)L.st, (const float*)L.v,
                          cur);
       for (int k = 0; k < o.coarse_sweeps; ++k) {

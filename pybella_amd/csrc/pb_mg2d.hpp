@@ -55,7 +55,6 @@ struct Hier {
   float* r0 = nullptr;
   // the network level with its periodic columns (rows) duplicated: nh x nw
   int nh = 0, nw = 0;
-  float *nr = nullptr, *nz = nullptr;   // [2][nh][nw] network input / output
   float *f1 = nullptr, *f2 = nullptr;   // coefficient features [nh][nw]
   std::vector<void*> allocs;
 };

@@ -1089,26 +1089,131 @@ int op_features(pb_nn* nn, const pb_op2d* op, hipStream_t s) {
   return PB_OK;
 }
 
+// The network's input formed from right-hand sides on a multigrid level's
+// unique grid (uny x unx; a periodic axis has no duplicated end node): the
+// image's (nh, nw) pixels with the end nodes duplicated, its rms (floored),
+// and the input channels r / scale, f1, f2 -- the values k_img_sums,
+// k_scale_from and k_input give for the duplicated image, with the same
+// sums bit for bit: 64 chunks of the image, each the 256-lane tree of
+// k_img_sums (with at most 256 pixels per chunk its first two levels are the
+// pair sums below, then a 64-lane butterfly adds the same pairs as the
+// tree), then the chunk sums in order.  Block (x, b) writes pixels
+// [256 x, 256 x + 256) of image b; every block of an image forms the same
+// sum, block 0 stores the scale.
+constexpr int kNetInThreads = 256;
+constexpr int kNetChunks = 64;   // k_img_sums' chunks per image
+__global__ __launch_bounds__(kNetInThreads) void k_net_in(const float* v, int uny, int unx, int nh, int nw,
+                                                        const float* f1, const float* f2, float* scale, bf16* x0) {
+  __shared__ double csum[kNetChunks];
+  __shared__ float sc;
+  const int b = blockIdx.y, t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+  const int np = nh * nw;
+  const float* vb = v + (long)b * uny * unx;
+  auto src = [&](int p) {
+    const int j = p / nw, i = p - j * nw;
+    return (j >= uny ? j - uny : j) * unx + (i >= unx ? i - unx : i);
+  };
+  const int p = blockIdx.x * kNetInThreads + t;
+  const float mine = p < np ? vb[src(p)] : 0.f;
+  const int chunk = (np + kNetChunks - 1) / kNetChunks;   // <= 256 (checked on the host)
+  for (int c = wid; c < kNetChunks; c += kNetInThreads / kWave) {
+    const int beg = c * chunk, end = min(beg + chunk, np);
+    auto sq = [&](int l) {   // lane l's k_img_sums value: its pixel squared, or 0
+      const int q = beg + l;
+      if (q >= end) return 0.0;
+      const double x = (double)vb[src(q)];
+      return x * x;   // exact: a float's square has at most 48 significant bits
+    };
+    // tree levels 128 and 64: L[l] = (L[l] + L[l + 128]) + (L[l + 64] + L[l + 192])
+    double s = (sq(lane) + sq(lane + 128)) + (sq(lane + 64) + sq(lane + 192));
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+    if (lane == 0) csum[c] = s;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double tot = 0.0;
+    for (int c = 0; c < kNetChunks; ++c) tot += csum[c];
+    const double rms = sqrt(tot / (double)np);
+    sc = (float)(rms > 1e-300 ? rms : 1.0);
+    if (blockIdx.x == 0) scale[b] = sc;
+  }
+  __syncthreads();
+  if (p >= np) return;
+  union {
+    bf16 h[16];
+    uint4 q[2];
+  } o;
+  o.h[0] = f2bf((float)((double)mine / (double)sc));
+  o.h[1] = f2bf(f1[p]);
+  o.h[2] = f2bf(f2[p]);
+#pragma unroll
+  for (int c = 3; c < 16; ++c) o.h[c] = 0;
+  uint4* dst = reinterpret_cast<uint4*>(x0 + ((long)b * np + p) * 16);
+  dst[0] = o.q[0];
+  dst[1] = o.q[1];
+}
+
+// The network's output back on the unique grid: scale * z of every copy of a
+// node, the copies of a duplicated end node averaged (the same arithmetic as
+// k_output followed by the multigrid's copy back).  Each copy's product is
+// rounded to float before the sum, as k_output stores it: contraction is off
+// here, or the product would be fused into the sum as one multiply-add.
+__global__ void k_net_out(const float* zo, const float* scale, int nimg, int uny, int unx, int upy, int upx, int nh,
+                          int nw, float* z) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= unx || j >= uny) return;
+  const long pu = (long)uny * unx, pd = (long)nh * nw;
+  const bool dx = upx && i == 0, dy = upy && j == 0;
+  for (int b = 0; b < nimg; ++b) {
+    const float* q = zo + b * pd;
+    const float sf = scale[b];
+    // the float product of two floats, i.e. k_output's (float)((double)z * scale)
+    auto val = [&](long o) { return q[o] * sf; };
+    float a = val((long)j * nw + i);
+    float cnt = 1.f;
+    if (dx) {
+      a += val((long)j * nw + (nw - 1));
+      cnt += 1.f;
+    }
+    if (dy) {
+      a += val((long)(nh - 1) * nw + i);
+      cnt += 1.f;
+    }
+    if (dx && dy) {
+      a += val((long)(nh - 1) * nw + (nw - 1));
+      cnt += 1.f;
+    }
+    z[b * pu + (long)j * unx + i] = a / cnt;
+  }
+}
+
 }  // namespace
 
-// z = rms(r) * net(r / rms(r), f1, f2) for a batch of ``nimg`` images of
-// h x w nodes that share one pair of feature fields: r, z float
-// [nimg][h][w], f1, f2 float [h][w] (device).  Used by the multigrid cycle,
-// whose coarse level hands the network both branches at once.
-int pb_nn_forward_batch(pb_nn* nn, int nimg, int h, int w, const float* r, const float* f1, const float* f2,
-                        float* z, hipStream_t s) {
-  if ((h & 1) == 0 || (w & 1) == 0 || ((h - 1) & 7) || ((w - 1) & 7) || h < 9 || w < 9)
-    return pb_fail(nn->ctx, PB_ERR_ARG, "network: a %d x %d grid is not 8k + 1 nodes per axis", h, w);
+// z = rms(v) * net(v / rms(v), f1, f2) for a batch of ``nimg`` right-hand
+// sides v and results z on a multigrid level's unique grid (uny x unx nodes,
+// float [nimg][uny][unx]; upy / upx: periodic axes, whose end node the
+// network's nh x nw grid duplicates) that share one pair of feature fields
+// f1, f2 float [nh][nw] (device).  Used by the multigrid cycle, whose coarse
+// level hands the network both branches at once: one launch before the
+// network and one after it.
+int pb_nn_forward_unique(pb_nn* nn, int nimg, int uny, int unx, int upy, int upx, int nh, int nw, const float* v,
+                         const float* f1, const float* f2, float* z, hipStream_t s) {
+  if ((nh & 1) == 0 || (nw & 1) == 0 || ((nh - 1) & 7) || ((nw - 1) & 7) || nh < 9 || nw < 9)
+    return pb_fail(nn->ctx, PB_ERR_ARG, "network: a %d x %d grid is not 8k + 1 nodes per axis", nh, nw);
+  if ((long)nh * nw > 256L * kNetChunks)
+    return pb_fail(nn->ctx, PB_ERR_ARG, "network: a %d x %d level exceeds %d pixels", nh, nw, 256 * kNetChunks);
+  if (nh != uny + (upy ? 1 : 0) || nw != unx + (upx ? 1 : 0))
+    return pb_fail(nn->ctx, PB_ERR_ARG, "network: a %d x %d grid does not duplicate the end nodes of %d x %d", nh, nw,
+                   uny, unx);
   int rc = 0;
-  Acts* a = get_shape(nn, nimg, h, w, false, &rc);
+  Acts* a = get_shape(nn, nimg, nh, nw, false, &rc);
   if (!a) return rc;
-  const long np = (long)h * w;
-  rms_scale<float>(a, r, s);
-  hipLaunchKernelGGL(k_input<float>, dim3(gs(np * nimg)), dim3(256), 0, s, r, (const float*)a->scale, f1, f2, 0, np,
-                     nimg, a->X0);
+  hipLaunchKernelGGL(k_net_in, dim3(ceil_div(nh * nw, kNetInThreads), nimg), dim3(kNetInThreads), 0, s, v, uny, unx,
+                     nh, nw, f1, f2, a->scale, a->X0);
   forward(nn, a, s);
-  hipLaunchKernelGGL(k_output<float>, dim3(gs(np * nimg)), dim3(256), 0, s, (const float*)a->zout,
-                     (const float*)a->scale, np, nimg, z);
+  hipLaunchKernelGGL(k_net_out, dim3(ceil_div(unx, 64), uny), dim3(64), 0, s, (const float*)a->zout,
+                     (const float*)a->scale, nimg, uny, unx, upy, upx, nh, nw, z);
   PB_LAUNCH_CHECK(nn->ctx);
   return PB_OK;
 }
