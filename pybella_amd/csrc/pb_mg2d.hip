@@ -12,8 +12,8 @@
 
 using namespace pb2d;
 
-int pb_nn_forward_unique(pb_nn* nn, int nimg, int uny, int unx, int upy, int upx, int nh, int nw, const float* v,
-                         const float* f1, const float* f2, float* z, hipStream_t s);
+int pb_nn_forward_level(pb_nn* nn, int nimg, int uny, int unx, int nh, int nw, const float* v, const float* f1,
+                        const float* f2, const float** zout, const float** scale, hipStream_t s);
 
 namespace pbmg {
 namespace {
@@ -491,24 +491,64 @@ __global__ void kc_prolong_sweep(Dims d, Dims c, const float* st, const float* i
   }
 }
 
-// z *= alpha per branch, alpha = (z . v) / (z . A z), the energy-norm
-// (Galerkin) scale of the direction z: the network predicts the shape of the
-// coarse correction, its scale is fitted here.  A residual fit (A z . v) /
-// (A z . A z) would weight the high-frequency part of A z and shrink the
-// smooth modes the coarse level is there for.  One block of 1024 threads (the
-// network's level has at most 65 x 33 nodes).
-constexpr int kAlphaThreads = 1024;
-__global__ __launch_bounds__(kAlphaThreads) void kc_net_alpha(Dims d, const float* st, const float* v, float* z) {
-  __shared__ double sh[4 * (kAlphaThreads / kWave)];
+// The network level's correction, both branches, in one block of 1024
+// threads with the level in LDS:
+//  1. z = scale * (network output) at every node, a duplicated end node's
+//     copies averaged (the network's nh x nw grid duplicates the end node of
+//     each periodic axis of the d.ny x d.nx level); each copy's product is
+//     rounded to float before the sum (FP contraction off there);
+//  2. z *= alpha per branch, alpha = (z . v) / (z . A z), the energy-norm
+//     (Galerkin) scale of the direction z: the network predicts the shape of
+//     the coarse correction, its scale is fitted here.  A residual fit
+//     (A z . v) / (A z . A z) would weight the high-frequency part of A z and
+//     shrink the smooth modes the coarse level is there for.  The sums run in
+//     double, thread t over nodes t, t + 1024, ..., then the block sum;
+//  3. ``sweeps`` damped Jacobi sweeps z += omega (v - A z) / diag.
+// The result goes to zfin.  Each step's arithmetic per node is that of the
+// separate launches it replaces (output scaling, the fit, kc_sweep).
+constexpr int kNetFinThreads = 1024;
+constexpr int kNetFinMax = 65 * 33;   // nodes of the network level, at most
+__global__ __launch_bounds__(kNetFinThreads) void kc_net_finish(Dims d, int nh, int nw, const float* zo,
+                                                                const float* scale, const float* st, const float* idg,
+                                                                const float* v, int sweeps, float omega,
+                                                                float* zfin) {
+  __shared__ float zb[2][2 * kNetFinMax];
+  __shared__ double sh[4 * (kNetFinThreads / kWave)];
   __shared__ float alpha[2];
-  const long n = (long)d.ny * d.nx;
+  const int n = d.ny * d.nx, t = threadIdx.x;
+  const int pd = nh * nw;
+  for (int g = t; g < n; g += kNetFinThreads) {
+#pragma clang fp contract(off)
+    const int j = g / d.nx, i = g - j * d.nx;
+    const bool dx = d.px && i == 0, dy = d.py && j == 0;
+    for (int b = 0; b < 2; ++b) {
+      const float* q = zo + b * pd;
+      const float sf = scale[b];
+      float a = q[j * nw + i] * sf;
+      float cnt = 1.f;
+      if (dx) {
+        a += q[j * nw + (nw - 1)] * sf;
+        cnt += 1.f;
+      }
+      if (dy) {
+        a += q[(nh - 1) * nw + i] * sf;
+        cnt += 1.f;
+      }
+      if (dx && dy) {
+        a += q[(nh - 1) * nw + (nw - 1)] * sf;
+        cnt += 1.f;
+      }
+      zb[0][b * n + g] = a / cnt;
+    }
+  }
+  __syncthreads();
   double acc[4] = {0.0, 0.0, 0.0, 0.0};   // (z.v, z.Az) per branch
-  for (long g = threadIdx.x; g < n; g += kAlphaThreads) {
+  for (int g = t; g < n; g += kNetFinThreads) {
     Nb nb;
     load_row(d, st, g, nb);
 #pragma unroll
     for (int ch = 0; ch < 2; ++ch) {
-      const float* zc = z + ch * n;
+      const float* zc = zb[0] + ch * n;
       float az = 0.f;
 #pragma unroll
       for (int k = 0; k < 9; ++k) az += nb.s[k] * zc[nb.idx[k]];
@@ -517,16 +557,39 @@ __global__ __launch_bounds__(kAlphaThreads) void kc_net_alpha(Dims d, const floa
       acc[2 * ch + 1] += zg * (double)az;
     }
   }
-  block_sum_w<4, kAlphaThreads / kWave>(acc, sh);
-  if (threadIdx.x == 0) {
+  block_sum_w<4, kNetFinThreads / kWave>(acc, sh);
+  if (t == 0) {
     alpha[0] = acc[1] != 0.0 ? (float)(acc[0] / acc[1]) : 0.f;
     alpha[1] = acc[3] != 0.0 ? (float)(acc[2] / acc[3]) : 0.f;
   }
   __syncthreads();
   const float a0 = alpha[0], a1 = alpha[1];
-  for (long g = threadIdx.x; g < n; g += kAlphaThreads) {
-    z[g] *= a0;
-    z[n + g] *= a1;
+  for (int g = t; g < n; g += kNetFinThreads) {
+    zb[0][g] *= a0;
+    zb[0][n + g] *= a1;
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int k = 0; k < sweeps; ++k) {
+    for (int g = t; g < n; g += kNetFinThreads) {
+      Nb nb;
+      load_row(d, st, g, nb);
+      const float ig = idg[g];
+#pragma unroll
+      for (int ch = 0; ch < 2; ++ch) {
+        const float* zc = zb[cur] + ch * n;
+        float acc1 = 0.f;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc1 += nb.s[q] * zc[nb.idx[q]];
+        zb[cur ^ 1][ch * n + g] = zc[g] + omega * (v[ch * n + g] - acc1) * ig;
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  for (int g = t; g < n; g += kNetFinThreads) {
+    zfin[g] = zb[cur][g];
+    zfin[n + g] = zb[cur][n + g];
   }
 }
 
@@ -628,16 +691,16 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
   auto swap = [&] { std::swap(cur, alt); };
   if (l == last) {
     if (o.coarse == COARSE_NET && l == H->nn_level && nn) {
-      *rc = pb_nn_forward_unique(nn, 2, d.ny, d.nx, d.py, d.px, H->nh, H->nw, (const float*)L.v, H->f1, H->f2, cur,
-                                 s);
-      if (*rc) return nullptr;
-      hipLaunchKernelGGL(kc_net_alpha, dim3(1), dim3(kAlphaThreads), 0, s, d, (const float*)L.st, (const float*)L.v,
-                         cur);
-      for (int k = 0; k < o.coarse_sweeps; ++k) {
-        hipLaunchKernelGGL(kc_sweep, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.idg,
-                           (const float*)L.v, (const float*)cur, o.omega, alt);
-        swap();
+      if (n > kNetFinMax || (long)H->nh * H->nw > kNetFinMax) {
+        *rc = pb_fail(op->ctx, PB_ERR_ARG, "multigrid: network level of %ld nodes exceeds %d", n, kNetFinMax);
+        return nullptr;
       }
+      const float *zo = nullptr, *sc = nullptr;
+      *rc = pb_nn_forward_level(nn, 2, d.ny, d.nx, H->nh, H->nw, (const float*)L.v, H->f1, H->f2, &zo, &sc, s);
+      if (*rc) return nullptr;
+      hipLaunchKernelGGL(kc_net_finish, dim3(1), dim3(kNetFinThreads), 0, s, d, H->nh, H->nw, zo, sc,
+                         (const float*)L.st, (const float*)L.idg, (const float*)L.v, std::max(0, o.coarse_sweeps),
+                         o.omega, cur);
       return cur;
     }
     const int sweeps = std::max(1, o.coarse == COARSE_MG ? o.bottom_sweeps : o.coarse_sweeps);

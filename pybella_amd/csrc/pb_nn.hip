@@ -1153,57 +1153,24 @@ __global__ __launch_bounds__(kNetInThreads) void k_net_in(const float* v, int un
   dst[1] = o.q[1];
 }
 
-// The network's output back on the unique grid: scale * z of every copy of a
-// node, the copies of a duplicated end node averaged (the same arithmetic as
-// k_output followed by the multigrid's copy back).  Each copy's product is
-// rounded to float before the sum, as k_output stores it: contraction is off
-// here, or the product would be fused into the sum as one multiply-add.
-__global__ void k_net_out(const float* zo, const float* scale, int nimg, int uny, int unx, int upy, int upx, int nh,
-                          int nw, float* z) {
-#pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= unx || j >= uny) return;
-  const long pu = (long)uny * unx, pd = (long)nh * nw;
-  const bool dx = upx && i == 0, dy = upy && j == 0;
-  for (int b = 0; b < nimg; ++b) {
-    const float* q = zo + b * pd;
-    const float sf = scale[b];
-    // the float product of two floats, i.e. k_output's (float)((double)z * scale)
-    auto val = [&](long o) { return q[o] * sf; };
-    float a = val((long)j * nw + i);
-    float cnt = 1.f;
-    if (dx) {
-      a += val((long)j * nw + (nw - 1));
-      cnt += 1.f;
-    }
-    if (dy) {
-      a += val((long)(nh - 1) * nw + i);
-      cnt += 1.f;
-    }
-    if (dx && dy) {
-      a += val((long)(nh - 1) * nw + (nw - 1));
-      cnt += 1.f;
-    }
-    z[b * pu + (long)j * unx + i] = a / cnt;
-  }
-}
-
 }  // namespace
 
-// z = rms(v) * net(v / rms(v), f1, f2) for a batch of ``nimg`` right-hand
-// sides v and results z on a multigrid level's unique grid (uny x unx nodes,
-// float [nimg][uny][unx]; upy / upx: periodic axes, whose end node the
-// network's nh x nw grid duplicates) that share one pair of feature fields
-// f1, f2 float [nh][nw] (device).  Used by the multigrid cycle, whose coarse
-// level hands the network both branches at once: one launch before the
-// network and one after it.
-int pb_nn_forward_unique(pb_nn* nn, int nimg, int uny, int unx, int upy, int upx, int nh, int nw, const float* v,
-                         const float* f1, const float* f2, float* z, hipStream_t s) {
+// The network applied to a batch of ``nimg`` right-hand sides v on a
+// multigrid level's unique grid (uny x unx nodes, float [nimg][uny][unx];
+// the network's nh x nw grid duplicates the end node of a periodic axis)
+// that share one pair of feature fields f1, f2 float [nh][nw] (device): one
+// launch forms the input (the duplicated image, its rms scale and the input
+// channels v / scale, f1, f2), then the forward.  On return *zout holds the
+// network's output z, float [nimg][nh][nw], and *scale the per-image rms
+// (device buffers of the network, valid until its next call); the result of
+// the batch is scale * z, which the multigrid's coarse-level kernel forms.
+int pb_nn_forward_level(pb_nn* nn, int nimg, int uny, int unx, int nh, int nw, const float* v, const float* f1,
+                        const float* f2, const float** zout, const float** scale, hipStream_t s) {
   if ((nh & 1) == 0 || (nw & 1) == 0 || ((nh - 1) & 7) || ((nw - 1) & 7) || nh < 9 || nw < 9)
     return pb_fail(nn->ctx, PB_ERR_ARG, "network: a %d x %d grid is not 8k + 1 nodes per axis", nh, nw);
   if ((long)nh * nw > 256L * kNetChunks)
     return pb_fail(nn->ctx, PB_ERR_ARG, "network: a %d x %d level exceeds %d pixels", nh, nw, 256 * kNetChunks);
-  if (nh != uny + (upy ? 1 : 0) || nw != unx + (upx ? 1 : 0))
+  if (nh < uny || nh > uny + 1 || nw < unx || nw > unx + 1)
     return pb_fail(nn->ctx, PB_ERR_ARG, "network: a %d x %d grid does not duplicate the end nodes of %d x %d", nh, nw,
                    uny, unx);
   int rc = 0;
@@ -1212,9 +1179,9 @@ int pb_nn_forward_unique(pb_nn* nn, int nimg, int uny, int unx, int upy, int upx
   hipLaunchKernelGGL(k_net_in, dim3(ceil_div(nh * nw, kNetInThreads), nimg), dim3(kNetInThreads), 0, s, v, uny, unx,
                      nh, nw, f1, f2, a->scale, a->X0);
   forward(nn, a, s);
-  hipLaunchKernelGGL(k_net_out, dim3(ceil_div(unx, 64), uny), dim3(64), 0, s, (const float*)a->zout,
-                     (const float*)a->scale, nimg, uny, unx, upy, upx, nh, nw, z);
   PB_LAUNCH_CHECK(nn->ctx);
+  *zout = a->zout;
+  *scale = a->scale;
   return PB_OK;
 }
 
