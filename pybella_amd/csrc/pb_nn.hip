@@ -184,8 +184,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
 // 3x3 convolution on small grids (forward, one 16-channel output tile per
 // block).  A small launch costs its chain of dependent memory latencies, so
 // all nine taps' weights of the tile are staged in LDS at once (one barrier
-// instead of two per tap), and tap t+1's input chunks are loaded while tap
-// t's MFMAs run.  Per output the MFMAs are issued in k_conv3x3's order (tap
+// instead of two per tap), and all nine taps' input chunks are loaded before
+// the first MFMA.  Per output the MFMAs are issued in k_conv3x3's order (tap
 // by tap, input chunks in order), so the sums are bitwise those of k_conv3x3.
 // ---------------------------------------------------------------------------
 template <int CIN>
@@ -209,15 +209,20 @@ __global__ __launch_bounds__(256) void k_conv3x3_small(const bf16* __restrict__ 
   const long ba = t1 / h;
   const int t0 = blockIdx.y;
   const int nout = gridDim.y * 16;
-  // the tile's weights of all nine taps, 16-byte global loads
-  constexpr int NQ = 9 * 16 * (CIN / 8);
-  for (int q = threadIdx.x; q < NQ; q += 256) {
+  // Every load of the block is issued before the first wait: the tile's
+  // weights of all nine taps (16-byte loads into registers), then all nine
+  // taps' input chunks; the weights go to LDS once they arrive (the input
+  // loads still in flight), and tap t's MFMAs wait only for tap t's inputs.
+  // On these small grids a block is one chain of memory latencies, so one
+  // round trip instead of one per tap.
+  constexpr int NQ = 9 * 16 * (CIN / 8), WQ = (NQ + 255) / 256;
+  uint4 wv[WQ];
+#pragma unroll
+  for (int e = 0; e < WQ; ++e) {
+    const int q = min((int)threadIdx.x + e * 256, NQ - 1);   // clamped, unconditional; stored only if in range
     const int tap = q / (16 * (CIN / 8)), rem = q - tap * 16 * (CIN / 8);
     const int row = rem / (CIN / 8), c8 = rem - row * (CIN / 8);
-    const uint4 v = *(const uint4*)(wg + ((long)tap * nout + t0 * 16 + row) * CIN + c8 * 8);
-    bf16* d = wsh + (tap * 16 + row) * WST + c8 * 8;
-    *(uint2*)d = uint2{v.x, v.y};
-    *(uint2*)(d + 4) = uint2{v.z, v.w};
+    wv[e] = *(const uint4*)(wg + ((long)tap * nout + t0 * 16 + row) * CIN + c8 * 8);
   }
   // a tap's input chunks (unconditional loads; zeroed outside the image)
   auto fetch = [&](int tap, bf16x4 (&a)[NCI]) -> bool {
@@ -228,19 +233,29 @@ __global__ __launch_bounds__(256) void k_conv3x3_small(const bf16* __restrict__ 
     for (int c = 0; c < NCI; ++c) a[c] = *(const bf16x4*)(in + src + c * 16);
     return ok;
   };
-  bf16x4 a[2][NCI];
-  bool ok[2];
-  ok[0] = fetch(0, a[0]);
+  bf16x4 a[9][NCI];
+  bool ok[9];
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) ok[tap] = fetch(tap, a[tap]);
+#pragma unroll
+  for (int e = 0; e < WQ; ++e) {
+    const int q = (int)threadIdx.x + e * 256;
+    if (q < NQ) {
+      const int tap = q / (16 * (CIN / 8)), rem = q - tap * 16 * (CIN / 8);
+      const int row = rem / (CIN / 8), c8 = rem - row * (CIN / 8);
+      bf16* d = wsh + (tap * 16 + row) * WST + c8 * 8;
+      *(uint2*)d = uint2{wv[e].x, wv[e].y};
+      *(uint2*)(d + 4) = uint2{wv[e].z, wv[e].w};
+    }
+  }
   __syncthreads();
   f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int tap = 0; tap < 9; ++tap) {
-    const int cur = tap & 1;
-    if (tap < 8) ok[cur ^ 1] = fetch(tap + 1, a[cur ^ 1]);
     const bf16* wrow = wsh + (tap * 16 + (lane & 15)) * WST + 4 * kg;
 #pragma unroll
     for (int c = 0; c < NCI; ++c) {
-      const bf16x4 av = ok[cur] ? a[cur][c] : bf16x4{0, 0, 0, 0};
+      const bf16x4 av = ok[tap] ? a[tap][c] : bf16x4{0, 0, 0, 0};
       const bf16x4 b = *(const bf16x4*)(wrow + c * 16);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b, acc, 0, 0, 0);
     }
