@@ -456,35 +456,46 @@ __device__ __forceinline__ float interp(const Dims& c, const float* e, int j, in
                   (e[(long)J1 * c.nx + I0] + e[(long)J1 * c.nx + I1]));
 }
 
-// z' = z + P e, then out = z' + omega (v - A z') / d.  The neighbours' rows
-// and columns come from the same wrap as their flat indices (no division).
-__global__ void kc_prolong_sweep(Dims d, Dims c, const float* st, const float* idg, const float* v, const float* z,
-                                 const float* e, float omega, float* out) {
+// z' = z + P e, then out = z' + omega (v - A z') / d, on tiles of 64 x 4
+// nodes (block (64, 4)): z' of the tile and its one-node halo is formed once
+// into LDS (at the halo's wrapped or clamped neighbour coordinates, as the
+// stencil's neighbour indices take them), and each node's nine neighbours
+// come from there.  Per node the arithmetic is that of forming z' at every
+// neighbour on its own: one pass of z and e loads per tile instead of nine
+// gathers of z and 36 of e per node, whose load issue bounded the launch.
+constexpr int kPsX = 64, kPsY = 4;
+__global__ __launch_bounds__(kPsX * kPsY) void kc_prolong_sweep(Dims d, Dims c, const float* st, const float* idg,
+                                                               const float* v, const float* z, const float* e,
+                                                               float omega, float* out) {
+  __shared__ float zp[2][kPsY + 2][kPsX + 2];
   const long n = (long)d.ny * d.nx, pc = (long)c.ny * c.nx;
-  const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  Nb nb;
-  load_row(d, st, g, nb);
-  const float ig = idg[g];
-  const int j = (int)g / d.nx, i = (int)g - j * d.nx;
-  int jn[3], in3[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
+  const int x0 = blockIdx.x * kPsX, y0 = blockIdx.y * kPsY;
+  const int t = threadIdx.y * kPsX + threadIdx.x;
+  for (int q = t; q < (kPsY + 2) * (kPsX + 2); q += kPsX * kPsY) {
+    const int r = q / (kPsX + 2), cc = q - r * (kPsX + 2);
+    const int j = y0 - 1 + r, i = x0 - 1 + cc;
+    if (j > d.ny || i > d.nx) continue;   // no node of the grid has it as a neighbour
     bool ok;
-    jn[a] = axis_idx(j + a - 1, d.ny, d.py, ok);
-    in3[a] = axis_idx(i + a - 1, d.nx, d.px, ok);
+    const int jq = axis_idx(j, d.ny, d.py, ok), iq = axis_idx(i, d.nx, d.px, ok);
+    const long gq = (long)jq * d.nx + iq;
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) zp[ch][r][cc] = z[ch * n + gq] + interp(c, e + ch * pc, jq, iq);
   }
+  __syncthreads();
+  const int j = y0 + threadIdx.y, i = x0 + threadIdx.x;
+  if (j >= d.ny || i >= d.nx) return;
+  const long g = (long)j * d.nx + i;
+  float sk[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) sk[k] = st[k * n + g];
+  const float ig = idg[g];
 #pragma unroll
   for (int ch = 0; ch < 2; ++ch) {
-    const float* zc = z + ch * n;
-    const float* ec = e + ch * pc;
     float acc = 0.f, zg = 0.f;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      const int q = nb.idx[k];
-      const int jq = jn[k / 3], iq = in3[k % 3];
-      const float zq = zc[q] + interp(c, ec, jq, iq);
-      acc += nb.s[k] * zq;
+      const float zq = zp[ch][threadIdx.y + k / 3][threadIdx.x + k % 3];
+      acc += sk[k] * zq;
       if (k == 4) zg = zq;
     }
     out[ch * n + g] = zg + omega * (v[ch * n + g] - acc) * ig;
@@ -739,8 +750,9 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
   const float* e = vcycle(op, H, nn, l + 1, last, o, s, rc);
   if (!e) return nullptr;
   // coarse correction and post-smoothing (at least one sweep)
-  hipLaunchKernelGGL(kc_prolong_sweep, dim3(nb), dim3(256), 0, s, d, C.d, (const float*)L.st, (const float*)L.idg,
-                     (const float*)L.v, (const float*)cur, e, o.omega, alt);
+  hipLaunchKernelGGL(kc_prolong_sweep, dim3(ceil_div(d.nx, kPsX), ceil_div(d.ny, kPsY)), dim3(kPsX, kPsY), 0, s, d,
+                     C.d, (const float*)L.st, (const float*)L.idg, (const float*)L.v, (const float*)cur, e, o.omega,
+                     alt);
   swap();
   for (int k = 1; k < o.nu_post; ++k) {
     hipLaunchKernelGGL(kc_sweep, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.idg,
