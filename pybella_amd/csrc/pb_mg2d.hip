@@ -319,68 +319,11 @@ __device__ __forceinline__ void load_row(const Dims& d, const float* st, long g,
   for (int k = 0; k < 9; ++k) nb.s[k] = st[k * p + g];
 }
 
-// z = omega v / d, then one sweep (the first two pre-sweeps)
-__global__ void kc_first(Dims d, const float* st, const float* idg, const float* v, float omega, float* z) {
-  const long n = (long)d.ny * d.nx;
-  const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  Nb nb;
-  load_row(d, st, g, nb);
-  float ig[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) ig[k] = idg[nb.idx[k]];
-#pragma unroll
-  for (int ch = 0; ch < 2; ++ch) {
-    const float* vc = v + ch * n;
-    float acc = 0.f;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) acc += nb.s[k] * (omega * vc[nb.idx[k]] * ig[k]);
-    const float vg = vc[g];
-    z[ch * n + g] = omega * vg * ig[4] + omega * (vg - acc) * ig[4];
-  }
-}
-
 // z = omega v / d (a single sweep from zero)
 __global__ void kc_init(long n, const float* idg, const float* v, float omega, float* z) {
   for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
     z[g] = omega * v[g] * idg[g];
     z[n + g] = omega * v[n + g] * idg[g];
-  }
-}
-
-// out = z + omega (v - A z) / d
-__global__ void kc_sweep(Dims d, const float* st, const float* idg, const float* v, const float* z, float omega,
-                         float* out) {
-  const long n = (long)d.ny * d.nx;
-  const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  Nb nb;
-  load_row(d, st, g, nb);
-  const float ig = idg[g];
-#pragma unroll
-  for (int ch = 0; ch < 2; ++ch) {
-    const float* zc = z + ch * n;
-    float acc = 0.f;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) acc += nb.s[k] * zc[nb.idx[k]];
-    out[ch * n + g] = zc[g] + omega * (v[ch * n + g] - acc) * ig;
-  }
-}
-
-// r = v - A z
-__global__ void kc_resid(Dims d, const float* st, const float* v, const float* z, float* r) {
-  const long n = (long)d.ny * d.nx;
-  const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  Nb nb;
-  load_row(d, st, g, nb);
-#pragma unroll
-  for (int ch = 0; ch < 2; ++ch) {
-    const float* zc = z + ch * n;
-    float acc = 0.f;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) acc += nb.s[k] * zc[nb.idx[k]];
-    r[ch * n + g] = v[ch * n + g] - acc;
   }
 }
 
@@ -456,19 +399,24 @@ __device__ __forceinline__ float interp(const Dims& c, const float* e, int j, in
                   (e[(long)J1 * c.nx + I0] + e[(long)J1 * c.nx + I1]));
 }
 
-// z' = z + P e, then out = z' + omega (v - A z') / d, on tiles of 64 x 4
-// nodes (block (64, 4)): z' of the tile and its one-node halo is formed once
-// into LDS (at the halo's wrapped or clamped neighbour coordinates, as the
-// stencil's neighbour indices take them), and each node's nine neighbours
-// come from there.  Per node the arithmetic is that of forming z' at every
-// neighbour on its own: one pass of z and e loads per tile instead of nine
-// gathers of z and 36 of e per node, whose load issue bounded the launch.
+// Stencil launches of levels 1 and below on tiles of 64 x 4 nodes (block
+// (64, 4)).  tile9 forms the values the stencil reads, both branches, at the
+// tile's nodes and its one-node halo into LDS once -- at the halo's wrapped
+// or clamped neighbour coordinates, as the stencil's neighbour indices take
+// them -- and hands each node of the grid its nine neighbours' values from
+// there (entry k = 3 (dj + 1) + (di + 1)).  Per node the arithmetic is that
+// of gathering each neighbour's value on its own; the gathers (nine per
+// node and branch, 36 more coarse values in the prolongation sweep) were
+// what bounded these launches.
 constexpr int kPsX = 64, kPsY = 4;
-__global__ __launch_bounds__(kPsX * kPsY) void kc_prolong_sweep(Dims d, Dims c, const float* st, const float* idg,
-                                                               const float* v, const float* z, const float* e,
-                                                               float omega, float* out) {
+inline dim3 tile_grid(const Dims& d) { return dim3(ceil_div(d.nx, kPsX), ceil_div(d.ny, kPsY)); }
+const dim3 kTileBlk(kPsX, kPsY);
+
+// fill(ch, jq, iq, gq): the value at node (jq, iq), flat index gq, of branch
+// ch; node(g, zn) with zn[ch][k] for each node g of the grid in the tile
+template <class Fill, class Node>
+__device__ __forceinline__ void tile9(const Dims& d, Fill fill, Node node) {
   __shared__ float zp[2][kPsY + 2][kPsX + 2];
-  const long n = (long)d.ny * d.nx, pc = (long)c.ny * c.nx;
   const int x0 = blockIdx.x * kPsX, y0 = blockIdx.y * kPsY;
   const int t = threadIdx.y * kPsX + threadIdx.x;
   for (int q = t; q < (kPsY + 2) * (kPsX + 2); q += kPsX * kPsY) {
@@ -479,27 +427,104 @@ __global__ __launch_bounds__(kPsX * kPsY) void kc_prolong_sweep(Dims d, Dims c, 
     const int jq = axis_idx(j, d.ny, d.py, ok), iq = axis_idx(i, d.nx, d.px, ok);
     const long gq = (long)jq * d.nx + iq;
 #pragma unroll
-    for (int ch = 0; ch < 2; ++ch) zp[ch][r][cc] = z[ch * n + gq] + interp(c, e + ch * pc, jq, iq);
+    for (int ch = 0; ch < 2; ++ch) zp[ch][r][cc] = fill(ch, jq, iq, gq);
   }
   __syncthreads();
   const int j = y0 + threadIdx.y, i = x0 + threadIdx.x;
   if (j >= d.ny || i >= d.nx) return;
-  const long g = (long)j * d.nx + i;
-  float sk[9];
+  float zn[2][9];
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) zn[ch][k] = zp[ch][threadIdx.y + k / 3][threadIdx.x + k % 3];
+  node((long)j * d.nx + i, zn);
+}
+
+__device__ __forceinline__ void load_st(const Dims& d, const float* st, long g, float (&sk)[9]) {
+  const long n = (long)d.ny * d.nx;
 #pragma unroll
   for (int k = 0; k < 9; ++k) sk[k] = st[k * n + g];
-  const float ig = idg[g];
+}
+
+// z = omega v / d, then one sweep (the first two pre-sweeps)
+__global__ __launch_bounds__(kPsX * kPsY) void kc_first(Dims d, const float* st, const float* idg, const float* v,
+                                                       float omega, float* z) {
+  const long n = (long)d.ny * d.nx;
+  tile9(
+      d, [&](int ch, int, int, long gq) { return omega * v[ch * n + gq] * idg[gq]; },
+      [&](long g, const float (&zn)[2][9]) {
+        float sk[9];
+        load_st(d, st, g, sk);
+        const float ig = idg[g];
 #pragma unroll
-  for (int ch = 0; ch < 2; ++ch) {
-    float acc = 0.f, zg = 0.f;
+        for (int ch = 0; ch < 2; ++ch) {
+          float acc = 0.f;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      const float zq = zp[ch][threadIdx.y + k / 3][threadIdx.x + k % 3];
-      acc += sk[k] * zq;
-      if (k == 4) zg = zq;
-    }
-    out[ch * n + g] = zg + omega * (v[ch * n + g] - acc) * ig;
-  }
+          for (int k = 0; k < 9; ++k) acc += sk[k] * zn[ch][k];
+          const float vg = v[ch * n + g];
+          z[ch * n + g] = omega * vg * ig + omega * (vg - acc) * ig;
+        }
+      });
+}
+
+// out = z + omega (v - A z) / d
+__global__ __launch_bounds__(kPsX * kPsY) void kc_sweep(Dims d, const float* st, const float* idg, const float* v,
+                                                       const float* z, float omega, float* out) {
+  const long n = (long)d.ny * d.nx;
+  tile9(
+      d, [&](int ch, int, int, long gq) { return z[ch * n + gq]; },
+      [&](long g, const float (&zn)[2][9]) {
+        float sk[9];
+        load_st(d, st, g, sk);
+        const float ig = idg[g];
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+          float acc = 0.f;
+#pragma unroll
+          for (int k = 0; k < 9; ++k) acc += sk[k] * zn[ch][k];
+          out[ch * n + g] = zn[ch][4] + omega * (v[ch * n + g] - acc) * ig;
+        }
+      });
+}
+
+// r = v - A z
+__global__ __launch_bounds__(kPsX * kPsY) void kc_resid(Dims d, const float* st, const float* v, const float* z,
+                                                       float* r) {
+  const long n = (long)d.ny * d.nx;
+  tile9(
+      d, [&](int ch, int, int, long gq) { return z[ch * n + gq]; },
+      [&](long g, const float (&zn)[2][9]) {
+        float sk[9];
+        load_st(d, st, g, sk);
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+          float acc = 0.f;
+#pragma unroll
+          for (int k = 0; k < 9; ++k) acc += sk[k] * zn[ch][k];
+          r[ch * n + g] = v[ch * n + g] - acc;
+        }
+      });
+}
+
+// z' = z + P e, then out = z' + omega (v - A z') / d
+__global__ __launch_bounds__(kPsX * kPsY) void kc_prolong_sweep(Dims d, Dims c, const float* st, const float* idg,
+                                                               const float* v, const float* z, const float* e,
+                                                               float omega, float* out) {
+  const long n = (long)d.ny * d.nx, pc = (long)c.ny * c.nx;
+  tile9(
+      d, [&](int ch, int jq, int iq, long gq) { return z[ch * n + gq] + interp(c, e + ch * pc, jq, iq); },
+      [&](long g, const float (&zn)[2][9]) {
+        float sk[9];
+        load_st(d, st, g, sk);
+        const float ig = idg[g];
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+          float acc = 0.f;
+#pragma unroll
+          for (int k = 0; k < 9; ++k) acc += sk[k] * zn[ch][k];
+          out[ch * n + g] = zn[ch][4] + omega * (v[ch * n + g] - acc) * ig;
+        }
+      });
 }
 
 // The network level's correction, both branches, in one block of 1024
@@ -723,7 +748,7 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
     hipLaunchKernelGGL(kc_init, dim3(std::min(nb, 1024)), dim3(256), 0, s, n, (const float*)L.idg,
                        (const float*)L.v, o.omega, cur);
     for (int k = 1; k < sweeps; ++k) {
-      hipLaunchKernelGGL(kc_sweep, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.idg,
+      hipLaunchKernelGGL(kc_sweep, tile_grid(d), kTileBlk, 0, s, d, (const float*)L.st, (const float*)L.idg,
                          (const float*)L.v, (const float*)cur, o.omega, alt);
       swap();
     }
@@ -731,10 +756,10 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
   }
   // pre-smoothing
   if (o.nu_pre >= 2) {
-    hipLaunchKernelGGL(kc_first, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.idg,
+    hipLaunchKernelGGL(kc_first, tile_grid(d), kTileBlk, 0, s, d, (const float*)L.st, (const float*)L.idg,
                        (const float*)L.v, o.omega, cur);
     for (int k = 2; k < o.nu_pre; ++k) {
-      hipLaunchKernelGGL(kc_sweep, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.idg,
+      hipLaunchKernelGGL(kc_sweep, tile_grid(d), kTileBlk, 0, s, d, (const float*)L.st, (const float*)L.idg,
                          (const float*)L.v, (const float*)cur, o.omega, alt);
       swap();
     }
@@ -742,7 +767,7 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
     hipLaunchKernelGGL(kc_init, dim3(std::min(nb, 1024)), dim3(256), 0, s, n, (const float*)L.idg,
                        (const float*)L.v, o.omega, cur);
   }
-  hipLaunchKernelGGL(kc_resid, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.v,
+  hipLaunchKernelGGL(kc_resid, tile_grid(d), kTileBlk, 0, s, d, (const float*)L.st, (const float*)L.v,
                      (const float*)cur, L.r);
   Level& C = H->L[l + 1];
   hipLaunchKernelGGL(kc_restrict, dim3(ceil_div(C.d.nx, 64), C.d.ny), dim3(64), 0, s, d, C.d, (const float*)L.r,
@@ -750,12 +775,12 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
   const float* e = vcycle(op, H, nn, l + 1, last, o, s, rc);
   if (!e) return nullptr;
   // coarse correction and post-smoothing (at least one sweep)
-  hipLaunchKernelGGL(kc_prolong_sweep, dim3(ceil_div(d.nx, kPsX), ceil_div(d.ny, kPsY)), dim3(kPsX, kPsY), 0, s, d,
+  hipLaunchKernelGGL(kc_prolong_sweep, tile_grid(d), kTileBlk, 0, s, d,
                      C.d, (const float*)L.st, (const float*)L.idg, (const float*)L.v, (const float*)cur, e, o.omega,
                      alt);
   swap();
   for (int k = 1; k < o.nu_post; ++k) {
-    hipLaunchKernelGGL(kc_sweep, dim3(nb), dim3(256), 0, s, d, (const float*)L.st, (const float*)L.idg,
+    hipLaunchKernelGGL(kc_sweep, tile_grid(d), kTileBlk, 0, s, d, (const float*)L.st, (const float*)L.idg,
                        (const float*)L.v, (const float*)cur, o.omega, alt);
     swap();
   }
