@@ -541,9 +541,13 @@ __global__ __launch_bounds__(kPsX * kPsY) void kc_prolong_sweep(Dims d, Dims c, 
 //     double, thread t over nodes t, t + 1024, ..., then the block sum;
 //  3. ``sweeps`` damped Jacobi sweeps z += omega (v - A z) / diag.
 // The result goes to zfin.  Each step's arithmetic per node is that of the
-// separate launches it replaces (output scaling, the fit, kc_sweep).
+// separate launches it replaces (output scaling, the fit, kc_sweep).  Thread
+// t owns nodes t, t + 1024, t + 2048 and loads everything they need (the
+// network's outputs, stencil rows, diag_inv, v) before the first step, so
+// the steps run from registers and LDS with no further round trip to memory.
 constexpr int kNetFinThreads = 1024;
 constexpr int kNetFinMax = 65 * 33;   // nodes of the network level, at most
+constexpr int kNetFinNpt = (kNetFinMax + kNetFinThreads - 1) / kNetFinThreads;   // nodes per thread
 __global__ __launch_bounds__(kNetFinThreads) void kc_net_finish(Dims d, int nh, int nw, const float* zo,
                                                                 const float* scale, const float* st, const float* idg,
                                                                 const float* v, int sweeps, float omega,
@@ -553,44 +557,74 @@ __global__ __launch_bounds__(kNetFinThreads) void kc_net_finish(Dims d, int nh, 
   __shared__ float alpha[2];
   const int n = d.ny * d.nx, t = threadIdx.x;
   const int pd = nh * nw;
-  for (int g = t; g < n; g += kNetFinThreads) {
-#pragma clang fp contract(off)
-    const int j = g / d.nx, i = g - j * d.nx;
-    const bool dx = d.px && i == 0, dy = d.py && j == 0;
+  // loads: per owned node its stencil row, diag_inv, v of both branches and
+  // the network's values at every copy of the node (q[b][0..3]: itself, the
+  // duplicated column, the duplicated row, the duplicated corner)
+  Nb nb[kNetFinNpt];
+  float ig[kNetFinNpt], vv[kNetFinNpt][2], q[kNetFinNpt][2][4];
+  bool dxs[kNetFinNpt], dys[kNetFinNpt];
+  const float sf[2] = {scale[0], scale[1]};
+#pragma unroll
+  for (int m = 0; m < kNetFinNpt; ++m) {
+    const int g = t + m * kNetFinThreads;
+    const bool own = g < n;
+    const int gg = own ? g : 0;
+    load_row(d, st, gg, nb[m]);
+    ig[m] = idg[gg];
+    vv[m][0] = v[gg];
+    vv[m][1] = v[n + gg];
+    const int j = gg / d.nx, i = gg - j * d.nx;
+    dxs[m] = d.px && i == 0;
+    dys[m] = d.py && j == 0;
+#pragma unroll
     for (int b = 0; b < 2; ++b) {
-      const float* q = zo + b * pd;
-      const float sf = scale[b];
-      float a = q[j * nw + i] * sf;
-      float cnt = 1.f;
-      if (dx) {
-        a += q[j * nw + (nw - 1)] * sf;
-        cnt += 1.f;
+      const float* qb = zo + b * pd;
+      q[m][b][0] = qb[j * nw + i];
+      q[m][b][1] = dxs[m] ? qb[j * nw + (nw - 1)] : 0.f;
+      q[m][b][2] = dys[m] ? qb[(nh - 1) * nw + i] : 0.f;
+      q[m][b][3] = dxs[m] && dys[m] ? qb[(nh - 1) * nw + (nw - 1)] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < kNetFinNpt; ++m) {
+#pragma clang fp contract(off)
+    const int g = t + m * kNetFinThreads;
+    if (g < n) {
+      for (int b = 0; b < 2; ++b) {
+        float a = q[m][b][0] * sf[b];
+        float cnt = 1.f;
+        if (dxs[m]) {
+          a += q[m][b][1] * sf[b];
+          cnt += 1.f;
+        }
+        if (dys[m]) {
+          a += q[m][b][2] * sf[b];
+          cnt += 1.f;
+        }
+        if (dxs[m] && dys[m]) {
+          a += q[m][b][3] * sf[b];
+          cnt += 1.f;
+        }
+        zb[0][b * n + g] = a / cnt;
       }
-      if (dy) {
-        a += q[(nh - 1) * nw + i] * sf;
-        cnt += 1.f;
-      }
-      if (dx && dy) {
-        a += q[(nh - 1) * nw + (nw - 1)] * sf;
-        cnt += 1.f;
-      }
-      zb[0][b * n + g] = a / cnt;
     }
   }
   __syncthreads();
   double acc[4] = {0.0, 0.0, 0.0, 0.0};   // (z.v, z.Az) per branch
-  for (int g = t; g < n; g += kNetFinThreads) {
-    Nb nb;
-    load_row(d, st, g, nb);
 #pragma unroll
-    for (int ch = 0; ch < 2; ++ch) {
-      const float* zc = zb[0] + ch * n;
-      float az = 0.f;
+  for (int m = 0; m < kNetFinNpt; ++m) {
+    const int g = t + m * kNetFinThreads;
+    if (g < n) {
 #pragma unroll
-      for (int k = 0; k < 9; ++k) az += nb.s[k] * zc[nb.idx[k]];
-      const double zg = (double)zc[g];
-      acc[2 * ch] += zg * (double)v[ch * n + g];
-      acc[2 * ch + 1] += zg * (double)az;
+      for (int ch = 0; ch < 2; ++ch) {
+        const float* zc = zb[0] + ch * n;
+        float az = 0.f;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) az += nb[m].s[k] * zc[nb[m].idx[k]];
+        const double zg = (double)zc[g];
+        acc[2 * ch] += zg * (double)vv[m][ch];
+        acc[2 * ch + 1] += zg * (double)az;
+      }
     }
   }
   block_sum_w<4, kNetFinThreads / kWave>(acc, sh);
@@ -600,32 +634,41 @@ __global__ __launch_bounds__(kNetFinThreads) void kc_net_finish(Dims d, int nh, 
   }
   __syncthreads();
   const float a0 = alpha[0], a1 = alpha[1];
-  for (int g = t; g < n; g += kNetFinThreads) {
-    zb[0][g] *= a0;
-    zb[0][n + g] *= a1;
+#pragma unroll
+  for (int m = 0; m < kNetFinNpt; ++m) {
+    const int g = t + m * kNetFinThreads;
+    if (g < n) {
+      zb[0][g] *= a0;
+      zb[0][n + g] *= a1;
+    }
   }
   __syncthreads();
   int cur = 0;
   for (int k = 0; k < sweeps; ++k) {
-    for (int g = t; g < n; g += kNetFinThreads) {
-      Nb nb;
-      load_row(d, st, g, nb);
-      const float ig = idg[g];
 #pragma unroll
-      for (int ch = 0; ch < 2; ++ch) {
-        const float* zc = zb[cur] + ch * n;
-        float acc1 = 0.f;
+    for (int m = 0; m < kNetFinNpt; ++m) {
+      const int g = t + m * kNetFinThreads;
+      if (g < n) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) acc1 += nb.s[q] * zc[nb.idx[q]];
-        zb[cur ^ 1][ch * n + g] = zc[g] + omega * (v[ch * n + g] - acc1) * ig;
+        for (int ch = 0; ch < 2; ++ch) {
+          const float* zc = zb[cur] + ch * n;
+          float acc1 = 0.f;
+#pragma unroll
+          for (int qq = 0; qq < 9; ++qq) acc1 += nb[m].s[qq] * zc[nb[m].idx[qq]];
+          zb[cur ^ 1][ch * n + g] = zc[g] + omega * (vv[m][ch] - acc1) * ig[m];
+        }
       }
     }
     __syncthreads();
     cur ^= 1;
   }
-  for (int g = t; g < n; g += kNetFinThreads) {
-    zfin[g] = zb[cur][g];
-    zfin[n + g] = zb[cur][n + g];
+#pragma unroll
+  for (int m = 0; m < kNetFinNpt; ++m) {
+    const int g = t + m * kNetFinThreads;
+    if (g < n) {
+      zfin[g] = zb[cur][g];
+      zfin[n + g] = zb[cur][n + g];
+    }
   }
 }
 

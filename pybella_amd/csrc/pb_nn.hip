@@ -1129,20 +1129,36 @@ __global__ __launch_bounds__(kNetInThreads) void k_net_in(const float* v, int un
     return (j >= uny ? j - uny : j) * unx + (i >= unx ? i - unx : i);
   };
   const int p = blockIdx.x * kNetInThreads + t;
-  const float mine = p < np ? vb[src(p)] : 0.f;
   const int chunk = (np + kNetChunks - 1) / kNetChunks;   // <= 256 (checked on the host)
-  for (int c = wid; c < kNetChunks; c += kNetInThreads / kWave) {
-    const int beg = c * chunk, end = min(beg + chunk, np);
-    auto sq = [&](int l) {   // lane l's k_img_sums value: its pixel squared, or 0
-      const int q = beg + l;
-      if (q >= end) return 0.0;
-      const double x = (double)vb[src(q)];
-      return x * x;   // exact: a float's square has at most 48 significant bits
+  // Every load of the thread first (its pixel, its features, and lane l's
+  // values of the wave's 16 chunks, 0 past a chunk's end), then the sums: one
+  // round trip instead of one per chunk.
+  constexpr int CPW = kNetChunks / (kNetInThreads / kWave);   // chunks per wave
+  static_assert(kNetChunks % (kNetInThreads / kWave) == 0, "the chunks divide evenly over the waves");
+  const float mine = p < np ? vb[src(p)] : 0.f;
+  const float g1 = p < np ? f1[p] : 0.f, g2 = p < np ? f2[p] : 0.f;
+  float xv[CPW][4];   // lane offsets 0, 128, 64, 192: the tree's pair order
+#pragma unroll
+  for (int m = 0; m < CPW; ++m) {
+    const int beg = (wid + m * (kNetInThreads / kWave)) * chunk, end = min(beg + chunk, np);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = beg + lane + (u == 1 ? 128 : u == 2 ? 64 : u == 3 ? 192 : 0);
+      xv[m][u] = q < end ? vb[src(q)] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < CPW; ++m) {
+    // lane l's k_img_sums value: its pixel squared (exact: a float's square
+    // has at most 48 significant bits), or 0; tree levels 128 and 64:
+    // L[l] = (L[l] + L[l + 128]) + (L[l + 64] + L[l + 192])
+    auto sq = [&](int u) {
+      const double x = (double)xv[m][u];
+      return x * x;
     };
-    // tree levels 128 and 64: L[l] = (L[l] + L[l + 128]) + (L[l + 64] + L[l + 192])
-    double s = (sq(lane) + sq(lane + 128)) + (sq(lane + 64) + sq(lane + 192));
+    double s = (sq(0) + sq(1)) + (sq(2) + sq(3));
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
-    if (lane == 0) csum[c] = s;
+    if (lane == 0) csum[wid + m * (kNetInThreads / kWave)] = s;
   }
   __syncthreads();
   if (t == 0) {
@@ -1159,8 +1175,8 @@ __global__ __launch_bounds__(kNetInThreads) void k_net_in(const float* v, int un
     uint4 q[2];
   } o;
   o.h[0] = f2bf((float)((double)mine / (double)sc));
-  o.h[1] = f2bf(f1[p]);
-  o.h[2] = f2bf(f2[p]);
+  o.h[1] = f2bf(g1);
+  o.h[2] = f2bf(g2);
 #pragma unroll
   for (int c = 3; c < 16; ++c) o.h[c] = 0;
   uint4* dst = reinterpret_cast<uint4*>(x0 + ((long)b * np + p) * 16);
