@@ -111,32 +111,51 @@ __device__ __forceinline__ int parents(int q, int (&o)[2], double (&w)[2]) {
 // Galerkin product R A P at coarse node (J, I): R = P^T / 4 gathers the fine
 // nodes 2J + a, 2I + b (a, b in -1..1, weights 1 and 1/2 per axis), each
 // fine stencil entry reaches 2J + a + dj, 2I + b + di, and P sends that node
-// to its coarse parents.  Also writes the FP32 copy and 1 / centre.
-__global__ void k_rap(Dims f, Dims c, const double* stf, double* stc, float* stc32, float* idg) {
+// to its coarse parents.  Also writes the FP32 copy and 1 / centre.  The 81
+// fine stencil entries are loaded first (at clamped indices past a wall,
+// which the sum skips), so the sum waits for one round trip, not one per
+// entry behind its zero test.
+__global__ __launch_bounds__(64) void k_rap(Dims f, Dims c, const double* stf, double* stc, float* stc32,
+                                            float* idg) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y;
   if (I >= c.nx || J >= c.ny) return;
   double acc[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) acc[k] = 0.0;
   const long pf = (long)f.ny * f.nx;
-  for (int a = -1; a <= 1; ++a) {
+  double svs[3][3][9];
+  bool okv[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
     bool oky;
-    const int jf = axis_idx(2 * J + a, f.ny, f.py, oky);
-    if (!oky) continue;
-    for (int b = -1; b <= 1; ++b) {
+    const int jf = axis_idx(2 * J + a - 1, f.ny, f.py, oky);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
       bool okx;
-      const int iF = axis_idx(2 * I + b, f.nx, f.px, okx);
-      if (!okx) continue;
-      const double wr = 0.25 * (a == 0 ? 1.0 : 0.5) * (b == 0 ? 1.0 : 0.5);
+      const int iF = axis_idx(2 * I + b - 1, f.nx, f.px, okx);
+      okv[a][b] = oky && okx;
       const long gf = (long)jf * f.nx + iF;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) svs[a][b][k] = stf[k * pf + gf];
+    }
+  }
+#pragma unroll
+  for (int a = -1; a <= 1; ++a) {
+#pragma unroll
+    for (int b = -1; b <= 1; ++b) {
+      if (!okv[a + 1][b + 1]) continue;
+      const double wr = 0.25 * (a == 0 ? 1.0 : 0.5) * (b == 0 ? 1.0 : 0.5);
+#pragma unroll
       for (int k = 0; k < 9; ++k) {
-        const double sv = stf[k * pf + gf];
+        const double sv = svs[a + 1][b + 1][k];
         if (sv == 0.0) continue;
         const int dj = k / 3 - 1, di = k % 3 - 1;
         int oy[2], ox[2];
         double wy[2], wx[2];
         const int ny_ = parents(a + dj, oy, wy), nx_ = parents(b + di, ox, wx);
+#pragma unroll
         for (int p = 0; p < ny_; ++p)
+#pragma unroll
           for (int q = 0; q < nx_; ++q) acc[3 * (oy[p] + 1) + (ox[q] + 1)] += wr * sv * wy[p] * wx[q];
       }
     }
