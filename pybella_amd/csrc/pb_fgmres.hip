@@ -151,6 +151,51 @@ __global__ __launch_bounds__(kFgThreads) void kf_orth1(const double* V, long n, 
   block_dots(V, n, K, w2, part, sh);
 }
 
+// kf_orth1 in one pass over the basis for K <= KM: each element's K basis
+// values stay in registers between forming w2 = w - V h1 and the partial
+// dots V^T w2.  Per thread the products and sums are kf_orth1's, in the
+// same order (the dot of basis vector k over the thread's elements in order,
+// then block sums of eight values at a time), so the results are bitwise
+// the same; the basis is read once instead of twice.
+template <int KM>
+__global__ __launch_bounds__(kFgThreads) void kf_orth1r(const double* V, long n, int K, const double* h1,
+                                                        const double* w, double* w2, double* part) {
+  __shared__ double h[kMaxM + 4];
+  __shared__ double sh[8 * (kFgThreads / kWave)];
+  if ((int)threadIdx.x < K) h[threadIdx.x] = h1[threadIdx.x];
+  __syncthreads();
+  double acc[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) acc[k] = 0.0;
+  for (long g = gfirst(); g < n; g += gstride()) {
+    double vk[KM];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) vk[k] = k < K ? V[(long)k * n + g] : 0.0;
+    double a = w[g];
+#pragma unroll
+    for (int k = 0; k < KM; ++k)
+      if (k < K) a -= h[k] * vk[k];
+    w2[g] = a;
+#pragma unroll
+    for (int k = 0; k < KM; ++k)
+      if (k < K) acc[k] += vk[k] * a;
+  }
+#pragma unroll
+  for (int k0 = 0; k0 < KM; k0 += 8) {
+    if (k0 >= K) break;
+    double s8[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s8[q] = acc[k0 + q];
+    block_sum_w<8, kFgThreads / kWave>(s8, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (k0 + q < K) part[(long)(k0 + q) * kFgBlocks + blockIdx.x] = s8[q];
+    }
+    __syncthreads();   // sh is reused by the next pass
+  }
+}
+
 // w = w2 - V h2, partial ||w||^2
 __global__ __launch_bounds__(kFgThreads) void kf_orth2(const double* V, long n, int K, const double* h2,
                                                        const double* w2, double* w, double* part3) {
@@ -185,24 +230,38 @@ __global__ __launch_bounds__(kFgThreads) void kf_norm(int j, int m, long n, cons
   const double hn = hn_s;
   const double inv = hn > 0.0 ? 1.0 / hn : 0.0;
   for (long q = gfirst(); q < n; q += gstride()) vnext[q] = w[q] * inv;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double* col = H + (long)j * (m + 1);
-    for (int i = 0; i <= j; ++i) col[i] = h1[i] + h2[i];
-    col[j + 1] = hn;
-    for (int i = 0; i < j; ++i) {
-      const double a = col[i], b = col[i + 1];
-      col[i] = cs[i] * a + sn[i] * b;
-      col[i + 1] = -sn[i] * a + cs[i] * b;
+  if (blockIdx.x == 0) {
+    // the column and the previous rotations staged in LDS by the whole block
+    // (parallel loads), rotated by one thread there, written back by all
+    __shared__ double cols[kMaxM + 2], css[kMaxM], sns[kMaxM];
+    for (int i = threadIdx.x; i <= j; i += kFgThreads) cols[i] = h1[i] + h2[i];
+    for (int i = threadIdx.x; i < j; i += kFgThreads) {
+      css[i] = cs[i];
+      sns[i] = sn[i];
     }
-    const double a = col[j], b = col[j + 1];
-    const double r = hypot(a, b);
-    cs[j] = r > 0.0 ? a / r : 1.0;
-    sn[j] = r > 0.0 ? b / r : 0.0;
-    col[j] = r;
-    col[j + 1] = 0.0;
-    g[j + 1] = -sn[j] * g[j];
-    g[j] = cs[j] * g[j];
-    est[j] = fabs(g[j + 1]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      cols[j + 1] = hn;
+      for (int i = 0; i < j; ++i) {
+        const double a = cols[i], b = cols[i + 1];
+        cols[i] = css[i] * a + sns[i] * b;
+        cols[i + 1] = -sns[i] * a + css[i] * b;
+      }
+      const double a = cols[j], b = cols[j + 1];
+      const double r = hypot(a, b);
+      const double c = r > 0.0 ? a / r : 1.0, sj = r > 0.0 ? b / r : 0.0;
+      cs[j] = c;
+      sn[j] = sj;
+      cols[j] = r;
+      cols[j + 1] = 0.0;
+      const double gj = g[j];
+      g[j + 1] = -sj * gj;
+      g[j] = c * gj;
+      est[j] = fabs(-sj * gj);
+    }
+    __syncthreads();
+    double* col = H + (long)j * (m + 1);
+    for (int i = threadIdx.x; i <= j + 1; i += kFgThreads) col[i] = cols[i];
   }
 }
 
@@ -322,8 +381,15 @@ int arnoldi_step(pb_op2d* op, pb_nn* nn, FgWork* f, int j, const pbmg::CycleOpts
   const dim3 gb(kFgBlocks), tb(kFgThreads);
   hipLaunchKernelGGL(kf_dots, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->w, f->part);
   hipLaunchKernelGGL(kf_colsum, dim3(K), tb, 0, s, (const double*)f->part, f->h1);
-  hipLaunchKernelGGL(kf_orth1, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h1, (const double*)f->w,
-                     f->w2, f->part);
+  if (K <= 16)
+    hipLaunchKernelGGL(kf_orth1r<16>, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h1,
+                       (const double*)f->w, f->w2, f->part);
+  else if (K <= 32)
+    hipLaunchKernelGGL(kf_orth1r<32>, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h1,
+                       (const double*)f->w, f->w2, f->part);
+  else
+    hipLaunchKernelGGL(kf_orth1, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h1, (const double*)f->w,
+                       f->w2, f->part);
   hipLaunchKernelGGL(kf_colsum, dim3(K), tb, 0, s, (const double*)f->part, f->h2);
   hipLaunchKernelGGL(kf_orth2, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h2, (const double*)f->w2,
                      f->w, f->part3);
