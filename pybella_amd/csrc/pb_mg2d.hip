@@ -546,15 +546,15 @@ __global__ __launch_bounds__(kPsX * kPsY) void kc_prolong_sweep(Dims d, Dims c, 
       });
 }
 
-// The network level's correction, both branches, in one block of 1024
-// threads with the level in LDS:
+// The network level's correction, one block of 1024 threads per branch
+// (blockIdx.x), with the level in LDS:
 //  1. z = scale * (network output) at every node, a duplicated end node's
 //     copies averaged (the network's nh x nw grid duplicates the end node of
 //     each periodic axis of the d.ny x d.nx level); each copy's product is
 //     rounded to float before the sum (FP contraction off there);
-//  2. z *= alpha per branch, alpha = (z . v) / (z . A z), the energy-norm
-//     (Galerkin) scale of the direction z: the network predicts the shape of
-//     the coarse correction, its scale is fitted here.  A residual fit
+//  2. z *= alpha, alpha = (z . v) / (z . A z), the energy-norm (Galerkin)
+//     scale of the direction z: the network predicts the shape of the coarse
+//     correction, its scale is fitted here.  A residual fit
 //     (A z . v) / (A z . A z) would weight the high-frequency part of A z and
 //     shrink the smooth modes the coarse level is there for.  The sums run in
 //     double, thread t over nodes t, t + 1024, ..., then the block sum;
@@ -571,95 +571,78 @@ __global__ __launch_bounds__(kNetFinThreads) void kc_net_finish(Dims d, int nh, 
                                                                 const float* scale, const float* st, const float* idg,
                                                                 const float* v, int sweeps, float omega,
                                                                 float* zfin) {
-  __shared__ float zb[2][2 * kNetFinMax];
-  __shared__ double sh[4 * (kNetFinThreads / kWave)];
-  __shared__ float alpha[2];
-  const int n = d.ny * d.nx, t = threadIdx.x;
-  const int pd = nh * nw;
-  // loads: per owned node its stencil row, diag_inv, v of both branches and
-  // the network's values at every copy of the node (q[b][0..3]: itself, the
-  // duplicated column, the duplicated row, the duplicated corner)
+  __shared__ float zb[2][kNetFinMax];
+  __shared__ double sh[2 * (kNetFinThreads / kWave)];
+  __shared__ float alpha;
+  const int n = d.ny * d.nx, t = threadIdx.x, br = blockIdx.x;
+  const float* qb = zo + br * nh * nw;
+  const float* vb = v + br * n;
+  const float sf = scale[br];
+  // loads: per owned node its stencil row, diag_inv, v and the network's
+  // values at every copy of the node (q[0..3]: itself, the duplicated
+  // column, the duplicated row, the duplicated corner)
   Nb nb[kNetFinNpt];
-  float ig[kNetFinNpt], vv[kNetFinNpt][2], q[kNetFinNpt][2][4];
+  float ig[kNetFinNpt], vv[kNetFinNpt], q[kNetFinNpt][4];
   bool dxs[kNetFinNpt], dys[kNetFinNpt];
-  const float sf[2] = {scale[0], scale[1]};
 #pragma unroll
   for (int m = 0; m < kNetFinNpt; ++m) {
     const int g = t + m * kNetFinThreads;
-    const bool own = g < n;
-    const int gg = own ? g : 0;
+    const int gg = g < n ? g : 0;
     load_row(d, st, gg, nb[m]);
     ig[m] = idg[gg];
-    vv[m][0] = v[gg];
-    vv[m][1] = v[n + gg];
+    vv[m] = vb[gg];
     const int j = gg / d.nx, i = gg - j * d.nx;
     dxs[m] = d.px && i == 0;
     dys[m] = d.py && j == 0;
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const float* qb = zo + b * pd;
-      q[m][b][0] = qb[j * nw + i];
-      q[m][b][1] = dxs[m] ? qb[j * nw + (nw - 1)] : 0.f;
-      q[m][b][2] = dys[m] ? qb[(nh - 1) * nw + i] : 0.f;
-      q[m][b][3] = dxs[m] && dys[m] ? qb[(nh - 1) * nw + (nw - 1)] : 0.f;
-    }
+    q[m][0] = qb[j * nw + i];
+    q[m][1] = dxs[m] ? qb[j * nw + (nw - 1)] : 0.f;
+    q[m][2] = dys[m] ? qb[(nh - 1) * nw + i] : 0.f;
+    q[m][3] = dxs[m] && dys[m] ? qb[(nh - 1) * nw + (nw - 1)] : 0.f;
   }
 #pragma unroll
   for (int m = 0; m < kNetFinNpt; ++m) {
 #pragma clang fp contract(off)
     const int g = t + m * kNetFinThreads;
     if (g < n) {
-      for (int b = 0; b < 2; ++b) {
-        float a = q[m][b][0] * sf[b];
-        float cnt = 1.f;
-        if (dxs[m]) {
-          a += q[m][b][1] * sf[b];
-          cnt += 1.f;
-        }
-        if (dys[m]) {
-          a += q[m][b][2] * sf[b];
-          cnt += 1.f;
-        }
-        if (dxs[m] && dys[m]) {
-          a += q[m][b][3] * sf[b];
-          cnt += 1.f;
-        }
-        zb[0][b * n + g] = a / cnt;
+      float a = q[m][0] * sf;
+      float cnt = 1.f;
+      if (dxs[m]) {
+        a += q[m][1] * sf;
+        cnt += 1.f;
       }
+      if (dys[m]) {
+        a += q[m][2] * sf;
+        cnt += 1.f;
+      }
+      if (dxs[m] && dys[m]) {
+        a += q[m][3] * sf;
+        cnt += 1.f;
+      }
+      zb[0][g] = a / cnt;
     }
   }
   __syncthreads();
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};   // (z.v, z.Az) per branch
+  double acc[2] = {0.0, 0.0};   // z.v, z.Az
 #pragma unroll
   for (int m = 0; m < kNetFinNpt; ++m) {
     const int g = t + m * kNetFinThreads;
     if (g < n) {
+      float az = 0.f;
 #pragma unroll
-      for (int ch = 0; ch < 2; ++ch) {
-        const float* zc = zb[0] + ch * n;
-        float az = 0.f;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) az += nb[m].s[k] * zc[nb[m].idx[k]];
-        const double zg = (double)zc[g];
-        acc[2 * ch] += zg * (double)vv[m][ch];
-        acc[2 * ch + 1] += zg * (double)az;
-      }
+      for (int k = 0; k < 9; ++k) az += nb[m].s[k] * zb[0][nb[m].idx[k]];
+      const double zg = (double)zb[0][g];
+      acc[0] += zg * (double)vv[m];
+      acc[1] += zg * (double)az;
     }
   }
-  block_sum_w<4, kNetFinThreads / kWave>(acc, sh);
-  if (t == 0) {
-    alpha[0] = acc[1] != 0.0 ? (float)(acc[0] / acc[1]) : 0.f;
-    alpha[1] = acc[3] != 0.0 ? (float)(acc[2] / acc[3]) : 0.f;
-  }
+  block_sum_w<2, kNetFinThreads / kWave>(acc, sh);
+  if (t == 0) alpha = acc[1] != 0.0 ? (float)(acc[0] / acc[1]) : 0.f;
   __syncthreads();
-  const float a0 = alpha[0], a1 = alpha[1];
+  const float al = alpha;
 #pragma unroll
   for (int m = 0; m < kNetFinNpt; ++m) {
     const int g = t + m * kNetFinThreads;
-    if (g < n) {
-      zb[0][g] *= a0;
-      zb[0][n + g] *= a1;
-    }
+    if (g < n) zb[0][g] *= al;
   }
   __syncthreads();
   int cur = 0;
@@ -668,14 +651,11 @@ __global__ __launch_bounds__(kNetFinThreads) void kc_net_finish(Dims d, int nh, 
     for (int m = 0; m < kNetFinNpt; ++m) {
       const int g = t + m * kNetFinThreads;
       if (g < n) {
+        const float* zc = zb[cur];
+        float acc1 = 0.f;
 #pragma unroll
-        for (int ch = 0; ch < 2; ++ch) {
-          const float* zc = zb[cur] + ch * n;
-          float acc1 = 0.f;
-#pragma unroll
-          for (int qq = 0; qq < 9; ++qq) acc1 += nb[m].s[qq] * zc[nb[m].idx[qq]];
-          zb[cur ^ 1][ch * n + g] = zc[g] + omega * (vv[m][ch] - acc1) * ig[m];
-        }
+        for (int qq = 0; qq < 9; ++qq) acc1 += nb[m].s[qq] * zc[nb[m].idx[qq]];
+        zb[cur ^ 1][g] = zc[g] + omega * (vv[m] - acc1) * ig[m];
       }
     }
     __syncthreads();
@@ -684,10 +664,7 @@ __global__ __launch_bounds__(kNetFinThreads) void kc_net_finish(Dims d, int nh, 
 #pragma unroll
   for (int m = 0; m < kNetFinNpt; ++m) {
     const int g = t + m * kNetFinThreads;
-    if (g < n) {
-      zfin[g] = zb[cur][g];
-      zfin[n + g] = zb[cur][n + g];
-    }
+    if (g < n) zfin[br * n + g] = zb[cur][g];
   }
 }
 
@@ -796,7 +773,7 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
       const float *zo = nullptr, *sc = nullptr;
       *rc = pb_nn_forward_level(nn, 2, d.ny, d.nx, H->nh, H->nw, (const float*)L.v, H->f1, H->f2, &zo, &sc, s);
       if (*rc) return nullptr;
-      hipLaunchKernelGGL(kc_net_finish, dim3(1), dim3(kNetFinThreads), 0, s, d, H->nh, H->nw, zo, sc,
+      hipLaunchKernelGGL(kc_net_finish, dim3(2), dim3(kNetFinThreads), 0, s, d, H->nh, H->nw, zo, sc,
                          (const float*)L.st, (const float*)L.idg, (const float*)L.v, std::max(0, o.coarse_sweeps),
                          o.omega, cur);
       return cur;

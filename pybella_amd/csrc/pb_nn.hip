@@ -1124,8 +1124,14 @@ __global__ __launch_bounds__(kNetInThreads) void k_net_in(const float* v, int un
   const int b = blockIdx.y, t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
   const int np = nh * nw;
   const float* vb = v + (long)b * uny * unx;
+  // row of pixel p: a float quotient corrected by one either way (exact for
+  // the at most 2^14 pixels of a level), not an integer division per value
+  const float rnw = 1.f / (float)nw;
   auto src = [&](int p) {
-    const int j = p / nw, i = p - j * nw;
+    int j = (int)((float)p * rnw);
+    j += (j + 1) * nw <= p ? 1 : 0;
+    j -= j * nw > p ? 1 : 0;
+    const int i = p - j * nw;
     return (j >= uny ? j - uny : j) * unx + (i >= unx ? i - unx : i);
   };
   const int p = blockIdx.x * kNetInThreads + t;
@@ -1143,8 +1149,10 @@ __global__ __launch_bounds__(kNetInThreads) void k_net_in(const float* v, int un
     const int beg = (wid + m * (kNetInThreads / kWave)) * chunk, end = min(beg + chunk, np);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int q = beg + lane + (u == 1 ? 128 : u == 2 ? 64 : u == 3 ? 192 : 0);
-      xv[m][u] = q < end ? vb[src(q)] : 0.f;
+      const int off = u == 1 ? 128 : u == 2 ? 64 : u == 3 ? 192 : 0;
+      const int q = beg + lane + off;
+      xv[m][u] = 0.f;
+      if (off < chunk && q < end) xv[m][u] = vb[src(q)];   // off < chunk: uniform, skips empty lanes' work
     }
   }
 #pragma unroll
