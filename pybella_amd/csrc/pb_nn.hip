@@ -237,6 +237,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_small(const bf16* __restrict__ 
   bool ok[9];
 #pragma unroll
   for (int tap = 0; tap < 9; ++tap) ok[tap] = fetch(tap, a[tap]);
+  const int co = t0 * 16 + (lane & 15);
+  const float bco = (bias && co < cout_valid) ? bias[co] : 0.f;   // with the other loads, not after the MFMAs
 #pragma unroll
   for (int e = 0; e < WQ; ++e) {
     const int q = (int)threadIdx.x + e * 256;
@@ -260,12 +262,11 @@ __global__ __launch_bounds__(256) void k_conv3x3_small(const bf16* __restrict__ 
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b, acc, 0, 0, 0);
     }
   }
-  const int co = t0 * 16 + (lane & 15);
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const long p = p0 + 4 * kg + e;
     if (p < npix && co < cout_valid) {
-      float v = acc[e] + (bias ? bias[co] : 0.f);
+      float v = acc[e] + bco;
       const long o = p * out_ctot + out_coff + co;
       if (pre) pre[p * pre_ctot + co] = f2bf(v);
       if (act) v = gelu(v);
