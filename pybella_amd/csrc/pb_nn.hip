@@ -393,14 +393,15 @@ __global__ void k_act_bwd(bf16* d, int ctot, int coff, int c, const bf16* pre, i
 // down-sampling by injection and its adjoint (accumulating)
 __global__ void k_down(const bf16* f, int fct, int fco, bf16* c, int cct, int cco, int ch, int nimg, int hc, int wc,
                        int hf, int wf) {
-  const long n = (long)nimg * hc * wc * ch;
-  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
-    const int cc = (int)(k % ch);
-    long t = k / ch;
-    const int x = (int)(t % wc);
+  // 32-bit indices (the host keeps every activation tensor below 2^31 values)
+  const int n = nimg * hc * wc * ch;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    int t = k / ch;
+    const int cc = k - t * ch;
+    const int x = t % wc;
     t /= wc;
-    const int y = (int)(t % hc);
-    const long b = t / hc;
+    const int y = t % hc;
+    const int b = t / hc;
     c[((b * hc + y) * wc + x) * cct + cco + cc] = f[((b * hf + 2 * y) * wf + 2 * x) * fct + fco + cc];
   }
 }
@@ -426,14 +427,15 @@ __global__ void k_down_bwd(const bf16* dc, int cct, int cco, bf16* df, int fct, 
 __global__ void k_up(const bf16* c, int cct, int cco, bf16* f, int fct, int fco, int ch, int nimg, int hc, int wc,
                      int hf, int wf) {
   const int cv = ch / 8;
-  const long n = (long)nimg * hf * wf * cv;
-  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
-    const long pix = k / cv;
-    const int c8 = (int)(k - pix * cv) * 8;
-    const int x = (int)(pix % wf);
-    const long t = pix / wf;
-    const int y = (int)(t % hf);
-    const long b = t / hf;
+  // 32-bit indices (the host keeps every activation tensor below 2^31 values)
+  const int n = nimg * hf * wf * cv;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int pix = k / cv;
+    const int c8 = (k - pix * cv) * 8;
+    const int x = pix % wf;
+    const int t = pix / wf;
+    const int y = t % hf;
+    const int b = t / hf;
     const int y0 = y >> 1, x0 = x >> 1;
     const int y1 = (y & 1) ? y0 + 1 : y0, x1 = (x & 1) ? x0 + 1 : x0;
     auto at = [&](int yy, int xx) {
@@ -753,6 +755,13 @@ int pack_weights(pb_nn* nn, hipStream_t s) {
 Acts* get_shape(pb_nn* nn, int nimg, int h, int w, bool train, int* rc) {
   for (Acts* a : nn->shapes)
     if (a->nimg == nimg && a->h == h && a->w == w && (a->train || !train)) return a;
+  // the widest activation (128 channels at full resolution) must index in
+  // 32 bits: the pooling and upsampling kernels use int indices
+  if (nimg < 1 || h < 1 || w < 1 || (long)nimg * h * w * 128 > 0x7fffffffL) {
+    *rc = pb_fail(nn->ctx, PB_ERR_ARG, "network: %d images of %d x %d pixels exceed 2^31 activation values", nimg,
+                  h, w);
+    return nullptr;
+  }
   Acts* a = new Acts();
   a->nimg = nimg;
   a->h = h;
