@@ -105,9 +105,14 @@ __device__ __forceinline__ void block_dots(const double* V, long n, int K, const
     for (int q = 0; q < 8; ++q) acc[q] = 0.0;
     for (long g = gfirst(); g < n; g += gstride()) {
       const double xg = x[g];
+      // unconditional loads (vector index clamped to K - 1), the sums guarded:
+      // a load behind the guard would wait for its result before the next
+      double vq[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) vq[q] = V[(long)min(k0 + q, K - 1) * n + g];
 #pragma unroll
       for (int q = 0; q < 8; ++q)
-        if (k0 + q < K) acc[q] += V[(long)(k0 + q) * n + g] * xg;
+        if (k0 + q < K) acc[q] += vq[q] * xg;
     }
     block_sum_w<8, kFgThreads / kWave>(acc, sh);
     if (threadIdx.x == 0) {
@@ -170,7 +175,7 @@ __global__ __launch_bounds__(kFgThreads) void kf_orth1r(const double* V, long n,
   for (long g = gfirst(); g < n; g += gstride()) {
     double vk[KM];
 #pragma unroll
-    for (int k = 0; k < KM; ++k) vk[k] = k < K ? V[(long)k * n + g] : 0.0;
+    for (int k = 0; k < KM; ++k) vk[k] = V[(long)min(k, K - 1) * n + g];   // unconditional, used for k < K
     double a = w[g];
 #pragma unroll
     for (int k = 0; k < KM; ++k)
@@ -185,7 +190,7 @@ __global__ __launch_bounds__(kFgThreads) void kf_orth1r(const double* V, long n,
     if (k0 >= K) break;
     double s8[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) s8[q] = acc[k0 + q];
+    for (int q = 0; q < 8; ++q) s8[q] = k0 + q < KM ? acc[k0 + q] : 0.0;   // KM may be 4
     block_sum_w<8, kFgThreads / kWave>(s8, sh);
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -194,6 +199,32 @@ __global__ __launch_bounds__(kFgThreads) void kf_orth1r(const double* V, long n,
     }
     __syncthreads();   // sh is reused by the next pass
   }
+}
+
+// kf_orth2 for K <= KM with the element's K basis values loaded together
+// (unconditional loads at clamped vector indices) before the sum, in
+// kf_orth2's order: bitwise the same
+template <int KM>
+__global__ __launch_bounds__(kFgThreads) void kf_orth2r(const double* V, long n, int K, const double* h2,
+                                                        const double* w2, double* w, double* part3) {
+  __shared__ double h[kMaxM + 4];
+  __shared__ double sh[kFgThreads / kWave];
+  if ((int)threadIdx.x < K) h[threadIdx.x] = h2[threadIdx.x];
+  __syncthreads();
+  double ss[1] = {0.0};
+  for (long g = gfirst(); g < n; g += gstride()) {
+    double vk[KM];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) vk[k] = V[(long)min(k, K - 1) * n + g];
+    double a = w2[g];
+#pragma unroll
+    for (int k = 0; k < KM; ++k)
+      if (k < K) a -= h[k] * vk[k];
+    w[g] = a;
+    ss[0] += a * a;
+  }
+  block_sum_w<1, kFgThreads / kWave>(ss, sh);
+  if (threadIdx.x == 0) part3[blockIdx.x] = ss[0];
 }
 
 // w = w2 - V h2, partial ||w||^2
@@ -381,18 +412,31 @@ int arnoldi_step(pb_op2d* op, pb_nn* nn, FgWork* f, int j, const pbmg::CycleOpts
   const dim3 gb(kFgBlocks), tb(kFgThreads);
   hipLaunchKernelGGL(kf_dots, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->w, f->part);
   hipLaunchKernelGGL(kf_colsum, dim3(K), tb, 0, s, (const double*)f->part, f->h1);
-  if (K <= 16)
-    hipLaunchKernelGGL(kf_orth1r<16>, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h1,
-                       (const double*)f->w, f->w2, f->part);
-  else if (K <= 32)
-    hipLaunchKernelGGL(kf_orth1r<32>, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h1,
-                       (const double*)f->w, f->w2, f->part);
+  // the register kernels load KM basis values per element unconditionally:
+  // the smallest KM that holds K
+#define PB_ORTH1R(KM)                                                                                  \
+  hipLaunchKernelGGL(kf_orth1r<KM>, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h1,     \
+                     (const double*)f->w, f->w2, f->part)
+  if (K <= 4) PB_ORTH1R(4);
+  else if (K <= 8) PB_ORTH1R(8);
+  else if (K <= 16) PB_ORTH1R(16);
+  else if (K <= 32) PB_ORTH1R(32);
   else
     hipLaunchKernelGGL(kf_orth1, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h1, (const double*)f->w,
                        f->w2, f->part);
+#undef PB_ORTH1R
   hipLaunchKernelGGL(kf_colsum, dim3(K), tb, 0, s, (const double*)f->part, f->h2);
-  hipLaunchKernelGGL(kf_orth2, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h2, (const double*)f->w2,
-                     f->w, f->part3);
+#define PB_ORTH2R(KM)                                                                                  \
+  hipLaunchKernelGGL(kf_orth2r<KM>, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h2,     \
+                     (const double*)f->w2, f->w, f->part3)
+  if (K <= 4) PB_ORTH2R(4);
+  else if (K <= 8) PB_ORTH2R(8);
+  else if (K <= 16) PB_ORTH2R(16);
+  else if (K <= 32) PB_ORTH2R(32);
+  else
+    hipLaunchKernelGGL(kf_orth2, gb, tb, 0, s, (const double*)f->V, n, K, (const double*)f->h2, (const double*)f->w2,
+                       f->w, f->part3);
+#undef PB_ORTH2R
   hipLaunchKernelGGL(kf_norm, gb, tb, 0, s, j, m, n, (const double*)f->part3, (const double*)f->w,
                      f->V + (long)(j + 1) * n, (const double*)f->h1, (const double*)f->h2, f->H, f->cs, f->sn, f->g,
                      f->est);

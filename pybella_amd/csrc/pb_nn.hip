@@ -1153,31 +1153,48 @@ __global__ __launch_bounds__(kNetInThreads) void k_net_in(const float* v, int un
   static_assert(kNetChunks % (kNetInThreads / kWave) == 0, "the chunks divide evenly over the waves");
   const float mine = p < np ? vb[src(p)] : 0.f;
   const float g1 = p < np ? f1[p] : 0.f, g2 = p < np ? f2[p] : 0.f;
-  float xv[CPW][4];   // lane offsets 0, 128, 64, 192: the tree's pair order
+  // lane offsets 0, 128, 64, 192 (the tree's pair order); all 64 loads are
+  // issued unconditionally at clamped pixels, with no branch between them,
+  // and the values past a chunk's end are zeroed after the last load
+  float xv[CPW][4];
+#pragma unroll
+  for (int m = 0; m < CPW; ++m)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int off = u == 1 ? 128 : u == 2 ? 64 : u == 3 ? 192 : 0;
+      const int q = (wid + m * (kNetInThreads / kWave)) * chunk + lane + off;
+      xv[m][u] = vb[src(min(q, np - 1))];
+    }
 #pragma unroll
   for (int m = 0; m < CPW; ++m) {
     const int beg = (wid + m * (kNetInThreads / kWave)) * chunk, end = min(beg + chunk, np);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int off = u == 1 ? 128 : u == 2 ? 64 : u == 3 ? 192 : 0;
-      const int q = beg + lane + off;
-      xv[m][u] = 0.f;
-      if (off < chunk && q < end) xv[m][u] = vb[src(q)];   // off < chunk: uniform, skips empty lanes' work
+      if (off >= chunk || beg + lane + off >= end) xv[m][u] = 0.f;
     }
   }
+  // lane l's k_img_sums values: its pixels squared (exact: a float's square
+  // has at most 48 significant bits), or 0; tree levels 128 and 64:
+  // L[l] = (L[l] + L[l + 128]) + (L[l + 64] + L[l + 192]); then each chunk's
+  // 64-lane butterfly, level by level over all chunks of the wave (the
+  // chunks' shuffles of a level in flight together)
+  double sm[CPW];
 #pragma unroll
   for (int m = 0; m < CPW; ++m) {
-    // lane l's k_img_sums value: its pixel squared (exact: a float's square
-    // has at most 48 significant bits), or 0; tree levels 128 and 64:
-    // L[l] = (L[l] + L[l + 128]) + (L[l + 64] + L[l + 192])
     auto sq = [&](int u) {
       const double x = (double)xv[m][u];
       return x * x;
     };
-    double s = (sq(0) + sq(1)) + (sq(2) + sq(3));
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
-    if (lane == 0) csum[wid + m * (kNetInThreads / kWave)] = s;
+    sm[m] = (sq(0) + sq(1)) + (sq(2) + sq(3));
   }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int m = 0; m < CPW; ++m) sm[m] += __shfl_xor(sm[m], off, kWave);
+  if (lane == 0)
+#pragma unroll
+    for (int m = 0; m < CPW; ++m) csum[wid + m * (kNetInThreads / kWave)] = sm[m];
   __syncthreads();
   if (t == 0) {
     double tot = 0.0;
