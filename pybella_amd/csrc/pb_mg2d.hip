@@ -384,31 +384,6 @@ __global__ __launch_bounds__(kBlockSweepThreads) void kc_sweeps_block(Dims d, co
   }
 }
 
-// full weighting onto the next level, both branches
-__global__ void kc_restrict(Dims f, Dims c, const float* r, float* vc) {
-  const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y;
-  if (I >= c.nx || J >= c.ny) return;
-  const long pf = (long)f.ny * f.nx;
-  float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-  for (int a = -1; a <= 1; ++a) {
-    bool oky;
-    const int jf = axis_idx(2 * J + a, f.ny, f.py, oky);
-#pragma unroll
-    for (int b = -1; b <= 1; ++b) {
-      bool okx;
-      const int iF = axis_idx(2 * I + b, f.nx, f.px, okx);
-      const float w = (oky && okx) ? 0.25f * (a == 0 ? 1.f : 0.5f) * (b == 0 ? 1.f : 0.5f) : 0.f;
-      const long gf = (long)jf * f.nx + iF;
-      s0 += w * r[gf];
-      s1 += w * r[pf + gf];
-    }
-  }
-  const long o = (long)J * c.nx + I, pc = (long)c.ny * c.nx;
-  vc[o] = s0;
-  vc[pc + o] = s1;
-}
-
 // bilinear value of the coarse field e at fine node (j, i)
 __device__ __forceinline__ float interp(const Dims& c, const float* e, int j, int i) {
   const int J0 = j >> 1, I0 = i >> 1;
@@ -506,23 +481,77 @@ __global__ __launch_bounds__(kPsX * kPsY) void kc_sweep(Dims d, const float* st,
       });
 }
 
-// r = v - A z
-__global__ __launch_bounds__(kPsX * kPsY) void kc_resid(Dims d, const float* st, const float* v, const float* z,
-                                                       float* r) {
-  const long n = (long)d.ny * d.nx;
-  tile9(
-      d, [&](int ch, int, int, long gq) { return z[ch * n + gq]; },
-      [&](long g, const float (&zn)[2][9]) {
-        float sk[9];
-        load_st(d, st, g, sk);
+// v_c = R (v - A z): the residual of both branches and its full-weighting
+// restriction in one launch (no fine residual in memory).  Block (bx, by)
+// holds z of its 64 x 4 fine tile with a two-node halo and forms the
+// residual on the tile with a one-node halo in LDS, at the halo's wrapped
+// or clamped neighbour coordinates as the stencil's neighbour indices take
+// them, with kc_resid's arithmetic; then the 2 x 32 coarse nodes whose fine
+// centres lie in the tile sum their nine weighted residuals in the order of
+// a gather of each fine value on its own.  A fine node past a wall, whose
+// weight is 0, adds +0 where the gather added 0 * r of the clamped node:
+// the same sum, since a round-to-nearest sum from +0 is never -0.
+__global__ __launch_bounds__(kPsX * kPsY) void kc_resid_restrict(Dims d, Dims c, const float* st, const float* v,
+                                                                const float* z, float* vc) {
+  __shared__ float zp[2][kPsY + 4][kPsX + 4];   // z at rows y0 - 2 .., cols x0 - 2 ..
+  __shared__ float rp[2][kPsY + 2][kPsX + 2];   // r at rows y0 - 1 .., cols x0 - 1 ..
+  const long n = (long)d.ny * d.nx, pc = (long)c.ny * c.nx;
+  const int x0 = blockIdx.x * kPsX, y0 = blockIdx.y * kPsY;
+  const int t = threadIdx.y * kPsX + threadIdx.x;
+  for (int q = t; q < (kPsY + 4) * (kPsX + 4); q += kPsX * kPsY) {
+    const int r = q / (kPsX + 4), cc = q - r * (kPsX + 4);
+    const int j = y0 - 2 + r, i = x0 - 2 + cc;
+    if (j > d.ny || i > d.nx) continue;   // no residual needed here has it as a neighbour
+    bool ok;
+    const long gq = (long)axis_idx(j, d.ny, d.py, ok) * d.nx + axis_idx(i, d.nx, d.px, ok);
 #pragma unroll
-        for (int ch = 0; ch < 2; ++ch) {
-          float acc = 0.f;
+    for (int ch = 0; ch < 2; ++ch) zp[ch][r][cc] = z[ch * n + gq];
+  }
+  __syncthreads();
+  for (int q = t; q < (kPsY + 2) * (kPsX + 2); q += kPsX * kPsY) {
+    const int rr = q / (kPsX + 2), rc = q - rr * (kPsX + 2);
+    const int j = y0 - 1 + rr, i = x0 - 1 + rc;
+    // the restriction reads a residual at a wrapped node, or at an in-range
+    // node; past a wall (weight 0) and past the grid's end it reads none
+    const bool need = j < d.ny && i < d.nx && (j >= 0 || d.py) && (i >= 0 || d.px);
+    if (!need) {
+      rp[0][rr][rc] = rp[1][rr][rc] = 0.f;
+      continue;
+    }
+    bool ok;
+    const long g = (long)axis_idx(j, d.ny, d.py, ok) * d.nx + axis_idx(i, d.nx, d.px, ok);
+    float sk[9];
+    load_st(d, st, g, sk);
 #pragma unroll
-          for (int k = 0; k < 9; ++k) acc += sk[k] * zn[ch][k];
-          r[ch * n + g] = v[ch * n + g] - acc;
-        }
-      });
+    for (int ch = 0; ch < 2; ++ch) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) acc += sk[k] * zp[ch][rr + k / 3][rc + k % 3];
+      rp[ch][rr][rc] = v[ch * n + g] - acc;
+    }
+  }
+  __syncthreads();
+  if (t >= 2 * (kPsX / 2)) return;
+  const int J = y0 / 2 + t / (kPsX / 2), I = x0 / 2 + t % (kPsX / 2);
+  if (J >= c.ny || I >= c.nx) return;
+  float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+  for (int a = -1; a <= 1; ++a) {
+    bool oky;
+    axis_idx(2 * J + a, d.ny, d.py, oky);
+#pragma unroll
+    for (int b = -1; b <= 1; ++b) {
+      bool okx;
+      axis_idx(2 * I + b, d.nx, d.px, okx);
+      const float w = (oky && okx) ? 0.25f * (a == 0 ? 1.f : 0.5f) * (b == 0 ? 1.f : 0.5f) : 0.f;
+      const int rr = 2 * J + a - (y0 - 1), rc = 2 * I + b - (x0 - 1);
+      s0 += w * rp[0][rr][rc];
+      s1 += w * rp[1][rr][rc];
+    }
+  }
+  const long o = (long)J * c.nx + I;
+  vc[o] = s0;
+  vc[pc + o] = s1;
 }
 
 // z' = z + P e, then out = z' + omega (v - A z') / d
@@ -728,7 +757,6 @@ int layout(pb_op2d* op, Hier* H) {
       rc |= hmalloc(op, H, &L.v, 2 * n);
       rc |= hmalloc(op, H, &L.z, 2 * n);
       rc |= hmalloc(op, H, &L.t, 2 * n);
-      rc |= hmalloc(op, H, &L.r, 2 * n);
     }
   }
   rc |= hmalloc(op, H, &H->id0, op->n);
@@ -806,11 +834,9 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
     hipLaunchKernelGGL(kc_init, dim3(std::min(nb, 1024)), dim3(256), 0, s, n, (const float*)L.idg,
                        (const float*)L.v, o.omega, cur);
   }
-  hipLaunchKernelGGL(kc_resid, tile_grid(d), kTileBlk, 0, s, d, (const float*)L.st, (const float*)L.v,
-                     (const float*)cur, L.r);
   Level& C = H->L[l + 1];
-  hipLaunchKernelGGL(kc_restrict, dim3(ceil_div(C.d.nx, 64), C.d.ny), dim3(64), 0, s, d, C.d, (const float*)L.r,
-                     C.v);
+  hipLaunchKernelGGL(kc_resid_restrict, tile_grid(d), kTileBlk, 0, s, d, C.d, (const float*)L.st,
+                     (const float*)L.v, (const float*)cur, C.v);
   const float* e = vcycle(op, H, nn, l + 1, last, o, s, rc);
   if (!e) return nullptr;
   // coarse correction and post-smoothing (at least one sweep)

@@ -39,7 +39,6 @@ struct Level {
   float* v = nullptr;       // [2][ny][nx] right-hand sides (the two branches)
   float* z = nullptr;       // [2][ny][nx] iterate
   float* t = nullptr;       // [2][ny][nx] second iterate buffer
-  float* r = nullptr;       // [2][ny][nx] residual
 };
 
 struct Hier {
