@@ -575,6 +575,82 @@ __global__ __launch_bounds__(kPsX * kPsY) void kc_prolong_sweep(Dims d, Dims c, 
       });
 }
 
+// kc_prolong_sweep followed by one kc_sweep, in one launch: z' = z + P e on
+// the tile with a two-node halo, the first sweep on the tile with a
+// one-node halo (at a periodic edge at the wrapped node; at a wall the halo
+// position takes its clamped node's value, which is what the second sweep's
+// gather reads there), the second sweep at the tile's nodes.  Per node the
+// arithmetic of the two launches.
+__global__ __launch_bounds__(kPsX * kPsY) void kc_prolong_sweep2(Dims d, Dims c, const float* st, const float* idg,
+                                                                const float* v, const float* z, const float* e,
+                                                                float omega, float* out) {
+  __shared__ float zp[2][kPsY + 4][kPsX + 4];   // z' at rows y0 - 2 .., cols x0 - 2 ..
+  __shared__ float s1[2][kPsY + 2][kPsX + 2];   // first sweep at rows y0 - 1 .., cols x0 - 1 ..
+  const long n = (long)d.ny * d.nx, pc = (long)c.ny * c.nx;
+  const int x0 = blockIdx.x * kPsX, y0 = blockIdx.y * kPsY;
+  const int t = threadIdx.y * kPsX + threadIdx.x;
+  for (int q = t; q < (kPsY + 4) * (kPsX + 4); q += kPsX * kPsY) {
+    const int r = q / (kPsX + 4), cc = q - r * (kPsX + 4);
+    const int j = y0 - 2 + r, i = x0 - 2 + cc;
+    if (j > d.ny + 1 || i > d.nx + 1) continue;   // no first-sweep value needed here has it as a neighbour
+    bool ok;
+    const int jq = axis_idx(j, d.ny, d.py, ok), iq = axis_idx(i, d.nx, d.px, ok);
+    const long gq = (long)jq * d.nx + iq;
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) zp[ch][r][cc] = z[ch * n + gq] + interp(c, e + ch * pc, jq, iq);
+  }
+  __syncthreads();
+  // first sweep where its position is a node of the grid or, on a periodic
+  // axis, wraps to one
+  auto inside = [](int k, int nk, int per) { return per ? (k >= -1 && k <= nk) : (k >= 0 && k < nk); };
+  for (int q = t; q < (kPsY + 2) * (kPsX + 2); q += kPsX * kPsY) {
+    const int rr = q / (kPsX + 2), rc = q - rr * (kPsX + 2);
+    const int j = y0 - 1 + rr, i = x0 - 1 + rc;
+    if (!inside(j, d.ny, d.py) || !inside(i, d.nx, d.px)) continue;
+    bool ok;
+    const long g = (long)axis_idx(j, d.ny, d.py, ok) * d.nx + axis_idx(i, d.nx, d.px, ok);
+    float sk[9];
+    load_st(d, st, g, sk);
+    const float ig = idg[g];
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) acc += sk[k] * zp[ch][rr + k / 3][rc + k % 3];
+      s1[ch][rr][rc] = zp[ch][rr + 1][rc + 1] + omega * (v[ch * n + g] - acc) * ig;
+    }
+  }
+  __syncthreads();
+  // wall halo: the clamped node's value (its position lies in the same tile)
+  for (int q = t; q < (kPsY + 2) * (kPsX + 2); q += kPsX * kPsY) {
+    const int rr = q / (kPsX + 2), rc = q - rr * (kPsX + 2);
+    const int j = y0 - 1 + rr, i = x0 - 1 + rc;
+    if (j < -1 || j > d.ny || i < -1 || i > d.nx) continue;
+    if (inside(j, d.ny, d.py) && inside(i, d.nx, d.px)) continue;
+    // an axis on which the position is inside keeps it (a periodic one holds
+    // the wrapped node's value already); a wall axis clamps
+    bool ok;
+    const int mr = inside(j, d.ny, d.py) ? rr : axis_idx(j, d.ny, d.py, ok) - (y0 - 1);
+    const int mc = inside(i, d.nx, d.px) ? rc : axis_idx(i, d.nx, d.px, ok) - (x0 - 1);
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) s1[ch][rr][rc] = s1[ch][mr][mc];
+  }
+  __syncthreads();
+  const int j = y0 + threadIdx.y, i = x0 + threadIdx.x;
+  if (j >= d.ny || i >= d.nx) return;
+  const long g = (long)j * d.nx + i;
+  float sk[9];
+  load_st(d, st, g, sk);
+  const float ig = idg[g];
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc += sk[k] * s1[ch][threadIdx.y + k / 3][threadIdx.x + k % 3];
+    out[ch * n + g] = s1[ch][threadIdx.y + 1][threadIdx.x + 1] + omega * (v[ch * n + g] - acc) * ig;
+  }
+}
+
 // The network level's correction, one block of 1024 threads per branch
 // (blockIdx.x), with the level in LDS:
 //  1. z = scale * (network output) at every node, a duplicated end node's
@@ -840,11 +916,17 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
   const float* e = vcycle(op, H, nn, l + 1, last, o, s, rc);
   if (!e) return nullptr;
   // coarse correction and post-smoothing (at least one sweep)
-  hipLaunchKernelGGL(kc_prolong_sweep, tile_grid(d), kTileBlk, 0, s, d,
-                     C.d, (const float*)L.st, (const float*)L.idg, (const float*)L.v, (const float*)cur, e, o.omega,
-                     alt);
+  int done = 1;
+  if (o.nu_post >= 2) {   // the prolongation sweep and the next sweep in one launch
+    hipLaunchKernelGGL(kc_prolong_sweep2, tile_grid(d), kTileBlk, 0, s, d, C.d, (const float*)L.st,
+                       (const float*)L.idg, (const float*)L.v, (const float*)cur, e, o.omega, alt);
+    done = 2;
+  } else {
+    hipLaunchKernelGGL(kc_prolong_sweep, tile_grid(d), kTileBlk, 0, s, d, C.d, (const float*)L.st,
+                       (const float*)L.idg, (const float*)L.v, (const float*)cur, e, o.omega, alt);
+  }
   swap();
-  for (int k = 1; k < o.nu_post; ++k) {
+  for (int k = done; k < o.nu_post; ++k) {
     hipLaunchKernelGGL(kc_sweep, tile_grid(d), kTileBlk, 0, s, d, (const float*)L.st, (const float*)L.idg,
                        (const float*)L.v, (const float*)cur, o.omega, alt);
     swap();
