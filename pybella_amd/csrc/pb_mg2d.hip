@@ -554,6 +554,119 @@ __global__ __launch_bounds__(kPsX * kPsY) void kc_resid_restrict(Dims d, Dims c,
   vc[pc + o] = s1;
 }
 
+// kc_first followed by kc_resid_restrict, in one launch: t = omega v / d on
+// the tile with a three-node halo, kc_first's z (the first two sweeps from
+// zero, a stencil over t) on the tile with a two-node halo (at a periodic
+// edge at the wrapped node; at a wall the halo position takes its clamped
+// node's value), the residual v - A z on the tile with a one-node halo and
+// its restriction as in kc_resid_restrict; z of the tile's nodes is stored.
+// Per node the arithmetic of the two launches.
+__global__ __launch_bounds__(kPsX * kPsY) void kc_first_restrict(Dims d, Dims c, const float* st, const float* idg,
+                                                                const float* v, float omega, float* z, float* vc) {
+  __shared__ float tp[2][kPsY + 6][kPsX + 6];   // omega v / d at rows y0 - 3 .., cols x0 - 3 ..
+  __shared__ float zp[2][kPsY + 4][kPsX + 4];   // z at rows y0 - 2 .., cols x0 - 2 ..
+  __shared__ float rp[2][kPsY + 2][kPsX + 2];   // r at rows y0 - 1 .., cols x0 - 1 ..
+  const long n = (long)d.ny * d.nx, pc = (long)c.ny * c.nx;
+  const int x0 = blockIdx.x * kPsX, y0 = blockIdx.y * kPsY;
+  const int t = threadIdx.y * kPsX + threadIdx.x;
+  for (int q = t; q < (kPsY + 6) * (kPsX + 6); q += kPsX * kPsY) {
+    const int r = q / (kPsX + 6), cc = q - r * (kPsX + 6);
+    const int j = y0 - 3 + r, i = x0 - 3 + cc;
+    if (j > d.ny + 2 || i > d.nx + 2) continue;   // no z needed here has it as a neighbour
+    bool ok;
+    const long gq = (long)axis_idx(j, d.ny, d.py, ok) * d.nx + axis_idx(i, d.nx, d.px, ok);
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) tp[ch][r][cc] = omega * v[ch * n + gq] * idg[gq];
+  }
+  __syncthreads();
+  // z where its position is a node of the grid or, on a periodic axis,
+  // wraps to one (the residual's neighbours reach two nodes past the tile)
+  auto inside = [](int k, int nk, int per) { return per ? (k >= -2 && k <= nk + 1) : (k >= 0 && k < nk); };
+  for (int q = t; q < (kPsY + 4) * (kPsX + 4); q += kPsX * kPsY) {
+    const int r = q / (kPsX + 4), cc = q - r * (kPsX + 4);
+    const int j = y0 - 2 + r, i = x0 - 2 + cc;
+    if (!inside(j, d.ny, d.py) || !inside(i, d.nx, d.px)) continue;
+    bool ok;
+    const long g = (long)axis_idx(j, d.ny, d.py, ok) * d.nx + axis_idx(i, d.nx, d.px, ok);
+    float sk[9];
+    load_st(d, st, g, sk);
+    const float ig = idg[g];
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) acc += sk[k] * tp[ch][r + k / 3][cc + k % 3];
+      const float vg = v[ch * n + g];
+      zp[ch][r][cc] = omega * vg * ig + omega * (vg - acc) * ig;
+    }
+  }
+  __syncthreads();
+  // wall halo one node past the tile's nodes: the clamped node's value (an
+  // axis on which the position is inside keeps it)
+  for (int q = t; q < (kPsY + 4) * (kPsX + 4); q += kPsX * kPsY) {
+    const int r = q / (kPsX + 4), cc = q - r * (kPsX + 4);
+    const int j = y0 - 2 + r, i = x0 - 2 + cc;
+    if (j < -1 || j > d.ny || i < -1 || i > d.nx) continue;
+    if (inside(j, d.ny, d.py) && inside(i, d.nx, d.px)) continue;
+    bool ok;
+    const int mr = inside(j, d.ny, d.py) ? r : axis_idx(j, d.ny, d.py, ok) - (y0 - 2);
+    const int mc = inside(i, d.nx, d.px) ? cc : axis_idx(i, d.nx, d.px, ok) - (x0 - 2);
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) zp[ch][r][cc] = zp[ch][mr][mc];
+  }
+  __syncthreads();
+  {
+    const int j = y0 + threadIdx.y, i = x0 + threadIdx.x;
+    if (j < d.ny && i < d.nx) {
+      const long g = (long)j * d.nx + i;
+#pragma unroll
+      for (int ch = 0; ch < 2; ++ch) z[ch * n + g] = zp[ch][threadIdx.y + 2][threadIdx.x + 2];
+    }
+  }
+  for (int q = t; q < (kPsY + 2) * (kPsX + 2); q += kPsX * kPsY) {
+    const int rr = q / (kPsX + 2), rc = q - rr * (kPsX + 2);
+    const int j = y0 - 1 + rr, i = x0 - 1 + rc;
+    const bool need = j < d.ny && i < d.nx && (j >= 0 || d.py) && (i >= 0 || d.px);
+    if (!need) {
+      rp[0][rr][rc] = rp[1][rr][rc] = 0.f;
+      continue;
+    }
+    bool ok;
+    const long g = (long)axis_idx(j, d.ny, d.py, ok) * d.nx + axis_idx(i, d.nx, d.px, ok);
+    float sk[9];
+    load_st(d, st, g, sk);
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) acc += sk[k] * zp[ch][rr + k / 3][rc + k % 3];
+      rp[ch][rr][rc] = v[ch * n + g] - acc;
+    }
+  }
+  __syncthreads();
+  if (t >= 2 * (kPsX / 2)) return;
+  const int J = y0 / 2 + t / (kPsX / 2), I = x0 / 2 + t % (kPsX / 2);
+  if (J >= c.ny || I >= c.nx) return;
+  float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+  for (int a = -1; a <= 1; ++a) {
+    bool oky;
+    axis_idx(2 * J + a, d.ny, d.py, oky);
+#pragma unroll
+    for (int b = -1; b <= 1; ++b) {
+      bool okx;
+      axis_idx(2 * I + b, d.nx, d.px, okx);
+      const float w = (oky && okx) ? 0.25f * (a == 0 ? 1.f : 0.5f) * (b == 0 ? 1.f : 0.5f) : 0.f;
+      const int rr = 2 * J + a - (y0 - 1), rc = 2 * I + b - (x0 - 1);
+      s0 += w * rp[0][rr][rc];
+      s1 += w * rp[1][rr][rc];
+    }
+  }
+  const long o = (long)J * c.nx + I;
+  vc[o] = s0;
+  vc[pc + o] = s1;
+}
+
 // z' = z + P e, then out = z' + omega (v - A z') / d
 __global__ __launch_bounds__(kPsX * kPsY) void kc_prolong_sweep(Dims d, Dims c, const float* st, const float* idg,
                                                                const float* v, const float* z, const float* e,
@@ -897,22 +1010,27 @@ const float* vcycle(pb_op2d* op, Hier* H, pb_nn* nn, int l, int last, const Cycl
     }
     return cur;
   }
-  // pre-smoothing
-  if (o.nu_pre >= 2) {
-    hipLaunchKernelGGL(kc_first, tile_grid(d), kTileBlk, 0, s, d, (const float*)L.st, (const float*)L.idg,
-                       (const float*)L.v, o.omega, cur);
-    for (int k = 2; k < o.nu_pre; ++k) {
-      hipLaunchKernelGGL(kc_sweep, tile_grid(d), kTileBlk, 0, s, d, (const float*)L.st, (const float*)L.idg,
-                         (const float*)L.v, (const float*)cur, o.omega, alt);
-      swap();
-    }
-  } else {
-    hipLaunchKernelGGL(kc_init, dim3(std::min(nb, 1024)), dim3(256), 0, s, n, (const float*)L.idg,
-                       (const float*)L.v, o.omega, cur);
-  }
+  // pre-smoothing, residual and restriction
   Level& C = H->L[l + 1];
-  hipLaunchKernelGGL(kc_resid_restrict, tile_grid(d), kTileBlk, 0, s, d, C.d, (const float*)L.st,
-                     (const float*)L.v, (const float*)cur, C.v);
+  if (o.nu_pre == 2) {   // the first two sweeps, the residual and its restriction in one launch
+    hipLaunchKernelGGL(kc_first_restrict, tile_grid(d), kTileBlk, 0, s, d, C.d, (const float*)L.st,
+                       (const float*)L.idg, (const float*)L.v, o.omega, cur, C.v);
+  } else {
+    if (o.nu_pre > 2) {
+      hipLaunchKernelGGL(kc_first, tile_grid(d), kTileBlk, 0, s, d, (const float*)L.st, (const float*)L.idg,
+                         (const float*)L.v, o.omega, cur);
+      for (int k = 2; k < o.nu_pre; ++k) {
+        hipLaunchKernelGGL(kc_sweep, tile_grid(d), kTileBlk, 0, s, d, (const float*)L.st, (const float*)L.idg,
+                           (const float*)L.v, (const float*)cur, o.omega, alt);
+        swap();
+      }
+    } else {
+      hipLaunchKernelGGL(kc_init, dim3(std::min(nb, 1024)), dim3(256), 0, s, n, (const float*)L.idg,
+                         (const float*)L.v, o.omega, cur);
+    }
+    hipLaunchKernelGGL(kc_resid_restrict, tile_grid(d), kTileBlk, 0, s, d, C.d, (const float*)L.st,
+                       (const float*)L.v, (const float*)cur, C.v);
+  }
   const float* e = vcycle(op, H, nn, l + 1, last, o, s, rc);
   if (!e) return nullptr;
   // coarse correction and post-smoothing (at least one sweep)
