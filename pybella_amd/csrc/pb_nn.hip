@@ -188,12 +188,16 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ in, in
 // the first MFMA.  Per output the MFMAs are issued in k_conv3x3's order (tap
 // by tap, input chunks in order), so the sums are bitwise those of k_conv3x3.
 // ---------------------------------------------------------------------------
+// dn (optional): the next level's input, [nimg][(h + 1) / 2][(w + 1) / 2][dn_ctot]
+// channels 0 .. cout_valid - 1: the output at every pixel of even row and
+// column (the 2x subsampling k_down makes of the output channels, written
+// here instead of by a launch of its own).
 template <int CIN>
 __global__ __launch_bounds__(256) void k_conv3x3_small(const bf16* __restrict__ in, int in_ctot, int in_coff,
                                                        const bf16* __restrict__ wg, const float* __restrict__ bias,
                                                        int act, bf16* out, int out_ctot, int out_coff, int cout_valid,
                                                        bf16* pre, int pre_ctot, float* out_f32, int nimg, int h,
-                                                       int w) {
+                                                       int w, bf16* dn, int dn_ctot) {
   constexpr int NCI = CIN / 16;
   constexpr int WST = CIN + 4;   // rows padded by 4 bf16 against bank conflicts
   __shared__ bf16 wsh[9 * 16 * WST];
@@ -272,6 +276,11 @@ __global__ __launch_bounds__(256) void k_conv3x3_small(const bf16* __restrict__ 
       if (act) v = gelu(v);
       if (out) out[o] = f2bf(v);
       if (out_f32 && co == 0) out_f32[p] = v;
+      if (dn) {   // small levels: the pixel index fits 32 bits
+        const int pi = (int)p, t1 = pi / w, xw = pi - t1 * w, b = t1 / h, yh = t1 - b * h;
+        if (((xw | yh) & 1) == 0)
+          dn[(((long)b * ((h + 1) / 2) + (yh >> 1)) * ((w + 1) / 2) + (xw >> 1)) * dn_ctot + co] = f2bf(v);
+      }
     }
   }
 }
@@ -860,8 +869,12 @@ long conv_slots(int nt) {
   return slots[nt];
 }
 
+void down(Acts* a, int lf, const bf16* f, int fct, int fco, bf16* c, int cct, int cco, int ch, hipStream_t s);
+
+// dn (optional): also the 2x subsampling of the output channels into the
+// next level's input dn (channels 0.., dn_ct per pixel), as down() makes it
 void conv(pb_nn* nn, Acts* a, int l, int level, const bf16* in, int in_ct, int in_co, bf16* out, int out_ct,
-          int out_co, bf16* pre, float* out_f32, hipStream_t s) {
+          int out_co, bf16* pre, float* out_f32, hipStream_t s, bf16* dn = nullptr, int dn_ct = 0) {
   const LayerSpec& L = kLayers[l];
   const int cin = pad16(L.cin_v), nt = pad16(L.cout_v) / 16;
   const long npix = (long)a->nimg * a->lv[level].h * a->lv[level].w;
@@ -880,7 +893,7 @@ void conv(pb_nn* nn, Acts* a, int l, int level, const bf16* in, int in_ct, int i
     const dim3 gs_((unsigned)bx, nt), bs_(256);
 #define PB_CONVS(CI)                                                                                         \
   hipLaunchKernelGGL((k_conv3x3_small<CI>), gs_, bs_, 0, s, in, in_ct, in_co, nn->wf[l], bias, L.act, out, out_ct, \
-                     out_co, L.cout_v, pre, pad16(L.cout_v), out_f32, a->nimg, h, w)
+                     out_co, L.cout_v, pre, pad16(L.cout_v), out_f32, a->nimg, h, w, dn, dn_ct)
     switch (cin) {
       case 16: PB_CONVS(16); break;
       case 32: PB_CONVS(32); break;
@@ -919,6 +932,7 @@ void conv(pb_nn* nn, Acts* a, int l, int level, const bf16* in, int in_ct, int i
     }
   }
 #undef PB_CONV
+  if (dn) down(a, level, out, out_ct, out_co, dn, dn_ct, 0, L.cout_v, s);
 }
 
 // input gradient of layer l: din (cin_p channels) from dz (cout_p channels)
@@ -1000,14 +1014,13 @@ void forward(pb_nn* nn, Acts* a, hipStream_t s) {
   const bool tr = a->train;
   auto P = [&](bf16* p) { return tr ? p : nullptr; };
   conv(nn, a, 0, 0, a->X0, 16, 0, a->E0a, 16, 0, P(a->pE0a), nullptr, s);
-  conv(nn, a, 1, 0, a->E0a, 16, 0, a->CAT0, 48, 32, P(a->pE0b), nullptr, s);
-  down(a, 0, a->CAT0, 48, 32, a->P1, 16, 0, 16, s);
+  // each encoder level's second convolution also writes the next level's
+  // input, its output's 2x subsampling (P1, P2, P3)
+  conv(nn, a, 1, 0, a->E0a, 16, 0, a->CAT0, 48, 32, P(a->pE0b), nullptr, s, a->P1, 16);
   conv(nn, a, 2, 1, a->P1, 16, 0, a->E1a, 32, 0, P(a->pE1a), nullptr, s);
-  conv(nn, a, 3, 1, a->E1a, 32, 0, a->CAT1, 96, 64, P(a->pE1b), nullptr, s);
-  down(a, 1, a->CAT1, 96, 64, a->P2, 32, 0, 32, s);
+  conv(nn, a, 3, 1, a->E1a, 32, 0, a->CAT1, 96, 64, P(a->pE1b), nullptr, s, a->P2, 32);
   conv(nn, a, 4, 2, a->P2, 32, 0, a->E2a, 64, 0, P(a->pE2a), nullptr, s);
-  conv(nn, a, 5, 2, a->E2a, 64, 0, a->CAT2, 128, 64, P(a->pE2b), nullptr, s);
-  down(a, 2, a->CAT2, 128, 64, a->P3, 64, 0, 64, s);
+  conv(nn, a, 5, 2, a->E2a, 64, 0, a->CAT2, 128, 64, P(a->pE2b), nullptr, s, a->P3, 64);
   conv(nn, a, 6, 3, a->P3, 64, 0, a->B3a, 64, 0, P(a->pB3a), nullptr, s);
   conv(nn, a, 7, 3, a->B3a, 64, 0, a->B3b, 64, 0, P(a->pB3b), nullptr, s);
   up(a, 2, a->B3b, 64, 0, a->CAT2, 128, 0, 64, s);
