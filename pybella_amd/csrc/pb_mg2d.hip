@@ -299,6 +299,15 @@ __global__ void k0_init(long n, const double* v, const float* id, float omega, f
 }
 
 // full weighting of r (level 0, Krylov layout) and of c * r onto level 1
+// A periodic y axis: the Krylov layout's duplicated end row is node row 0;
+// the level-0 sweeps leave it holding their own value of that node, which
+// the cycle's result replaces by row 0's (a periodic x axis's duplicated
+// column already agrees with column 0 to the last bits)
+__global__ void k0_dup_row(int nxd, int nyd, float* z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nxd) z[(long)(nyd - 1) * nxd + i] = z[i];
+}
+
 __global__ void k0_restrict2(Dims u, int nxd, Dims c, const float* r, float dgain, float* vc) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y;
   if (I >= c.nx || J >= c.ny) return;
@@ -1130,6 +1139,7 @@ int cycle(pb_op2d* op, pb_nn* nn, const double* v, float* z, const CycleOpts& o,
     alt = cur;
     cur = out;
   }
+  if (u.py) hipLaunchKernelGGL(k0_dup_row, dim3(ceil_div(nxd, 256)), dim3(256), 0, s, nxd, op->g.ny, z);
   PB_LAUNCH_CHECK(op->ctx);
   return PB_OK;
 }

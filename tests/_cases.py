@@ -30,6 +30,9 @@ def make_case(kind, nx, ny):
         case = W.random_momentum_state(nx, ny, seed=4, compressible=True)
         case[4].bdry_type[0] = W.BdryType.WALL
         return case
+    if kind == "yper":   # the travelling vortex: 65 x 65 nodes, periodic on both axes (nx, ny ignored)
+        Sol, flux, mpv, elem, node, ud, th = W.travelling_vortex_init(opj.euler_backward_non_advective_impl_part)
+        return Sol, mpv, elem, node, ud, th
     raise ValueError(kind)
 
 

@@ -3,7 +3,8 @@
   * the hierarchy: level-0 stencil (expand9) and every Galerkin level R A P
     against oracle/mg.py in FP64, relative <= 1e-12, on the stratified
     pseudo-incompressible operator, a compressible one with Coriolis cross
-    terms and one with walls on both axes;
+    terms, one with walls on both axes and the doubly periodic travelling
+    vortex;
   * one application z = M v of the FP32 cycle against the FP64 oracle cycle
     with the same coarse solve (Jacobi sweeps on the network's level, or the
     cycle continued to the coarsest level), with one and two branches and 1
@@ -49,7 +50,7 @@ def case(kind, nx, ny):
 
 
 @pytest.mark.parametrize("kind,nx,ny", [("strat", 257, 129), ("coriolis", 129, 65), ("xwall", 65, 33),
-                                        ("strat", 1025, 513)])
+                                        ("yper", 65, 65), ("strat", 1025, 513)])
 def test_hierarchy_stencils(kind, nx, ny):
     print(f"case hierarchy {kind} {nx}x{ny}", flush=True)
     st = case(kind, nx, ny)
@@ -84,7 +85,8 @@ def _oracle_cycle(L, nnl, v_dup, coarse, nu, omega, branches):
 
 @pytest.mark.parametrize("kind,nx,ny,coarse,nu,branches", [
     ("strat", 257, 129, "smooth", 2, 2), ("strat", 257, 129, "mg", 2, 2), ("strat", 257, 129, "smooth", 1, 1),
-    ("coriolis", 129, 65, "smooth", 2, 2), ("xwall", 65, 33, "mg", 2, 2), ("strat", 1025, 513, "smooth", 2, 2)])
+    ("coriolis", 129, 65, "smooth", 2, 2), ("xwall", 65, 33, "mg", 2, 2), ("yper", 65, 65, "mg", 2, 2),
+    ("yper", 65, 65, "smooth", 2, 2), ("strat", 1025, 513, "smooth", 2, 2)])
 def test_cycle_matches_oracle(kind, nx, ny, coarse, nu, branches):
     import torch
     print(f"case cycle {kind} {nx}x{ny} {coarse} nu={nu} branches={branches}", flush=True)
@@ -99,6 +101,8 @@ def test_cycle_matches_oracle(kind, nx, ny, coarse, nu, branches):
     v = rng.standard_normal((ny, nx))
     if not op_o["x_wall"]:
         v[:, -1] = v[:, 0]
+    if not op_o["y_wall"]:
+        v[-1, :] = v[0, :]
     vd = torch.from_numpy(v).cuda()
     z = op.precond(vd, coarse=coarse, nu=nu, omega=0.8, branches=branches)
     torch.cuda.synchronize()
